@@ -1,0 +1,220 @@
+/*
+ * rtcore -- MI355X-native LBVH build + trace core (C ABI).
+ *
+ * One context = one HIP device + one HIP stream (default priority).  All
+ * calls on a context are ordered on that stream; a context is not
+ * thread-safe (the caller serialises), as the reference's single D3D
+ * immediate context is not.  Every entry point returns RT_OK (0) or a
+ * negative rt_status; rt_last_error() then holds a message.  No entry point
+ * aborts the process.
+ *
+ * Byte layouts are the reference's own (RayTracerCS/Shaders/Structures.hlsl),
+ * so facades can hand buffers across without repacking.
+ */
+#ifndef RTCORE_H
+#define RTCORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rt_status {
+    RT_OK = 0,
+    RT_ERR_ARG = -1,      /* bad argument (null pointer, size, unknown scene) */
+    RT_ERR_HIP = -2,      /* a HIP runtime call failed */
+    RT_ERR_NOMEM = -3,    /* device allocation failed */
+    RT_ERR_STATE = -4,    /* call out of order (e.g. trace before build) */
+    RT_ERR_CAPACITY = -5, /* input exceeds what the scene was created for */
+    RT_ERR_BUDGET = -6,   /* the device-memory budget cannot hold the request */
+    RT_ERR_NODEV = -7     /* no usable HIP device */
+} rt_status;
+
+/* Node / LBVHNode: Structures.hlsl:18-26, LBVH.h:25-28 (32 bytes).
+ * prim_count = (count << 3) | leaf(4) | axis(0..2); prim_pos = first index
+ * into the sorted primitive arrays.  Heap layout: node 0 unused, root 1,
+ * children of n at 2n and 2n+1. */
+typedef struct rt_node32 {
+    float mn[3];
+    float mx[3];
+    int32_t prim_count;
+    int32_t prim_pos;
+} rt_node32;
+
+/* Ray: Structures.hlsl:28-35 (48 bytes).  tri = triangle id to skip (self),
+ * -1 for none. */
+typedef struct rt_ray48 {
+    float o[3];
+    float d[3];
+    float refl[3];
+    float tmin, tmax;
+    int32_t tri;
+} rt_ray48;
+
+/* Intersection: Structures.hlsl:44-50 (24 bytes).  visited counts +1 per box
+ * test and +65536 per triangle test (LBVH.hlsl:389,404); root is the heap
+ * index of the leaf holding the hit (1 when there is none). */
+typedef struct rt_hit24 {
+    int32_t tri;
+    float u, v, t;
+    int32_t visited;
+    int32_t root;
+} rt_hit24;
+
+/* Device time of the build stages of the last rt_lbvh_build, milliseconds. */
+typedef struct rt_build_times {
+    float bounds_ms; /* per-primitive boxes + scene-box partials */
+    float morton_ms; /* scene box, 30-bit codes, digit histograms */
+    float sort_ms;   /* onesweep radix sort of (code, id) */
+    float emit_ms;   /* heap emission, one pass per tree level */
+    float refit_ms;  /* bottom-up boxes + triangle relayout */
+    float total_ms;  /* first launch to last, on the device */
+} rt_build_times;
+
+/* Pinhole camera: unit basis and half-angle tangents (see rt_camera_look_at).
+ * Pixel (px, py) of a W x H image: x = ((2px+1) - W)/W, y = -((2py+1) - H)/H
+ * (CSPrimaryRays.hlsl:37-40), d = normalize((fwd + x*tan_x*right) + y*tan_y*up). */
+typedef struct rt_camera {
+    float pos[3];
+    float fwd[3];
+    float right[3];
+    float up[3];
+    float tan_x, tan_y;
+} rt_camera;
+
+/* Direct-lighting frame: primary rays, one shadow ray per light per hit
+ * (CSComputeColorRT.hlsl:126-177), flat normals, a white Lambert material
+ * of alpha 1: color = (ambient + sum_lights max(0, -n.L)) * albedo; misses
+ * return env. */
+typedef struct rt_render_desc {
+    rt_camera cam;
+    uint32_t width, height;
+    const float *lights; /* host array, 3 floats per point light */
+    uint32_t nlights;    /* 0..8 */
+    float ambient, albedo, env;
+} rt_render_desc;
+
+/* Counters of the last frame / trace call. */
+typedef struct rt_frame_stats {
+    float trace_ms;         /* device time of the trace kernel(s) */
+    uint64_t primary_rays;
+    uint64_t shadow_rays;
+    uint64_t bounce_rays;
+    uint64_t box_tests;     /* filled only with RT_FLAG_COUNT */
+    uint64_t tri_tests;     /* filled only with RT_FLAG_COUNT */
+} rt_frame_stats;
+
+enum {
+    RT_FLAG_ANYHIT = 1u << 0,    /* stop at the first accepted triangle */
+    RT_FLAG_COUNT = 1u << 1,     /* count box / triangle tests into stats */
+    RT_FLAG_KEEP_HITS = 1u << 2, /* keep per-pixel primary hits + shadow results */
+    RT_FLAG_TIMING = 1u << 3     /* time the call with HIP events (synchronises) */
+};
+
+typedef struct rt_ctx rt_ctx;
+
+/* ---- context (DeviceCore role: Device::Initialize, DeviceCore/Device.hpp:68-87) */
+int rt_create(rt_ctx **out, int device);
+void rt_destroy(rt_ctx *ctx);
+const char *rt_last_error(const rt_ctx *ctx);
+int rt_sync(rt_ctx *ctx);
+int rt_get_stream(rt_ctx *ctx, void **hip_stream);
+/* Device bytes held by the context (all scenes and frame buffers). */
+int rt_memory_in_use(rt_ctx *ctx, uint64_t *bytes);
+
+/* ---- scenes (Model + its primitive/vertex buffers, Common/Scenes/Model.cpp:163-205;
+ *      buffer creation RayTracerCS.cpp:343-462) */
+/* Reserve device storage for up to max_tris triangles over max_verts vertices
+ * and a heap of 1 << depth nodes (conf.ini lbvh_depth; 2 <= depth <= 26). */
+int rt_scene_create(rt_ctx *ctx, uint32_t max_tris, uint32_t max_verts,
+                    uint32_t depth, int *scene);
+int rt_scene_destroy(rt_ctx *ctx, int scene);
+/* Copy vertices (3 floats each) and triangle indices (3 uint32 each; the
+ * reference's DWORD, which is 8 bytes on LP64 Linux, is not used). */
+int rt_scene_upload(rt_ctx *ctx, int scene, const float *xyz, uint32_t nv,
+                    const uint32_t *idx, uint32_t nt);
+/* Same, from device pointers (stream-ordered device-to-device copy). */
+int rt_scene_upload_device(rt_ctx *ctx, int scene, const float *d_xyz,
+                           uint32_t nv, const uint32_t *d_idx, uint32_t nt);
+/* Rigid motion of the uploaded (rest) vertices: v = R (v0 - c) + c, with
+ * R row-major in m[0..8] and c in m[9..11]; float arithmetic in the order
+ * ((R0 dx + R1 dy) + R2 dz) + cx, no FMA. */
+int rt_scene_transform(rt_ctx *ctx, int scene, const float m[12]);
+int rt_scene_info(rt_ctx *ctx, int scene, uint32_t *nt, uint32_t *nv,
+                  uint32_t *depth);
+
+/* ---- build (StructureConstruction::GPUBuildStructure, StructureConstruction.cpp:86-131) */
+/* Rebuilds the LBVH of the scene's current vertices on the context stream.
+ * times != NULL synchronises and fills the per-stage device times. */
+int rt_lbvh_build(rt_ctx *ctx, int scene, rt_build_times *times);
+/* Copies build results to the host; any pointer may be NULL.
+ * nodes: 1 << depth entries (AccelerationStructure::GetPtrToElements);
+ * sorted_codes / sorted_ids: nt entries (GetOrderedElementsIds);
+ * scene_box: {min xyz, max xyz} used for the Morton quantisation;
+ * Unreachable heap slots hold unspecified values, as in the CPU builder
+ * (LBVH.cpp:577); children of leaves hold -2147483647 in both ints. */
+int rt_lbvh_read(rt_ctx *ctx, int scene, rt_node32 *nodes,
+                 uint32_t *sorted_codes, int32_t *sorted_ids, float scene_box[6]);
+
+/* ---- trace (CSComputeIntersections -> LBVH_IntersectP, CSIntersections.hlsl:18-39,
+ *      LBVH.hlsl:363-440).  Closest hit from the root, strict '<' in the
+ *      reference visit order; rays with tri <= -2 are dead (hit.tri = -2). */
+int rt_trace(rt_ctx *ctx, int scene, const rt_ray48 *rays, uint32_t n,
+             rt_hit24 *hits, uint32_t flags, rt_frame_stats *stats);
+/* Same with device pointers; asynchronous unless RT_FLAG_TIMING/COUNT. */
+int rt_trace_device(rt_ctx *ctx, int scene, const rt_ray48 *d_rays, uint32_t n,
+                    rt_hit24 *d_hits, uint32_t flags, rt_frame_stats *stats);
+
+/* ---- frame (RayTracerCS::Render, RayTracerCS.cpp:241-311, without rebuild) */
+/* Primary + shadow rays + direct shading into the context's frame buffer. */
+int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc,
+                     uint32_t flags, rt_frame_stats *stats);
+/* Copy the last frame to the host; any pointer may be NULL.  rgba: 4 floats
+ * per pixel; hits: primary hits (needs RT_FLAG_KEEP_HITS); occluded: one byte
+ * per pixel per light, 0 lit, 1 occluded, 2 no shadow ray (KEEP_HITS). */
+int rt_frame_read(rt_ctx *ctx, float *rgba, rt_hit24 *hits, uint8_t *occluded);
+
+/* ---- profiling window (Timer role, DeviceCore/Timer.hpp:27-66; Performance,
+ *      Common/Performance.cpp:52-101).  Between begin and end every build,
+ *      trace and render call records HIP events around its kernels on the
+ *      context stream, without synchronising, and the device counters
+ *      (shadow / bounce rays always, box / triangle tests with RT_FLAG_COUNT)
+ *      accumulate.  end synchronises and sums.  Per-call stats arguments
+ *      reset the counters, so do not pass them inside a window. */
+typedef struct rt_profile {
+    uint32_t builds;     /* rt_lbvh_build calls in the window */
+    float build_ms[6];   /* summed stage times, as rt_build_times */
+    uint32_t launches;   /* trace / render kernel launches in the window */
+    float trace_ms;      /* summed device time of those launches */
+    uint64_t primary_rays, shadow_rays, bounce_rays, box_tests, tri_tests;
+} rt_profile;
+int rt_profile_begin(rt_ctx *ctx);
+int rt_profile_end(rt_ctx *ctx, rt_profile *out);
+
+/* ---- camera helper (Camera::SetCamera role, Common/Camera.cpp:154-211) */
+/* Fills a pinhole camera at pos looking at target, vertical field of view
+ * fov_y_deg, aspect = width / height (basis computed in double, rounded). */
+void rt_camera_look_at(rt_camera *cam, const float pos[3], const float target[3],
+                       const float up[3], float fov_y_deg, float aspect);
+
+/* ---- seeded synthetic scenes (host only, no device work) */
+/* Each generator writes vertices (3 floats each) and indices (3 uint32 per
+ * triangle); call with NULL outputs to get the sizes in *nv / *nt. */
+int rt_gen_torus(float R, float r, uint32_t nu, uint32_t nv_ring, float jitter,
+                 uint64_t seed, float *xyz, uint32_t *nv, uint32_t *idx,
+                 uint32_t *nt);
+int rt_gen_triangle_soup(uint32_t ntri, float sigma, uint64_t seed, float *xyz,
+                         uint32_t *nv, uint32_t *idx, uint32_t *nt);
+int rt_gen_noisy_sphere(uint32_t nlat, uint32_t nlon, float amplitude,
+                        uint64_t seed, float *xyz, uint32_t *nv, uint32_t *idx,
+                        uint32_t *nt);
+int rt_gen_object_mix(uint32_t target_tris, uint64_t seed, float *xyz,
+                      uint32_t *nv, uint32_t *idx, uint32_t *nt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTCORE_H */

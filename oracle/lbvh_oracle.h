@@ -1,0 +1,117 @@
+/*
+ * CPU oracle for the LBVH build + trace path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this code; only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py use it, and only as the checker / the CPU baseline.
+ *
+ * It restates, scalar and in source order, the reference's own CPU code and
+ * HLSL kernels for this path (citations are into elfrank/raytracer-gpupro4):
+ *   - orc_prim_bounds / orc_morton_codes : AccelerationStructures/LBVH/LBVH.cpp:495-553
+ *                                          (same arithmetic as LBVH.hlsl:26-114)
+ *   - orc_lbvh_build                     : LBVH.cpp:190-314 (build_recursive, in-place
+ *                                          MSD partition) + LBVH.cpp:577-629
+ *   - orc_sort_pairs                     : the (code, id) order of cmpMorton, LBVH.cpp:21-38
+ *   - orc_trace                          : LBVH.hlsl:363-440 (stackless trail traversal)
+ *                                          + Shaders/Core/Intersection.hlsl:18-78
+ *   - orc_raygen_pinhole                 : pixel-centre rule of CSPrimaryRays.hlsl:37-40
+ *   - orc_render_direct                  : shadow-ray construction + direct term of
+ *                                          CSComputeColorRT.hlsl:54-177
+ *
+ * Floating point: every float expression is evaluated in IEEE single
+ * precision, in the order written, with no contraction into FMA (the build
+ * recipe passes -ffp-contract=off and never -ffast-math).
+ *
+ * How each function is pinned is listed in tests/test_oracle.py.
+ */
+#ifndef LBVH_ORACLE_H
+#define LBVH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* LBVHNode (LBVH.h:25-28) == HLSL Node (Structures.hlsl:18-26), 32 bytes. */
+typedef struct {
+    float mn[3];
+    float mx[3];
+    int32_t prim_count; /* (count << 3) | leaf(4) | axis(0..2) */
+    int32_t prim_pos;
+} orc_node;
+
+/* HLSL Ray (Structures.hlsl:28-35), 48 bytes. */
+typedef struct {
+    float o[3];
+    float d[3];
+    float refl[3];
+    float tmin, tmax;
+    int32_t tri; /* triangle to skip (self id); -1 = none */
+} orc_ray;
+
+/* HLSL Intersection (Structures.hlsl:44-50), 24 bytes. */
+typedef struct {
+    int32_t tri;
+    float u, v, t;
+    int32_t visited; /* +1 per box test, +65536 per triangle test (LBVH.hlsl:389,404) */
+    int32_t root;    /* heap index of the leaf that holds the hit; 1 when no hit */
+} orc_hit;
+
+/* Marker written into heap slots the builder never reaches. */
+#define ORC_INVALID (-2147483647)
+
+/* Per-triangle AABB, 3x centroid and scene box (LBVH.cpp:495-516). */
+void orc_prim_bounds(const float *xyz, const uint32_t *idx, uint32_t nt,
+                     float *bmin, float *bmax, float *cent, float scene[6]);
+
+/* 30-bit Morton codes of the 3x centroids (LBVH.cpp:520-553). */
+void orc_morton_codes(const float *cent, uint32_t nt, const float scene[6],
+                      uint32_t *codes);
+
+/* Closed-form interleave of Part1By2 (MortonCode.hlsl:15-27) applied to
+ * q & 1023; used only as an independent cross-check of the loop form. */
+uint32_t orc_morton_closed_form(uint32_t qx, uint32_t qy, uint32_t qz);
+
+/* build_recursive(0, nt, 1<<30, 1, 0) over (codes, ids), permuting them in
+ * place exactly as the reference's in-place partition does.  nodes must hold
+ * 1 << depth entries; unreachable slots get ORC_INVALID in both int fields and
+ * zero boxes.  Returns the number of reachable nodes. */
+int orc_lbvh_build(uint32_t *codes, int32_t *ids, uint32_t nt, uint32_t depth,
+                   const float *bmin, const float *bmax, orc_node *nodes);
+
+/* Sort (code, id) pairs ascending by code, then id (cmpMorton order). */
+void orc_sort_pairs(uint32_t *codes, int32_t *ids, uint32_t n);
+
+/* LBVH_IntersectP for each ray.  start[i] is the heap node to start from
+ * (NULL = root 1).  anyhit != 0 stops at the first accepted triangle (same
+ * visit order).  nthreads > 1 splits rays statically over pthreads. */
+void orc_trace(const orc_node *nodes, const int32_t *ids, const float *xyz,
+               const uint32_t *idx, const orc_ray *rays, const int32_t *start,
+               uint32_t n, int anyhit, orc_hit *hits, int nthreads);
+
+/* Pinhole camera.  cam = {pos[3], fwd[3], right[3], up[3], tan_x, tan_y}
+ * (unit basis vectors and the half-angle tangents, already rounded to float).
+ * Pixel (px, py): sx = ((2px+1) - W)/W * tan_x, sy = -((2py+1) - H)/H * tan_y,
+ * d = normalize((fwd + sx*right) + sy*up). */
+void orc_raygen_pinhole(const float cam[14], uint32_t w, uint32_t h, orc_ray *rays);
+
+/* Direct lighting of one primary-hit buffer: per pixel, the flat normal,
+ * and for each light a shadow ray {o = hit, d = -Ldir, maxT = |hit - L|,
+ * self = hit triangle} traced with closest-hit from the primary hit's leaf
+ * (reference semantics, CSComputeColorRT.hlsl:126-160).  shade = {ambient,
+ * albedo, env} with a white material of alpha 1 (no specular term).
+ * rgba: 4 floats per pixel.  occluded (optional): one byte per pixel per
+ * light: 0 = lit, 1 = occluded, 2 = no shadow ray cast. */
+void orc_render_direct(const orc_node *nodes, const int32_t *ids,
+                       const float *xyz, const uint32_t *idx,
+                       const orc_ray *rays, const orc_hit *hits, uint32_t n,
+                       const float *lights, uint32_t nlights,
+                       const float shade[3], float *rgba, uint8_t *occluded,
+                       int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

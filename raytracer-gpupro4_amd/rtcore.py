@@ -1,0 +1,348 @@
+"""ctypes binding of librtcore.so (the C ABI declared in include/rtcore.h).
+
+This module is plumbing for tests, the bench and Python callers: it moves
+numpy arrays across the C ABI and turns status codes into exceptions.  All
+computation happens in the HIP kernels of librtcore.so; there is no Python or
+CPU fallback, and importing fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librtcore.so")
+
+RT_OK = 0
+RT_ERR_ARG = -1
+RT_ERR_HIP = -2
+RT_ERR_NOMEM = -3
+RT_ERR_STATE = -4
+RT_ERR_CAPACITY = -5
+RT_ERR_BUDGET = -6
+RT_ERR_NODEV = -7
+
+FLAG_ANYHIT = 1 << 0
+FLAG_COUNT = 1 << 1
+FLAG_KEEP_HITS = 1 << 2
+FLAG_TIMING = 1 << 3
+
+# Byte layouts of RayTracerCS/Shaders/Structures.hlsl (Node 32 B, Ray 48 B,
+# Intersection 24 B) as numpy dtypes.
+NODE_DTYPE = np.dtype([("mn", "<f4", 3), ("mx", "<f4", 3), ("prim_count", "<i4"), ("prim_pos", "<i4")])
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("refl", "<f4", 3), ("tmin", "<f4"),
+                      ("tmax", "<f4"), ("tri", "<i4")])
+HIT_DTYPE = np.dtype([("tri", "<i4"), ("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("visited", "<i4"),
+                      ("root", "<i4")])
+assert NODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 48 and HIT_DTYPE.itemsize == 24
+
+
+class BuildTimes(C.Structure):
+    _fields_ = [("bounds_ms", C.c_float), ("morton_ms", C.c_float), ("sort_ms", C.c_float),
+                ("emit_ms", C.c_float), ("refit_ms", C.c_float), ("total_ms", C.c_float)]
+
+    def as_dict(self):
+        return {k: float(getattr(self, k)) for k, _ in self._fields_}
+
+
+class Profile(C.Structure):
+    _fields_ = [("builds", C.c_uint32), ("build_ms", C.c_float * 6), ("launches", C.c_uint32),
+                ("trace_ms", C.c_float), ("primary_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("bounce_rays", C.c_uint64), ("box_tests", C.c_uint64), ("tri_tests", C.c_uint64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["build_ms"] = dict(zip(("bounds", "morton", "sort", "emit", "refit", "total"),
+                                 [float(x) for x in self.build_ms]))
+        d["trace_ms"] = float(self.trace_ms)
+        return d
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("tan_x", C.c_float), ("tan_y", C.c_float)]
+
+    def as_array(self) -> np.ndarray:
+        """The 14 floats {pos, fwd, right, up, tan_x, tan_y}."""
+        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+
+
+class RenderDesc(C.Structure):
+    _fields_ = [("cam", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("lights", C.POINTER(C.c_float)), ("nlights", C.c_uint32), ("ambient", C.c_float),
+                ("albedo", C.c_float), ("env", C.c_float)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("trace_ms", C.c_float), ("primary_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("bounce_rays", C.c_uint64), ("box_tests", C.c_uint64), ("tri_tests", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: (float(getattr(self, k)) if k == "trace_ms" else int(getattr(self, k)))
+                for k, _ in self._fields_}
+
+
+def build_library(force: bool = False) -> None:
+    """Compiles librtcore.so in place with the package Makefile (hipcc, gfx950)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-j8"], check=True)
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"librtcore.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P, U32, I32, U64 = C.c_void_p, C.c_uint32, C.c_int, C.c_uint64
+    sig = {
+        "rt_create": (I32, [C.POINTER(P), I32]),
+        "rt_destroy": (None, [P]),
+        "rt_last_error": (C.c_char_p, [P]),
+        "rt_sync": (I32, [P]),
+        "rt_get_stream": (I32, [P, C.POINTER(P)]),
+        "rt_memory_in_use": (I32, [P, C.POINTER(U64)]),
+        "rt_scene_create": (I32, [P, U32, U32, U32, C.POINTER(I32)]),
+        "rt_scene_destroy": (I32, [P, I32]),
+        "rt_scene_upload": (I32, [P, I32, P, U32, P, U32]),
+        "rt_scene_upload_device": (I32, [P, I32, P, U32, P, U32]),
+        "rt_scene_transform": (I32, [P, I32, P]),
+        "rt_scene_info": (I32, [P, I32, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)]),
+        "rt_lbvh_build": (I32, [P, I32, C.POINTER(BuildTimes)]),
+        "rt_lbvh_read": (I32, [P, I32, P, P, P, P]),
+        "rt_trace": (I32, [P, I32, P, U32, P, U32, C.POINTER(FrameStats)]),
+        "rt_trace_device": (I32, [P, I32, P, U32, P, U32, C.POINTER(FrameStats)]),
+        "rt_render_direct": (I32, [P, I32, C.POINTER(RenderDesc), U32, C.POINTER(FrameStats)]),
+        "rt_frame_read": (I32, [P, P, P, P]),
+        "rt_profile_begin": (I32, [P]),
+        "rt_profile_end": (I32, [P, C.POINTER(Profile)]),
+        "rt_camera_look_at": (None, [C.POINTER(Camera), P, P, P, C.c_float, C.c_float]),
+        "rt_gen_torus": (I32, [C.c_float, C.c_float, U32, U32, C.c_float, U64, P, C.POINTER(U32), P,
+                               C.POINTER(U32)]),
+        "rt_gen_triangle_soup": (I32, [U32, C.c_float, U64, P, C.POINTER(U32), P, C.POINTER(U32)]),
+        "rt_gen_noisy_sphere": (I32, [U32, U32, C.c_float, U64, P, C.POINTER(U32), P, C.POINTER(U32)]),
+        "rt_gen_object_mix": (I32, [U32, U64, P, C.POINTER(U32), P, C.POINTER(U32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class RtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+# ---------------------------------------------------------------- generators
+def _gen(fn, *args):
+    L = lib()
+    nv, nt = C.c_uint32(0), C.c_uint32(0)
+    rc = getattr(L, fn)(*args, None, C.byref(nv), None, C.byref(nt))
+    if rc != RT_OK:
+        raise RtError(rc, fn)
+    xyz = np.empty((nv.value, 3), np.float32)
+    idx = np.empty((nt.value, 3), np.uint32)
+    rc = getattr(L, fn)(*args, _ptr(xyz), C.byref(nv), _ptr(idx), C.byref(nt))
+    if rc != RT_OK:
+        raise RtError(rc, fn)
+    return xyz, idx
+
+
+def gen_torus(R=1.0, r=0.3, nu=128, nv=64, jitter=1e-3, seed=1):
+    return _gen("rt_gen_torus", R, r, nu, nv, jitter, seed)
+
+
+def gen_triangle_soup(ntri=262144, sigma=0.004, seed=2):
+    return _gen("rt_gen_triangle_soup", ntri, sigma, seed)
+
+
+def gen_noisy_sphere(nlat=1025, nlon=1024, amplitude=0.05, seed=3):
+    return _gen("rt_gen_noisy_sphere", nlat, nlon, amplitude, seed)
+
+
+def gen_object_mix(target_tris, seed=4):
+    return _gen("rt_gen_object_mix", target_tris, seed)
+
+
+def camera_look_at(pos, target, up=(0.0, 1.0, 0.0), fov_y_deg=60.0, aspect=1.0) -> Camera:
+    cam = Camera()
+    p = np.asarray(pos, np.float32)
+    t = np.asarray(target, np.float32)
+    u = np.asarray(up, np.float32)
+    lib().rt_camera_look_at(C.byref(cam), _ptr(p), _ptr(t), _ptr(u), fov_y_deg, aspect)
+    return cam
+
+
+def rotation_about(axis, degrees, center) -> np.ndarray:
+    """12 floats {R row-major, c} for rt_scene_transform (rotation about an
+    axis through `center`), computed in float64 and rounded once."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    th = np.deg2rad(degrees)
+    x, y, z = a
+    c, s = np.cos(th), np.sin(th)
+    R = np.array([[c + x * x * (1 - c), x * y * (1 - c) - z * s, x * z * (1 - c) + y * s],
+                  [y * x * (1 - c) + z * s, c + y * y * (1 - c), y * z * (1 - c) - x * s],
+                  [z * x * (1 - c) - y * s, z * y * (1 - c) + x * s, c + z * z * (1 - c)]])
+    return np.concatenate([R.reshape(-1), np.asarray(center, np.float64)]).astype(np.float32)
+
+
+# ------------------------------------------------------------------- context
+class Context:
+    """One HIP device + stream (rt_create)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.rt_create(C.byref(h), device)
+        if rc != RT_OK:
+            raise RtError(rc, "rt_create")
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if self.h:
+            self.L.rt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int, what: str):
+        if rc != RT_OK:
+            raise RtError(rc, f"{what}: {self.L.rt_last_error(self.h).decode()}")
+
+    def sync(self):
+        self.check(self.L.rt_sync(self.h), "rt_sync")
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        self.check(self.L.rt_get_stream(self.h, C.byref(s)), "rt_get_stream")
+        return s.value or 0
+
+    def memory_in_use(self) -> int:
+        b = C.c_uint64()
+        self.check(self.L.rt_memory_in_use(self.h, C.byref(b)), "rt_memory_in_use")
+        return b.value
+
+    def profile_begin(self):
+        self.check(self.L.rt_profile_begin(self.h), "rt_profile_begin")
+
+    def profile_end(self) -> dict:
+        p = Profile()
+        self.check(self.L.rt_profile_end(self.h, C.byref(p)), "rt_profile_end")
+        return p.as_dict()
+
+    # scenes
+    def scene_create(self, max_tris: int, max_verts: int, depth: int) -> int:
+        s = C.c_int()
+        self.check(self.L.rt_scene_create(self.h, max_tris, max_verts, depth, C.byref(s)),
+                   "rt_scene_create")
+        return s.value
+
+    def scene_destroy(self, scene: int):
+        self.check(self.L.rt_scene_destroy(self.h, scene), "rt_scene_destroy")
+
+    def scene_upload(self, scene: int, xyz: np.ndarray, idx: np.ndarray):
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        self.check(self.L.rt_scene_upload(self.h, scene, _ptr(xyz), xyz.size // 3, _ptr(idx),
+                                          idx.size // 3), "rt_scene_upload")
+
+    def scene_upload_device(self, scene: int, d_xyz: int, nv: int, d_idx: int, nt: int):
+        self.check(self.L.rt_scene_upload_device(self.h, scene, C.c_void_p(d_xyz), nv,
+                                                 C.c_void_p(d_idx), nt), "rt_scene_upload_device")
+
+    def scene_transform(self, scene: int, m12: np.ndarray):
+        m = np.ascontiguousarray(m12, np.float32)
+        assert m.size == 12
+        self.check(self.L.rt_scene_transform(self.h, scene, _ptr(m)), "rt_scene_transform")
+
+    def scene_info(self, scene: int):
+        nt, nv, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self.check(self.L.rt_scene_info(self.h, scene, C.byref(nt), C.byref(nv), C.byref(d)),
+                   "rt_scene_info")
+        return nt.value, nv.value, d.value
+
+    def add_scene(self, xyz: np.ndarray, idx: np.ndarray, depth: int) -> int:
+        s = self.scene_create(len(idx), len(xyz), depth)
+        self.scene_upload(s, xyz, idx)
+        return s
+
+    # build
+    def build(self, scene: int, timed: bool = False):
+        t = BuildTimes()
+        self.check(self.L.rt_lbvh_build(self.h, scene, C.byref(t) if timed else None), "rt_lbvh_build")
+        return t.as_dict() if timed else None
+
+    def read(self, scene: int):
+        nt, _, depth = self.scene_info(scene)
+        nodes = np.empty(1 << depth, NODE_DTYPE)
+        codes = np.empty(nt, np.uint32)
+        ids = np.empty(nt, np.int32)
+        box = np.empty(6, np.float32)
+        self.check(self.L.rt_lbvh_read(self.h, scene, _ptr(nodes), _ptr(codes), _ptr(ids), _ptr(box)),
+                   "rt_lbvh_read")
+        return nodes, codes, ids, box
+
+    # trace
+    def trace(self, scene: int, rays: np.ndarray, flags: int = 0, want_stats: bool = False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.empty(len(rays), HIT_DTYPE)
+        st = FrameStats()
+        self.check(self.L.rt_trace(self.h, scene, _ptr(rays), len(rays), _ptr(hits), flags,
+                                   C.byref(st) if want_stats else None), "rt_trace")
+        return (hits, st.as_dict()) if want_stats else hits
+
+    def trace_device(self, scene: int, d_rays: int, n: int, d_hits: int, flags: int = 0,
+                     want_stats: bool = False):
+        st = FrameStats()
+        self.check(self.L.rt_trace_device(self.h, scene, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
+                                          flags, C.byref(st) if want_stats else None),
+                   "rt_trace_device")
+        return st.as_dict() if want_stats else None
+
+    def render_direct(self, scene: int, cam: Camera, width: int, height: int, lights,
+                      ambient=0.1, albedo=0.8, env=0.0, flags: int = 0, want_stats: bool = True):
+        lights = np.ascontiguousarray(np.asarray(lights, np.float32).reshape(-1))
+        d = RenderDesc()
+        d.cam = cam
+        d.width, d.height = width, height
+        d.lights = lights.ctypes.data_as(C.POINTER(C.c_float))
+        d.nlights = lights.size // 3
+        d.ambient, d.albedo, d.env = ambient, albedo, env
+        st = FrameStats()
+        self.check(self.L.rt_render_direct(self.h, scene, C.byref(d), flags,
+                                           C.byref(st) if want_stats else None), "rt_render_direct")
+        return st.as_dict() if want_stats else None
+
+    def frame_read(self, width: int, height: int, nlights: int, hits: bool = False):
+        rgba = np.empty((height * width, 4), np.float32)
+        h = np.empty(height * width, HIT_DTYPE) if hits else None
+        occ = np.empty((height * width, max(nlights, 1)), np.uint8) if hits else None
+        self.check(self.L.rt_frame_read(self.h, _ptr(rgba), _ptr(h), _ptr(occ)), "rt_frame_read")
+        return rgba, h, occ

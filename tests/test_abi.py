@@ -1,0 +1,93 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point
+declared in include/rtcore.h, and its host-only parts (scene generators,
+camera helper, argument errors) behave."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "rtcore.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    names = declared_functions()
+    for must in ("rt_create", "rt_lbvh_build", "rt_lbvh_read", "rt_trace", "rt_render_direct",
+                 "rt_scene_upload", "rt_scene_transform", "rt_gen_torus"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(rtcore):
+    lib = rtcore.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", rtcore.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (rt_[a-z0-9_]+)$", out, flags=re.M))
+    assert set(declared_functions()) <= exported
+
+
+def test_library_has_gfx950_code_object(rtcore):
+    # the offload bundle of our own build names its target triple
+    blob = open(rtcore.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_torus_generator(rtcore):
+    xyz, idx = rtcore.gen_torus()
+    assert xyz.shape == (128 * 64, 3) and idx.shape == (16384, 3)
+    assert idx.max() < len(xyz)
+    r_major = np.hypot(xyz[:, 0], xyz[:, 2])
+    tube = np.hypot(r_major - 1.0, xyz[:, 1])
+    assert np.all(np.abs(tube - 0.3) < 0.003)
+    xyz2, idx2 = rtcore.gen_torus()
+    assert np.array_equal(xyz, xyz2) and np.array_equal(idx, idx2)
+    xyz3, _ = rtcore.gen_torus(seed=2)
+    assert not np.array_equal(xyz, xyz3)
+
+
+def test_triangle_soup_generator(rtcore):
+    xyz, idx = rtcore.gen_triangle_soup(262144, 0.004, seed=2)
+    assert idx.shape == (262144, 3) and xyz.shape == (3 * 262144, 3)
+    c = xyz.reshape(-1, 3, 3).mean(1)
+    assert np.all(c > -0.05) and np.all(c < 1.05)
+    e1 = xyz[1::3] - xyz[0::3]
+    assert abs(float(e1.std()) - 0.004) < 2e-4
+
+
+def test_sphere_and_mix_generators(rtcore):
+    xyz, idx = rtcore.gen_noisy_sphere(65, 64, 0.05, seed=3)
+    assert len(idx) == 2 * 64 * 64
+    r = np.linalg.norm(xyz, axis=1)
+    assert r.min() > 0.95 - 1e-6 and r.max() < 1.05 + 1e-6
+    xyz, idx = rtcore.gen_object_mix(5000, seed=4)
+    assert len(idx) == 5000 and idx.max() < len(xyz)
+
+
+def test_camera_look_at_basis(rtcore):
+    cam = rtcore.camera_look_at((0, 0.5, 3), (0, 0, 0), (0, 1, 0), 60.0, 16 / 9)
+    c = cam.as_array()
+    f, r, u = c[3:6], c[6:9], c[9:12]
+    for v in (f, r, u):
+        assert abs(np.linalg.norm(v) - 1) < 1e-6
+    assert abs(f @ r) < 1e-6 and abs(f @ u) < 1e-6 and abs(r @ u) < 1e-6
+    assert abs(c[13] - np.tan(np.deg2rad(30))) < 1e-6 and abs(c[12] - c[13] * 16 / 9) < 1e-6
+
+
+def test_create_without_gpu_reports_status(rtcore):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    rc = rtcore.lib().rt_create(ctypes.byref(h), 0)
+    assert rc == rtcore.RT_ERR_NODEV and not h.value
+    assert rtcore.lib().rt_last_error(None) == b"null context"

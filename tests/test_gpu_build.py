@@ -1,0 +1,164 @@
+"""GPU build parity: Morton codes, sorted (code, id) arrays, scene box, heap
+topology and boxes from the HIP build against the CPU oracle, bit for bit.
+
+The oracle is run in presorted mode (stable (code, id) order, the order the
+onesweep sort produces); its topology and boxes equal the reference's
+in-place-partition mode (tests/test_oracle.py pins that), whose per-leaf id
+multisets are checked here too.  Boxes are compared as float values (exact
+min/max; -0.0 == +0.0)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def soup(n, sigma=0.004, seed=2):
+    import rtcore
+
+    return rtcore.gen_triangle_soup(n, sigma, seed)
+
+
+def dup_grid(n, cells=6, seed=1):
+    rng = np.random.default_rng(seed)
+    c = np.floor(rng.random((n, 3)) * cells) / cells
+    xyz = np.concatenate([c, c + [0.01, 0, 0], c + [0, 0.01, 0]]).astype(np.float32)
+    idx = np.stack([np.arange(n), np.arange(n) + n, np.arange(n) + 2 * n], 1).astype(np.uint32)
+    return xyz, idx
+
+
+def check_build(ctx, oracle, xyz, idx, depth, check_partition_mode=False):
+    print(f"build parity: nt={len(idx)} depth={depth}", flush=True)
+    s = ctx.add_scene(xyz, idx, depth)
+    try:
+        ctx.build(s)
+        nodes, codes, ids, box = ctx.read(s)
+    finally:
+        ctx.scene_destroy(s)
+    ref = oracle.build_scene(xyz, idx, depth, presorted=True)
+    assert np.array_equal(box, ref["scene"])
+    assert np.array_equal(codes, ref["codes"])
+    assert np.array_equal(ids, ref["ids"])
+    if len(idx) == 0:
+        return
+    reach = oracle.reachable(ref["nodes"])
+    g, r = nodes[reach], ref["nodes"][reach]
+    assert np.array_equal(g["prim_count"], r["prim_count"])
+    assert np.array_equal(g["prim_pos"], r["prim_pos"])
+    assert np.all(g["mn"] == r["mn"]) and np.all(g["mx"] == r["mx"])
+    # children of leaves inside the heap carry the invalid marker
+    leaves = reach[(r["prim_count"] & 4) != 0]
+    inner = leaves[leaves < (1 << (depth - 1))]
+    for c in (2 * inner, 2 * inner + 1):
+        assert np.all(nodes["prim_count"][c] == oracle.INVALID)
+        assert np.all(nodes["prim_pos"][c] == oracle.INVALID)
+    if check_partition_mode:
+        cpu = oracle.build_scene(xyz, idx, depth, presorted=False)
+        assert np.array_equal(cpu["nodes"][reach]["prim_count"], g["prim_count"])
+        for h in leaves:
+            pos, cnt = int(r[reach == h]["prim_pos"][0]), int(r[reach == h]["prim_count"][0]) >> 3
+            assert sorted(ids[pos:pos + cnt]) == sorted(cpu["ids"][pos:pos + cnt])
+
+
+def test_build_torus_config1(ctx, oracle, rtcore):
+    xyz, idx = rtcore.gen_torus()
+    check_build(ctx, oracle, xyz, idx, 20, check_partition_mode=True)
+
+
+def test_build_soup_262144(ctx, oracle):
+    xyz, idx = soup(262144)
+    check_build(ctx, oracle, xyz, idx, 20)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 1000, 1023, 1024, 1025, 4097, 65537])
+def test_build_ragged_sizes(ctx, oracle, n):
+    xyz, idx = soup(n, sigma=0.01, seed=n)
+    check_build(ctx, oracle, xyz, idx, 20)
+
+
+@pytest.mark.parametrize("depth", [2, 3, 6, 10, 14])
+def test_build_depth_limit(ctx, oracle, depth):
+    xyz, idx = soup(50000, seed=depth)
+    check_build(ctx, oracle, xyz, idx, depth)
+
+
+def test_build_duplicate_codes(ctx, oracle):
+    xyz, idx = dup_grid(20000)
+    check_build(ctx, oracle, xyz, idx, 16, check_partition_mode=True)
+
+
+def test_build_all_identical_triangles(ctx, oracle):
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    xyz = np.tile(tri, (500, 1))
+    idx = np.arange(1500, dtype=np.uint32).reshape(500, 3)
+    check_build(ctx, oracle, xyz, idx, 8)
+
+
+def test_build_indexed_mesh_unreferenced_vertices(ctx, oracle, rtcore):
+    # vertices outside every triangle must not widen the scene box
+    xyz, idx = rtcore.gen_torus(nu=32, nv=16)
+    xyz = np.concatenate([xyz, [[50, 50, 50], [-50, -50, -50]]]).astype(np.float32)
+    check_build(ctx, oracle, xyz, idx, 12)
+
+
+def test_build_sphere_2m(ctx, oracle, rtcore):
+    xyz, idx = rtcore.gen_noisy_sphere(1025, 1024, 0.05, seed=3)
+    assert len(idx) == 2097152
+    check_build(ctx, oracle, xyz, idx, 22)
+
+
+def test_empty_scene(ctx):
+    s = ctx.scene_create(16, 16, 8)
+    ctx.scene_upload(s, np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    ctx.build(s)
+    import rtcore
+
+    rays = np.zeros(4, rtcore.RAY_DTYPE)
+    rays["d"] = [0, 0, 1]
+    rays["tmax"] = 5.0
+    rays["tri"] = -1
+    h = ctx.trace(s, rays)
+    assert np.all(h["tri"] == -1) and np.all(h["t"] == 5.0)
+    ctx.scene_destroy(s)
+
+
+def test_rebuild_is_deterministic_and_transform_exact(ctx, oracle, rtcore):
+    xyz, idx = soup(100000, seed=12)
+    s = ctx.add_scene(xyz, idx, 20)
+    try:
+        ctx.build(s)
+        first = ctx.read(s)
+        ctx.build(s)
+        again = ctx.read(s)
+        reach = oracle.reachable(first[0])
+        assert np.array_equal(first[0][reach], again[0][reach])
+        assert np.array_equal(first[2], again[2])
+        for frame in (1, 37):
+            m = rtcore.rotation_about((0.3, 1.0, 0.2), float(frame), (0.5, 0.5, 0.5))
+            ctx.scene_transform(s, m)
+            ctx.build(s)
+            nodes, codes, ids, box = ctx.read(s)
+            f = np.float32
+            d = (xyz - m[9:12]).astype(f)
+            R = m[:9].reshape(3, 3)
+            rot = np.stack([((((R[r, 0] * d[:, 0]).astype(f) + (R[r, 1] * d[:, 1]).astype(f)).astype(f)
+                              + (R[r, 2] * d[:, 2]).astype(f)).astype(f) + m[9 + r]).astype(f)
+                            for r in range(3)], 1)
+            ref = oracle.build_scene(rot, idx, 20)
+            assert np.array_equal(box, ref["scene"])
+            assert np.array_equal(codes, ref["codes"]) and np.array_equal(ids, ref["ids"])
+            rr = oracle.reachable(ref["nodes"])
+            assert np.array_equal(nodes[rr]["prim_count"], ref["nodes"][rr]["prim_count"])
+            assert np.all(nodes[rr]["mn"] == ref["nodes"][rr]["mn"])
+    finally:
+        ctx.scene_destroy(s)
+
+
+def test_upload_rejects_bad_indices(ctx, rtcore):
+    s = ctx.scene_create(4, 4, 6)
+    with pytest.raises(rtcore.RtError) as e:
+        ctx.scene_upload(s, np.zeros((3, 3), np.float32), np.array([[0, 1, 3]], np.uint32))
+    assert e.value.code == rtcore.RT_ERR_ARG
+    with pytest.raises(rtcore.RtError) as e:
+        ctx.scene_upload(s, np.zeros((9, 3), np.float32), np.zeros((5, 3), np.uint32))
+    assert e.value.code == rtcore.RT_ERR_CAPACITY
+    ctx.scene_destroy(s)
