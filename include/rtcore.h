@@ -68,8 +68,9 @@ typedef struct rt_build_times {
     float bounds_ms; /* per-primitive boxes + scene-box partials */
     float morton_ms; /* scene box, 30-bit codes, digit histograms */
     float sort_ms;   /* onesweep radix sort of (code, id) */
-    float emit_ms;   /* heap emission, one pass per tree level */
-    float refit_ms;  /* bottom-up boxes + triangle relayout */
+    float emit_ms;   /* split tables + emission of the nodes above 512 primitives */
+    float refit_ms;  /* in-LDS emission + refit of the small subtrees, refit of
+                        the big nodes, triangle relayout */
     float total_ms;  /* first launch to last, on the device */
 } rt_build_times;
 
@@ -130,6 +131,13 @@ int rt_memory_in_use(rt_ctx *ctx, uint64_t *bytes);
  * and a heap of 1 << depth nodes (conf.ini lbvh_depth; 2 <= depth <= 26). */
 int rt_scene_create(rt_ctx *ctx, uint32_t max_tris, uint32_t max_verts,
                     uint32_t depth, int *scene);
+/* A batch: nscenes scenes in one set of device arrays, built together by one
+ * sequence of launches (a plain scene is a batch of one).  Scene handles are
+ * first_scene .. first_scene + nscenes - 1; rt_lbvh_build on any of them
+ * rebuilds every scene of the batch.  At most 256 scenes per batch. */
+int rt_batch_create(rt_ctx *ctx, uint32_t nscenes, const uint32_t *max_tris,
+                    const uint32_t *max_verts, const uint32_t *depths, int *first_scene);
+/* Releases the handle; a batch's memory is freed with its last handle. */
 int rt_scene_destroy(rt_ctx *ctx, int scene);
 /* Copy vertices (3 floats each) and triangle indices (3 uint32 each; the
  * reference's DWORD, which is 8 bytes on LP64 Linux, is not used). */
@@ -146,7 +154,8 @@ int rt_scene_info(rt_ctx *ctx, int scene, uint32_t *nt, uint32_t *nv,
                   uint32_t *depth);
 
 /* ---- build (StructureConstruction::GPUBuildStructure, StructureConstruction.cpp:86-131) */
-/* Rebuilds the LBVH of the scene's current vertices on the context stream.
+/* Rebuilds the LBVH of the scene's current vertices (and of every other scene
+ * of its batch) on the context stream.
  * times != NULL synchronises and fills the per-stage device times. */
 int rt_lbvh_build(rt_ctx *ctx, int scene, rt_build_times *times);
 /* Copies build results to the host; any pointer may be NULL.

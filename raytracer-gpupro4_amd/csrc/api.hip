@@ -10,7 +10,8 @@
 
 #include "rt_internal.h"
 
-using rt::Scene;
+using rt::Batch;
+using rt::SceneDesc;
 
 namespace {
 
@@ -37,30 +38,54 @@ int hip_fail(rt_ctx *ctx, hipError_t e, const char *what)
         if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
     } while (0)
 
-Scene *get_scene(rt_ctx *ctx, int scene)
+// Resolves a scene handle to its batch and index within the batch.
+bool get_scene(rt_ctx *ctx, int scene, Batch **b, uint32_t *index)
 {
-    if (!ctx || scene < 0 || scene >= (int)ctx->scenes.size()) return nullptr;
-    return ctx->scenes[scene];
+    if (!ctx || scene < 0 || scene >= (int)ctx->scenes.size() || !ctx->scenes[scene].batch)
+        return false;
+    *b = ctx->scenes[scene].batch;
+    *index = ctx->scenes[scene].index;
+    return true;
 }
 
 template <typename T>
-hipError_t scene_alloc(Scene &s, T **p, size_t count)
+hipError_t batch_alloc(Batch &b, T **p, size_t count)
 {
     const size_t bytes = sizeof(T) * (count ? count : 1);
     void *v = nullptr;
     hipError_t e = hipMalloc(&v, bytes);
     if (e != hipSuccess) return e;
-    s.allocs.push_back(v);
-    s.bytes += bytes;
+    b.allocs.push_back(v);
+    b.bytes += bytes;
     *p = static_cast<T *>(v);
     return hipSuccess;
 }
 
-void scene_free(Scene *s)
+void batch_free(Batch *b)
 {
-    if (!s) return;
-    for (void *p : s->allocs) hipFree(p);
-    delete s;
+    if (!b) return;
+    for (void *p : b->allocs) hipFree(p);
+    delete b;
+}
+
+// Recomputes the compact (current-count) offsets of the build arrays and
+// copies the scene descriptors to the device.
+int batch_sync_desc(rt_ctx *ctx, Batch &b)
+{
+    uint32_t prim = 0, blocks = 0;
+    for (SceneDesc &d : b.desc) {
+        d.prim_off = prim;
+        d.block_off = blocks;
+        d.nblocks = rt::bounds_blocks(d.nt);
+        prim += d.nt;
+        blocks += d.nblocks;
+    }
+    b.total_tris = prim;
+    b.total_blocks = blocks;
+    RT_HIP(ctx, hipMemcpyAsync(b.d_desc, b.desc.data(), sizeof(SceneDesc) * b.nscenes,
+                               hipMemcpyHostToDevice, ctx->stream));
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RT_OK;
 }
 
 int ensure_staging(rt_ctx *ctx, uint64_t bytes)
@@ -129,6 +154,100 @@ hipEvent_t *prof_events(rt_ctx *ctx, int kind, size_t n)
     return &ctx->pool[r.first];
 }
 
+int create_batch(rt_ctx *ctx, uint32_t n, const uint32_t *max_tris, const uint32_t *max_verts,
+                 const uint32_t *depths, int *first)
+{
+    if (!ctx || !first || n == 0 || !max_tris || !max_verts || !depths)
+        return fail(ctx, RT_ERR_ARG, "scene creation: null argument or no scenes");
+    if (n > (uint32_t)rt::kMaxBatchScenes)
+        return fail(ctx, RT_ERR_ARG, "a batch holds at most %d scenes, got %u", rt::kMaxBatchScenes, n);
+    uint64_t cap_tris = 0, cap_verts = 0, heap = 0, table = 0, blocks = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (depths[i] < 2 || depths[i] > 26)
+            return fail(ctx, RT_ERR_ARG, "scene %u: depth %u outside [2, 26]", i, depths[i]);
+        cap_tris += max_tris[i];
+        cap_verts += max_verts[i];
+        heap += (uint64_t)1 << depths[i];
+        table += ((uint64_t)1 << rt::table_bits(max_tris[i])) + 1;
+        blocks += rt::bounds_blocks(max_tris[i]);
+    }
+    if (cap_tris > (1u << 25) || cap_verts > (3u << 25))
+        return fail(ctx, RT_ERR_CAPACITY, "batch capacity %llu triangles / %llu vertices too large",
+                    (unsigned long long)cap_tris, (unsigned long long)cap_verts);
+    hipSetDevice(ctx->device);
+    Batch *b = new Batch();
+    b->nscenes = n;
+    b->desc.resize(n);
+    b->max_tris.assign(max_tris, max_tris + n);
+    b->max_verts.assign(max_verts, max_verts + n);
+    b->rest.assign(n, 0);
+    uint32_t in_prim = 0, vert = 0;
+    uint64_t hoff = 0, toff = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        SceneDesc &d = b->desc[i];
+        memset(&d, 0, sizeof(d));
+        d.in_prim_off = in_prim;
+        d.vert_off = vert;
+        d.depth = depths[i];
+        d.tbits = rt::table_bits(max_tris[i]);
+        d.heap_off = hoff;
+        d.table_off = toff;
+        in_prim += max_tris[i];
+        vert += max_verts[i];
+        hoff += (uint64_t)1 << depths[i];
+        toff += ((uint64_t)1 << d.tbits) + 1;
+        if (depths[i] > b->max_depth) b->max_depth = depths[i];
+    }
+    b->total_heap = heap;
+    b->total_table = table;
+    const uint32_t nt = cap_tris ? (uint32_t)cap_tris : 1;
+    const uint32_t tiles = rt::sort_tiles(nt);
+    const uint32_t passes = rt::kCodePasses + (n > 1 ? 1 : 0);
+    const size_t nwords = rt::kCodePasses * rt::kRadix + rt::kCounters + (size_t)passes * tiles * rt::kRadix;
+    hipError_t e = hipSuccess;
+    uint32_t *words = nullptr;
+#define ALLOC(p, cnt) \
+    if (e == hipSuccess) e = batch_alloc(*b, &(p), (cnt))
+    ALLOC(b->xyz, 3 * (size_t)(cap_verts ? cap_verts : 1));
+    ALLOC(b->idx, 3 * (size_t)nt);
+    ALLOC(b->d_desc, n);
+    ALLOC(b->hdr, n);
+    ALLOC(b->pbox, nt);
+    ALLOC(b->cent, 3 * (size_t)nt);
+    ALLOC(b->partials, 6 * (size_t)(blocks ? blocks : 1));
+    ALLOC(b->keys[0], nt);
+    ALLOC(b->keys[1], nt);
+    ALLOC(b->vals[0], nt);
+    ALLOC(b->vals[1], nt);
+    ALLOC(words, nwords);
+    ALLOC(b->table, table);
+    ALLOC(b->big[0], nt);
+    ALLOC(b->big[1], nt);
+    ALLOC(b->small, nt);
+    ALLOC(b->top_nodes, nt);
+    ALLOC(b->top_counts, (size_t)n * rt::kMaxLevels);
+    ALLOC(b->heap, heap);
+    ALLOC(b->trav, nt);
+    ALLOC(b->tris, nt);
+#undef ALLOC
+    if (e != hipSuccess) {
+        batch_free(b);
+        return hip_fail(ctx, e, "scene creation: hipMalloc");
+    }
+    b->hist = words;
+    b->counters = words + rt::kCodePasses * rt::kRadix;
+    b->status = b->counters + rt::kCounters;
+    int rc = batch_sync_desc(ctx, *b);
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+    *first = (int)ctx->scenes.size();
+    for (uint32_t i = 0; i < n; ++i) ctx->scenes.push_back(rt::SceneRef{b, i});
+    b->refs = (int)n;
+    return RT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -143,6 +262,10 @@ int rt_create(rt_ctx **out, int device)
     if (hipSetDevice(device) != hipSuccess) return RT_ERR_HIP;
     rt_ctx *ctx = new rt_ctx();
     ctx->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        ctx->cu_count = cus;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return RT_ERR_HIP;
@@ -162,7 +285,10 @@ void rt_destroy(rt_ctx *ctx)
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (Scene *s : ctx->scenes) scene_free(s);
+    for (rt::SceneRef &r : ctx->scenes) {
+        if (r.batch && --r.batch->refs == 0) batch_free(r.batch);
+        r.batch = nullptr;
+    }
     hipFree(ctx->frame.rgba);
     hipFree(ctx->frame.hits);
     hipFree(ctx->frame.occ);
@@ -195,183 +321,171 @@ int rt_memory_in_use(rt_ctx *ctx, uint64_t *bytes)
 {
     if (!ctx || !bytes) return RT_ERR_ARG;
     uint64_t b = ctx->bytes;
-    for (Scene *s : ctx->scenes)
-        if (s) b += s->bytes;
+    std::vector<Batch *> seen;
+    for (const rt::SceneRef &r : ctx->scenes) {
+        if (!r.batch) continue;
+        bool dup = false;
+        for (Batch *x : seen) dup |= (x == r.batch);
+        if (!dup) {
+            seen.push_back(r.batch);
+            b += r.batch->bytes;
+        }
+    }
     *bytes = b;
     return RT_OK;
 }
 
 int rt_scene_create(rt_ctx *ctx, uint32_t max_tris, uint32_t max_verts, uint32_t depth, int *scene)
 {
-    if (!ctx || !scene) return fail(ctx, RT_ERR_ARG, "rt_scene_create: null argument");
-    if (depth < 2 || depth > 26)
-        return fail(ctx, RT_ERR_ARG, "rt_scene_create: depth %u outside [2, 26]", depth);
-    if (max_tris > (1u << 25))
-        return fail(ctx, RT_ERR_CAPACITY, "rt_scene_create: max_tris %u above 2^25", max_tris);
-    hipSetDevice(ctx->device);
-    Scene *s = new Scene();
-    s->max_tris = max_tris;
-    s->max_verts = max_verts;
-    s->depth = depth;
-    const uint32_t nt = max_tris ? max_tris : 1;
-    const uint32_t tiles = rt::sort_tiles(nt);
-    const uint32_t nb = rt::bounds_blocks(nt);
-    hipError_t e = hipSuccess;
-    uint32_t *words = nullptr;
-    const size_t nwords = rt::kSortPasses * rt::kRadix + rt::kCounters +
-                          (size_t)rt::kSortPasses * tiles * rt::kRadix;
-#define ALLOC(p, n) \
-    if (e == hipSuccess) e = scene_alloc(*s, &(p), (n))
-    ALLOC(s->xyz, 3 * (size_t)(max_verts ? max_verts : 1));
-    ALLOC(s->idx, 3 * (size_t)nt);
-    ALLOC(s->pbox, nt);
-    ALLOC(s->cent, 3 * (size_t)nt);
-    ALLOC(s->partials, 6 * (size_t)nb);
-    ALLOC(s->keys[0], nt);
-    ALLOC(s->keys[1], nt);
-    ALLOC(s->vals[0], nt);
-    ALLOC(s->vals[1], nt);
-    ALLOC(words, nwords);
-    ALLOC(s->frontier[0], nt);
-    ALLOC(s->frontier[1], nt);
-    ALLOC(s->leaves, nt);
-    ALLOC(s->heap, (size_t)1 << depth);
-    ALLOC(s->trav, nt);
-    ALLOC(s->trav_parent, nt);
-    ALLOC(s->flags, nt);
-    ALLOC(s->tris, nt);
-    ALLOC(s->hdr, 1);
-#undef ALLOC
-    if (e != hipSuccess) {
-        scene_free(s);
-        return hip_fail(ctx, e, "rt_scene_create: hipMalloc");
-    }
-    s->hist = words;
-    s->counters = words + rt::kSortPasses * rt::kRadix;
-    s->status = s->counters + rt::kCounters;
-    ctx->scenes.push_back(s);
-    *scene = (int)ctx->scenes.size() - 1;
-    return RT_OK;
+    return create_batch(ctx, 1, &max_tris, &max_verts, &depth, scene);
+}
+
+int rt_batch_create(rt_ctx *ctx, uint32_t nscenes, const uint32_t *max_tris,
+                    const uint32_t *max_verts, const uint32_t *depths, int *first_scene)
+{
+    return create_batch(ctx, nscenes, max_tris, max_verts, depths, first_scene);
 }
 
 int rt_scene_destroy(rt_ctx *ctx, int scene)
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s) return fail(ctx, RT_ERR_ARG, "rt_scene_destroy: unknown scene %d", scene);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_scene_destroy: unknown scene %d", scene);
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
-    scene_free(s);
-    ctx->scenes[scene] = nullptr;
+    ctx->scenes[scene].batch = nullptr;
+    if (--b->refs == 0) batch_free(b);
     return RT_OK;
 }
 
-static int check_upload(rt_ctx *ctx, Scene *s, int scene, uint32_t nv, uint32_t nt)
+static int check_upload(rt_ctx *ctx, Batch *b, uint32_t i, int scene, uint32_t nv, uint32_t nt)
 {
-    if (!s) return fail(ctx, RT_ERR_ARG, "unknown scene %d", scene);
-    if (nt > s->max_tris || nv > s->max_verts)
+    if (nt > b->max_tris[i] || nv > b->max_verts[i])
         return fail(ctx, RT_ERR_CAPACITY, "scene %d holds %u tris / %u verts, got %u / %u", scene,
-                    s->max_tris, s->max_verts, nt, nv);
+                    b->max_tris[i], b->max_verts[i], nt, nv);
     return RT_OK;
+}
+
+static int finish_upload(rt_ctx *ctx, Batch *b, uint32_t i, uint32_t nv, uint32_t nt)
+{
+    b->desc[i].nv = nv;
+    b->desc[i].nt = nt;
+    b->built = false;
+    b->rest[i] = 0;
+    return batch_sync_desc(ctx, *b);
 }
 
 int rt_scene_upload(rt_ctx *ctx, int scene, const float *xyz, uint32_t nv, const uint32_t *idx,
                     uint32_t nt)
 {
-    Scene *s = get_scene(ctx, scene);
-    int rc = check_upload(ctx, s, scene, nv, nt);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_scene_upload: unknown scene %d", scene);
+    int rc = check_upload(ctx, b, i, scene, nv, nt);
     if (rc) return rc;
     if ((nv && !xyz) || (nt && !idx)) return fail(ctx, RT_ERR_ARG, "rt_scene_upload: null array");
-    for (uint64_t i = 0; i < 3ull * nt; ++i)
-        if (idx[i] >= nv)
-            return fail(ctx, RT_ERR_ARG, "rt_scene_upload: index %u at %llu >= %u vertices", idx[i],
-                        (unsigned long long)i, nv);
+    for (uint64_t k = 0; k < 3ull * nt; ++k)
+        if (idx[k] >= nv)
+            return fail(ctx, RT_ERR_ARG, "rt_scene_upload: index %u at %llu >= %u vertices", idx[k],
+                        (unsigned long long)k, nv);
     hipSetDevice(ctx->device);
-    if (nv) RT_HIP(ctx, hipMemcpyAsync(s->xyz, xyz, 12ull * nv, hipMemcpyHostToDevice, ctx->stream));
-    if (nt) RT_HIP(ctx, hipMemcpyAsync(s->idx, idx, 12ull * nt, hipMemcpyHostToDevice, ctx->stream));
+    const SceneDesc &d = b->desc[i];
+    if (nv)
+        RT_HIP(ctx, hipMemcpyAsync(b->xyz + 3 * (size_t)d.vert_off, xyz, 12ull * nv,
+                                   hipMemcpyHostToDevice, ctx->stream));
+    if (nt)
+        RT_HIP(ctx, hipMemcpyAsync(b->idx + 3 * (size_t)d.in_prim_off, idx, 12ull * nt,
+                                   hipMemcpyHostToDevice, ctx->stream));
     RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    s->nv = nv;
-    s->nt = nt;
-    s->built = false;
-    s->have_rest = false;
-    return RT_OK;
+    return finish_upload(ctx, b, i, nv, nt);
 }
 
 int rt_scene_upload_device(rt_ctx *ctx, int scene, const float *d_xyz, uint32_t nv,
                            const uint32_t *d_idx, uint32_t nt)
 {
-    Scene *s = get_scene(ctx, scene);
-    int rc = check_upload(ctx, s, scene, nv, nt);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_scene_upload_device: unknown scene %d", scene);
+    int rc = check_upload(ctx, b, i, scene, nv, nt);
     if (rc) return rc;
     if ((nv && !d_xyz) || (nt && !d_idx))
         return fail(ctx, RT_ERR_ARG, "rt_scene_upload_device: null array");
     hipSetDevice(ctx->device);
+    const SceneDesc &d = b->desc[i];
     if (nv)
-        RT_HIP(ctx, hipMemcpyAsync(s->xyz, d_xyz, 12ull * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        RT_HIP(ctx, hipMemcpyAsync(b->xyz + 3 * (size_t)d.vert_off, d_xyz, 12ull * nv,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
     if (nt) {
-        RT_HIP(ctx, hipMemcpyAsync(s->idx, d_idx, 12ull * nt, hipMemcpyDeviceToDevice, ctx->stream));
+        uint32_t *didx = b->idx + 3 * (size_t)d.in_prim_off;
+        RT_HIP(ctx, hipMemcpyAsync(didx, d_idx, 12ull * nt, hipMemcpyDeviceToDevice, ctx->stream));
         // indices are validated on the device before any kernel gathers through them
-        RT_HIP(ctx, hipMemsetAsync(s->counters, 0, sizeof(uint32_t), ctx->stream));
-        hipLaunchKernelGGL(k_max_index, dim3(256), dim3(256), 0, ctx->stream, s->idx, 3ull * nt,
-                           s->counters);
+        RT_HIP(ctx, hipMemsetAsync(b->counters, 0, sizeof(uint32_t), ctx->stream));
+        hipLaunchKernelGGL(k_max_index, dim3(256), dim3(256), 0, ctx->stream, didx, 3ull * nt,
+                           b->counters);
         uint32_t mx = 0;
-        RT_HIP(ctx, hipMemcpyAsync(&mx, s->counters, 4, hipMemcpyDeviceToHost, ctx->stream));
+        RT_HIP(ctx, hipMemcpyAsync(&mx, b->counters, 4, hipMemcpyDeviceToHost, ctx->stream));
         RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (mx >= nv)
+        if (mx >= nv) {
+            b->desc[i].nt = 0;  // the rejected indices stay out of every build
+            batch_sync_desc(ctx, *b);
             return fail(ctx, RT_ERR_ARG, "rt_scene_upload_device: index %u >= %u vertices", mx, nv);
+        }
     }
     RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    s->nv = nv;
-    s->nt = nt;
-    s->built = false;
-    s->have_rest = false;
-    return RT_OK;
+    return finish_upload(ctx, b, i, nv, nt);
 }
 
 int rt_scene_transform(rt_ctx *ctx, int scene, const float m[12])
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s || !m) return fail(ctx, RT_ERR_ARG, "rt_scene_transform: bad argument");
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i) || !m) return fail(ctx, RT_ERR_ARG, "rt_scene_transform: bad argument");
     hipSetDevice(ctx->device);
-    if (!s->have_rest) {
-        if (!s->xyz_rest) {
-            hipError_t e = scene_alloc(*s, &s->xyz_rest, 3 * (size_t)(s->max_verts ? s->max_verts : 1));
+    const SceneDesc &d = b->desc[i];
+    if (!b->rest[i]) {
+        if (!b->xyz_rest) {
+            uint64_t cap = 0;
+            for (uint32_t v : b->max_verts) cap += v;
+            hipError_t e = batch_alloc(*b, &b->xyz_rest, 3 * (size_t)(cap ? cap : 1));
             if (e != hipSuccess) return hip_fail(ctx, e, "rt_scene_transform: hipMalloc");
         }
-        if (s->nv)
-            RT_HIP(ctx, hipMemcpyAsync(s->xyz_rest, s->xyz, 12ull * s->nv, hipMemcpyDeviceToDevice,
-                                       ctx->stream));
-        s->have_rest = true;
+        if (d.nv)
+            RT_HIP(ctx, hipMemcpyAsync(b->xyz_rest + 3 * (size_t)d.vert_off, b->xyz + 3 * (size_t)d.vert_off,
+                                       12ull * d.nv, hipMemcpyDeviceToDevice, ctx->stream));
+        b->rest[i] = 1;
     }
     rt::Xform x;
     memcpy(x.m, m, sizeof(x.m));
-    RT_HIP(ctx, rt::launch_transform(*s, x, ctx->stream));
-    s->built = false;
+    RT_HIP(ctx, rt::launch_transform(*b, i, x, ctx->stream));
+    b->built = false;
     return RT_OK;
 }
 
 int rt_scene_info(rt_ctx *ctx, int scene, uint32_t *nt, uint32_t *nv, uint32_t *depth)
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s) return fail(ctx, RT_ERR_ARG, "rt_scene_info: unknown scene %d", scene);
-    if (nt) *nt = s->nt;
-    if (nv) *nv = s->nv;
-    if (depth) *depth = s->depth;
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_scene_info: unknown scene %d", scene);
+    if (nt) *nt = b->desc[i].nt;
+    if (nv) *nv = b->desc[i].nv;
+    if (depth) *depth = b->desc[i].depth;
     return RT_OK;
 }
 
 int rt_lbvh_build(rt_ctx *ctx, int scene, rt_build_times *times)
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s) return fail(ctx, RT_ERR_ARG, "rt_lbvh_build: unknown scene %d", scene);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_lbvh_build: unknown scene %d", scene);
     hipSetDevice(ctx->device);
     hipEvent_t *ev = times ? ctx->ev : nullptr;
     if (!times && ctx->profiling) ev = prof_events(ctx, 0, 6);
-    RT_HIP(ctx, rt::launch_build(*s, ctx->stream, ev));
-    s->built = true;
+    RT_HIP(ctx, rt::launch_build(*b, ctx->stream, ev, ctx->cu_count));
+    b->built = true;
     if (times) {
         RT_HIP(ctx, hipEventSynchronize(ctx->ev[5]));
         float ms[5];
-        for (int i = 0; i < 5; ++i) hipEventElapsedTime(&ms[i], ctx->ev[i], ctx->ev[i + 1]);
+        for (int k = 0; k < 5; ++k) hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]);
         times->bounds_ms = ms[0];
         times->morton_ms = ms[1];
         times->sort_ms = ms[2];
@@ -382,30 +496,44 @@ int rt_lbvh_build(rt_ctx *ctx, int scene, rt_build_times *times)
     return RT_OK;
 }
 
+__global__ void k_local_ids(const int32_t *vals, uint32_t off, uint32_t n, int32_t *out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = vals[off + i] - (int32_t)off;
+}
+
 int rt_lbvh_read(rt_ctx *ctx, int scene, rt_node32 *nodes, uint32_t *sorted_codes,
                  int32_t *sorted_ids, float scene_box[6])
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s) return fail(ctx, RT_ERR_ARG, "rt_lbvh_read: unknown scene %d", scene);
-    if (!s->built) return fail(ctx, RT_ERR_STATE, "rt_lbvh_read: scene %d not built", scene);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_lbvh_read: unknown scene %d", scene);
+    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_lbvh_read: scene %d not built", scene);
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
+    const SceneDesc &d = b->desc[i];
     if (nodes)
-        RT_HIP(ctx, hipMemcpyAsync(nodes, s->heap, sizeof(rt_node32) << s->depth,
+        RT_HIP(ctx, hipMemcpyAsync(nodes, b->heap + d.heap_off, sizeof(rt_node32) << d.depth,
                                    hipMemcpyDeviceToHost, st));
-    if (sorted_codes && s->nt)
-        RT_HIP(ctx, hipMemcpyAsync(sorted_codes, s->keys[0], 4ull * s->nt, hipMemcpyDeviceToHost, st));
-    if (sorted_ids && s->nt)
-        RT_HIP(ctx, hipMemcpyAsync(sorted_ids, s->vals[0], 4ull * s->nt, hipMemcpyDeviceToHost, st));
+    if (sorted_codes && d.nt)
+        RT_HIP(ctx, hipMemcpyAsync(sorted_codes, b->keys[0] + d.prim_off, 4ull * d.nt,
+                                   hipMemcpyDeviceToHost, st));
+    if (sorted_ids && d.nt) {
+        // ids are batch-global on the device; shift them to the scene's own
+        int32_t *tmp = b->vals[1];
+        hipLaunchKernelGGL(k_local_ids, dim3((d.nt + 255) / 256), dim3(256), 0, st, b->vals[0],
+                           d.prim_off, d.nt, tmp);
+        RT_HIP(ctx, hipMemcpyAsync(sorted_ids, tmp, 4ull * d.nt, hipMemcpyDeviceToHost, st));
+    }
     rt::SceneHdr h;
-    RT_HIP(ctx, hipMemcpyAsync(&h, s->hdr, sizeof(h), hipMemcpyDeviceToHost, st));
+    RT_HIP(ctx, hipMemcpyAsync(&h, b->hdr + i, sizeof(h), hipMemcpyDeviceToHost, st));
     RT_HIP(ctx, hipStreamSynchronize(st));
     if (scene_box) memcpy(scene_box, h.scene_box, sizeof(float) * 6);
     return RT_OK;
 }
 
-static int trace_common(rt_ctx *ctx, Scene *s, const rt_ray48 *d_rays, uint32_t n, rt_hit24 *d_hits,
-                        uint32_t flags, rt_frame_stats *stats, bool sync_out)
+static int trace_common(rt_ctx *ctx, Batch *b, uint32_t i, const rt_ray48 *d_rays, uint32_t n,
+                        rt_hit24 *d_hits, uint32_t flags, rt_frame_stats *stats)
 {
     hipStream_t st = ctx->stream;
     const bool count = (flags & RT_FLAG_COUNT) != 0;
@@ -414,7 +542,7 @@ static int trace_common(rt_ctx *ctx, Scene *s, const rt_ray48 *d_rays, uint32_t 
     hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
     if (!stats && ctx->profiling) ev = prof_events(ctx, 1, 2);
     if (ev) hipEventRecord(ev[0], st);
-    RT_HIP(ctx, rt::launch_trace(*s, d_rays, n, d_hits, flags, ctx->stats, st));
+    RT_HIP(ctx, rt::launch_trace(*b, i, d_rays, n, d_hits, flags, ctx->stats, st));
     if (ev) hipEventRecord(ev[1], st);
     if (ctx->profiling) ctx->prof_primary += n;
     if (stats) {
@@ -427,8 +555,6 @@ static int trace_common(rt_ctx *ctx, Scene *s, const rt_ray48 *d_rays, uint32_t 
         stats->primary_rays = n;
         stats->box_tests = c[0];
         stats->tri_tests = c[1];
-    } else if (sync_out) {
-        RT_HIP(ctx, hipStreamSynchronize(st));
     }
     return RT_OK;
 }
@@ -436,9 +562,10 @@ static int trace_common(rt_ctx *ctx, Scene *s, const rt_ray48 *d_rays, uint32_t 
 int rt_trace(rt_ctx *ctx, int scene, const rt_ray48 *rays, uint32_t n, rt_hit24 *hits,
              uint32_t flags, rt_frame_stats *stats)
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s) return fail(ctx, RT_ERR_ARG, "rt_trace: unknown scene %d", scene);
-    if (!s->built) return fail(ctx, RT_ERR_STATE, "rt_trace: scene %d not built", scene);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_trace: unknown scene %d", scene);
+    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_trace: scene %d not built", scene);
     if (n && (!rays || !hits)) return fail(ctx, RT_ERR_ARG, "rt_trace: null array");
     hipSetDevice(ctx->device);
     const uint64_t rb = (sizeof(rt_ray48) * (uint64_t)n + 255) & ~255ull;
@@ -449,7 +576,7 @@ int rt_trace(rt_ctx *ctx, int scene, const rt_ray48 *rays, uint32_t n, rt_hit24 
     if (n)
         RT_HIP(ctx, hipMemcpyAsync(d_rays, rays, sizeof(rt_ray48) * (uint64_t)n,
                                    hipMemcpyHostToDevice, ctx->stream));
-    rc = trace_common(ctx, s, d_rays, n, d_hits, flags, stats, false);
+    rc = trace_common(ctx, b, i, d_rays, n, d_hits, flags, stats);
     if (rc) return rc;
     if (n)
         RT_HIP(ctx, hipMemcpyAsync(hits, d_hits, sizeof(rt_hit24) * (uint64_t)n,
@@ -461,22 +588,24 @@ int rt_trace(rt_ctx *ctx, int scene, const rt_ray48 *rays, uint32_t n, rt_hit24 
 int rt_trace_device(rt_ctx *ctx, int scene, const rt_ray48 *d_rays, uint32_t n, rt_hit24 *d_hits,
                     uint32_t flags, rt_frame_stats *stats)
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s) return fail(ctx, RT_ERR_ARG, "rt_trace_device: unknown scene %d", scene);
-    if (!s->built) return fail(ctx, RT_ERR_STATE, "rt_trace_device: scene %d not built", scene);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_trace_device: unknown scene %d", scene);
+    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_trace_device: scene %d not built", scene);
     if (n && (!d_rays || !d_hits)) return fail(ctx, RT_ERR_ARG, "rt_trace_device: null array");
     if (((uintptr_t)d_rays & 15) || ((uintptr_t)d_hits & 7))
         return fail(ctx, RT_ERR_ARG, "rt_trace_device: rays need 16-byte, hits 8-byte alignment");
     hipSetDevice(ctx->device);
-    return trace_common(ctx, s, d_rays, n, d_hits, flags, stats, false);
+    return trace_common(ctx, b, i, d_rays, n, d_hits, flags, stats);
 }
 
 int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, uint32_t flags,
                      rt_frame_stats *stats)
 {
-    Scene *s = get_scene(ctx, scene);
-    if (!s || !desc) return fail(ctx, RT_ERR_ARG, "rt_render_direct: bad argument");
-    if (!s->built) return fail(ctx, RT_ERR_STATE, "rt_render_direct: scene %d not built", scene);
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i) || !desc) return fail(ctx, RT_ERR_ARG, "rt_render_direct: bad argument");
+    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_render_direct: scene %d not built", scene);
     if (desc->nlights > 8 || (desc->nlights && !desc->lights))
         return fail(ctx, RT_ERR_ARG, "rt_render_direct: 0..8 lights with a light array");
     if (desc->width == 0 || desc->height == 0 || desc->width > 16384 || desc->height > 16384)
@@ -491,7 +620,7 @@ int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, uint32_
     hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
     if (!stats && ctx->profiling) ev = prof_events(ctx, 1, 2);
     if (ev) hipEventRecord(ev[0], st);
-    RT_HIP(ctx, rt::launch_render_direct(*s, *desc, ctx->frame, flags, ctx->stats, st));
+    RT_HIP(ctx, rt::launch_render_direct(*b, i, *desc, ctx->frame, flags, ctx->stats, st));
     if (ev) hipEventRecord(ev[1], st);
     if (ctx->profiling) ctx->prof_primary += (uint64_t)desc->width * desc->height;
     if (stats) {

@@ -1,14 +1,21 @@
-// LBVH build on gfx950: primitive bounds, 30-bit Morton codes, onesweep radix
-// sort, level-by-level heap emission and a bottom-up refit with atomic flags.
+// LBVH build on gfx950 for a batch of scenes: primitive bounds, 30-bit Morton
+// codes, onesweep radix sort, heap emission and refit.
 //
 // Reference stages (StructureConstruction::GPUBuildStructure,
 // RayTracerCS/StructureConstruction.cpp:86-131):
-//   LBVH_PreBuild   (LBVH.hlsl:15-132)  -> k_bounds + k_morton
-//   BitonicSort     (BitonicSort.cpp:56-117) -> k_onesweep x 4 (stable LSD radix)
-//   LBVH_Build      (LBVH.hlsl:134-226) -> k_emit_level x depth
-//   LBVH_BuildAABB  (LBVH.hlsl:228-270) -> k_refit
+//   LBVH_PreBuild   (LBVH.hlsl:15-132)       -> k_bounds + k_morton
+//   BitonicSort     (BitonicSort.cpp:56-117) -> k_onesweep x 4 (+1 scene pass)
+//   LBVH_Build      (LBVH.hlsl:134-226)      -> k_split_table + k_top + k_subtree
+//   LBVH_BuildAABB  (LBVH.hlsl:228-270)      -> k_subtree + k_top_refit
 // Node words follow the CPU builder (LBVH.cpp:190-314), including its axis
 // rule; float arithmetic is IEEE single with no contraction (-ffp-contract=off).
+//
+// Emission splits the tree by size.  k_top (one block per scene) emits the
+// nodes holding more than kSubtreeMax primitives level by level, finding each
+// split through a bucket table over the sorted codes.  Every smaller subtree
+// is emitted and refitted by one block of k_subtree entirely in LDS, with
+// only block barriers.  k_top_refit (one block per scene) then merges the big
+// nodes bottom-up.  No kernel needs device-scope fences.
 #include "rt_internal.h"
 
 #include <cmath>
@@ -32,8 +39,9 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-// Reduces lo[3]/hi[3] over a 256-thread block; the result is valid in every
-// thread.  red must hold 4 waves x 6 floats.
+// Reduces lo[3]/hi[3] over a block of NW waves; the result is valid in every
+// thread.  red must hold NW x 6 floats.
+template <int NW>
 __device__ __forceinline__ void block_box_reduce(float lo[3], float hi[3], float *red)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -53,7 +61,7 @@ __device__ __forceinline__ void block_box_reduce(float lo[3], float hi[3], float
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         float a = red[k], b = red[3 + k];
-        for (int w = 1; w < 4; ++w) {
+        for (int w = 1; w < NW; ++w) {
             a = fminf(a, red[w * 6 + k]);
             b = fmaxf(b, red[w * 6 + 3 + k]);
         }
@@ -87,20 +95,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
     return wbase + x - v;
 }
 
-// Appends `mult` consecutive slots for every lane with pred set; all 64 lanes
-// of the wave must call it.  Returns this lane's first slot.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred, uint32_t mult)
-{
-    const uint64_t m = __ballot(pred);
-    const int lane = __lane_id();
-    if (m == 0) return 0;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m) * mult);
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) * mult;
-}
-
 __device__ __forceinline__ uint32_t part1by2(uint32_t n)
 {
     n &= 1023u;
@@ -118,41 +112,131 @@ __device__ __forceinline__ uint32_t quantise(float c, float m, float f)
     return (uint32_t)(int)s;
 }
 
+// Last scene s with key(s) <= x over S ascending keys read through get().
+template <typename F>
+__device__ __forceinline__ uint32_t upper_scene(uint32_t S, uint32_t x, F get)
+{
+    uint32_t lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (get(mid) <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t scene_of_prim(const SceneDesc *d, uint32_t S, uint32_t g)
+{
+    return upper_scene(S, g, [&](uint32_t s) { return d[s].prim_off; });
+}
+
+// Leaf rule and axis bits of build_recursive (LBVH.cpp:194-198, 231-266,
+// 304-310) for a node of cnt primitives at heap index h entered under a split
+// at bit pp (31 for the root): returns 1 for a leaf and sets *axis; returns 0
+// for an internal node and sets *split_bit.
+__device__ __forceinline__ int classify(uint32_t h, int32_t cnt, int pp, uint32_t leaf_level,
+                                        uint32_t code_first, uint32_t code_last, int *axis,
+                                        int *split_bit)
+{
+    if (cnt <= 1 || h >= leaf_level || pp == 0) {
+        *axis = (31 - pp) % 3;
+        return 1;
+    }
+    const uint32_t x = code_first ^ code_last;
+    if (x == 0u) {
+        *axis = 1;
+        return 1;
+    }
+    *split_bit = 31 - __clz(x);
+    *axis = (30 - *split_bit) % 3;
+    return 0;
+}
+
+__device__ __forceinline__ void write_heap(rt_node32 *n, const float lo[3], const float hi[3],
+                                           int32_t prim_count, int32_t prim_pos)
+{
+    float4 *p = reinterpret_cast<float4 *>(n);
+    p[0] = make_float4(lo[0], lo[1], lo[2], hi[0]);
+    p[1] = make_float4(hi[1], hi[2], __int_as_float(prim_count), __int_as_float(prim_pos));
+}
+
+__device__ __forceinline__ void write_heap_word(rt_node32 *n, int32_t prim_count, int32_t prim_pos)
+{
+    *reinterpret_cast<int2 *>(&n->prim_count) = make_int2(prim_count, prim_pos);
+}
+
+// Box and sorted-order triangle record of one leaf primitive.
+__device__ __forceinline__ void leaf_prim(const SceneDesc &D, const int32_t *__restrict__ vals,
+                                          const PrimBox *__restrict__ pbox,
+                                          const float *__restrict__ xyz,
+                                          const uint32_t *__restrict__ idx, TravTri *__restrict__ tris,
+                                          uint32_t pos, int32_t j, int32_t cnt, uint32_t h,
+                                          float lo[3], float hi[3])
+{
+    const int32_t g = vals[pos];
+    const PrimBox b = pbox[g];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(lo[k], b.mn[k]);
+        hi[k] = fmaxf(hi[k], b.mx[k]);
+    }
+    const uint32_t local = (uint32_t)g - D.prim_off;
+    const uint32_t *tri = idx + 3 * ((size_t)D.in_prim_off + local);
+    const float *va = xyz + 3 * ((size_t)D.vert_off + tri[0]);
+    const float *vb = xyz + 3 * ((size_t)D.vert_off + tri[1]);
+    const float *vc = xyz + 3 * ((size_t)D.vert_off + tri[2]);
+    const float ax = va[0], ay = va[1], az = va[2];
+    TravTri t;
+    t.a = make_float4(ax, ay, az, __int_as_float((int32_t)local));
+    t.e1 = make_float4(vb[0] - ax, vb[1] - ay, vb[2] - az, __int_as_float(j == 0 ? cnt : 0));
+    t.e2 = make_float4(vc[0] - ax, vc[1] - ay, vc[2] - az, __int_as_float(j == 0 ? (int32_t)h : 0));
+    tris[pos] = t;
+}
+
 // ---------------------------------------------------------------------------
 // Stage 1: per-triangle box and 3x centroid (LBVH.hlsl:26-36, LBVH.cpp:501-516);
 // per-block partial scene boxes; clears the sort/emit scratch words.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBoundsThreads) void k_bounds(
-    const float *__restrict__ xyz, const uint32_t *__restrict__ idx, uint32_t nt,
-    PrimBox *__restrict__ pbox, float *__restrict__ cent, float *__restrict__ partials,
-    uint32_t *__restrict__ zero_words, uint32_t nzero)
+    const SceneDesc *__restrict__ desc, uint32_t S, uint32_t total_blocks,
+    const float *__restrict__ xyz, const uint32_t *__restrict__ idx, PrimBox *__restrict__ pbox,
+    float *__restrict__ cent, float *__restrict__ partials, uint32_t *__restrict__ zero_words,
+    uint32_t nzero)
 {
     __shared__ float red[24];
+    for (uint32_t j = blockIdx.x * kBoundsThreads + threadIdx.x; j < nzero;
+         j += gridDim.x * kBoundsThreads)
+        zero_words[j] = 0u;
+    if (blockIdx.x >= total_blocks) return;
+    const uint32_t s = upper_scene(S, blockIdx.x, [&](uint32_t i) { return desc[i].block_off; });
+    const SceneDesc D = desc[s];
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const uint32_t base = blockIdx.x * (uint32_t)kBoundsTile;
+    const uint32_t base = (blockIdx.x - D.block_off) * (uint32_t)kBoundsTile;
 #pragma unroll
     for (int r = 0; r < kBoundsItems; ++r) {
         const uint32_t i = base + r * kBoundsThreads + threadIdx.x;
-        if (i < nt) {
-            const uint32_t ia = idx[3 * (size_t)i + 0], ib = idx[3 * (size_t)i + 1],
-                           ic = idx[3 * (size_t)i + 2];
+        if (i < D.nt) {
+            const uint32_t *tri = idx + 3 * ((size_t)D.in_prim_off + i);
+            const float *va = xyz + 3 * ((size_t)D.vert_off + tri[0]);
+            const float *vb = xyz + 3 * ((size_t)D.vert_off + tri[1]);
+            const float *vc = xyz + 3 * ((size_t)D.vert_off + tri[2]);
+            const size_t g = (size_t)D.prim_off + i;
             PrimBox b;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const float a = xyz[3 * (size_t)ia + k], bb = xyz[3 * (size_t)ib + k],
-                            c = xyz[3 * (size_t)ic + k];
+                const float a = va[k], bb = vb[k], c = vc[k];
                 b.mn[k] = fminf(a, fminf(bb, c));
                 b.mx[k] = fmaxf(a, fmaxf(bb, c));
-                const float s = a + bb;
-                cent[3 * (size_t)i + k] = s + c;
+                const float sum = a + bb;
+                cent[3 * g + k] = sum + c;
                 lo[k] = fminf(lo[k], b.mn[k]);
                 hi[k] = fmaxf(hi[k], b.mx[k]);
             }
-            pbox[i] = b;
+            pbox[g] = b;
         }
     }
-    block_box_reduce(lo, hi, red);
+    block_box_reduce<4>(lo, hi, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -160,44 +244,41 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(
             partials[blockIdx.x * 6 + 3 + k] = hi[k];
         }
     }
-    for (uint32_t j = blockIdx.x * kBoundsThreads + threadIdx.x; j < nzero;
-         j += gridDim.x * kBoundsThreads)
-        zero_words[j] = 0u;
 }
 
 // ---------------------------------------------------------------------------
-// Stage 2: scene box from the partials, Morton codes (LBVH.hlsl:92-114,
-// LBVH.cpp:520-553), identity ids, digit histograms of all sort passes.
-// Block 0 seeds the root of the emit frontier (LBVH.hlsl:123-131).
+// Stage 2: scene box from the scene's partials, Morton codes (LBVH.hlsl:92-114,
+// LBVH.cpp:520-553), identity ids (batch-global triangle index), and the digit
+// histograms of the four code passes.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBoundsThreads) void k_morton(
-    const float *__restrict__ cent, uint32_t nt, const float *__restrict__ partials,
-    uint32_t nparts, uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
-    uint32_t *__restrict__ hist, SceneHdr *__restrict__ hdr, int4 *__restrict__ frontier0,
-    uint32_t *__restrict__ counters)
+    const SceneDesc *__restrict__ desc, uint32_t S, const float *__restrict__ cent,
+    const float *__restrict__ partials, uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
+    uint32_t *__restrict__ hist, SceneHdr *__restrict__ hdr)
 {
     __shared__ float red[24];
-    __shared__ uint32_t lhist[kSortPasses * kRadix];
+    __shared__ uint32_t lhist[kCodePasses * kRadix];
+    const uint32_t s = upper_scene(S, blockIdx.x, [&](uint32_t i) { return desc[i].block_off; });
+    const SceneDesc D = desc[s];
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t j = threadIdx.x; j < nparts; j += kBoundsThreads) {
+    for (uint32_t j = threadIdx.x; j < D.nblocks; j += kBoundsThreads) {
+        const float *p = partials + 6 * ((size_t)D.block_off + j);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], partials[j * 6 + k]);
-            hi[k] = fmaxf(hi[k], partials[j * 6 + 3 + k]);
+            lo[k] = fminf(lo[k], p[k]);
+            hi[k] = fmaxf(hi[k], p[3 + k]);
         }
     }
-    for (int j = threadIdx.x; j < kSortPasses * kRadix; j += kBoundsThreads) lhist[j] = 0;
-    block_box_reduce(lo, hi, red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int j = threadIdx.x; j < kCodePasses * kRadix; j += kBoundsThreads) lhist[j] = 0;
+    block_box_reduce<4>(lo, hi, red);
+    const uint32_t lb = blockIdx.x - D.block_off;
+    if (lb == 0 && threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            hdr->scene_box[k] = lo[k];
-            hdr->scene_box[3 + k] = hi[k];
+            hdr[s].scene_box[k] = lo[k];
+            hdr[s].scene_box[3 + k] = hi[k];
         }
-        hdr->root_ref = -1;
-        frontier0[0] = make_int4(1, 0, (int)nt, -1);
-        counters[0] = 1u;
     }
     float m[3], f[3];
 #pragma unroll
@@ -208,24 +289,25 @@ __global__ __launch_bounds__(kBoundsThreads) void k_morton(
         m[k] = lk;
         f[k] = 1024.0f / ext;
     }
-    const uint32_t base = blockIdx.x * (uint32_t)kBoundsTile;
+    const uint32_t base = lb * (uint32_t)kBoundsTile;
 #pragma unroll
     for (int r = 0; r < kBoundsItems; ++r) {
         const uint32_t i = base + r * kBoundsThreads + threadIdx.x;
-        if (i < nt) {
-            const uint32_t qx = quantise(cent[3 * (size_t)i + 0], m[0], f[0]);
-            const uint32_t qy = quantise(cent[3 * (size_t)i + 1], m[1], f[1]);
-            const uint32_t qz = quantise(cent[3 * (size_t)i + 2], m[2], f[2]);
+        if (i < D.nt) {
+            const size_t g = (size_t)D.prim_off + i;
+            const uint32_t qx = quantise(cent[3 * g + 0], m[0], f[0]);
+            const uint32_t qy = quantise(cent[3 * g + 1], m[1], f[1]);
+            const uint32_t qz = quantise(cent[3 * g + 2], m[2], f[2]);
             const uint32_t code = (part1by2(qx) << 2) | (part1by2(qy) << 1) | part1by2(qz);
-            keys[i] = code;
-            vals[i] = (int32_t)i;
+            keys[g] = code;
+            vals[g] = (int32_t)g;
 #pragma unroll
-            for (int p = 0; p < kSortPasses; ++p)
+            for (int p = 0; p < kCodePasses; ++p)
                 atomicAdd(&lhist[p * kRadix + ((code >> (p * kRadixBits)) & (kRadix - 1))], 1u);
         }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < kSortPasses * kRadix; j += kBoundsThreads)
+    for (int j = threadIdx.x; j < kCodePasses * kRadix; j += kBoundsThreads)
         if (lhist[j]) atomicAdd(&hist[j], lhist[j]);
 }
 
@@ -234,16 +316,20 @@ __global__ __launch_bounds__(kBoundsThreads) void k_morton(
 // Tiles are claimed in launch order through an atomic counter, so a tile only
 // waits (decoupled look-back) on tiles that are already running.  Keys inside
 // a tile are ranked in load order (wave, item, lane), which keeps the sort
-// stable; ids start as 0..n-1, so the result is the (code, id) order.
+// stable; ids start as the batch-global index, so each scene ends in the
+// (code, id) order of cmpMorton (LBVH.cpp:21-38).  The scene pass (digit =
+// scene of the id, global base = the scene's first sorted position) groups the
+// scenes of a batch after the four code passes.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 
+template <bool SCENE_PASS>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
-    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
-    uint32_t *__restrict__ kout, int32_t *__restrict__ vout, uint32_t n, uint32_t shift,
-    const uint32_t *__restrict__ hist, uint32_t *__restrict__ status,
+    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin, uint32_t *__restrict__ kout,
+    int32_t *__restrict__ vout, uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
+    const SceneDesc *__restrict__ desc, uint32_t S, uint32_t *__restrict__ status,
     uint32_t *__restrict__ tile_counter)
 {
     constexpr int kWaves = kSortThreads / 64;
@@ -254,17 +340,25 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     __shared__ uint32_t s_scan[kWaves];
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ int32_t s_vals[kSortTile];
+    __shared__ uint32_t s_soff[SCENE_PASS ? kRadix : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     for (int j = tid; j < kWaves * kRadix; j += kSortThreads) (&s_wave[0][0])[j] = 0u;
+    if (SCENE_PASS)
+        for (int j = tid; j < kRadix; j += kSortThreads) s_soff[j] = j < (int)S ? desc[j].prim_off : 0xFFFFFFFFu;
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t base = tile * (uint32_t)kSortTile;
+    auto digit_of = [&](uint32_t key, int32_t val) -> uint32_t {
+        if (SCENE_PASS) return upper_scene(S, (uint32_t)val, [&](uint32_t i) { return s_soff[i]; });
+        return (key >> shift) & (kRadix - 1);
+    };
 
     uint32_t key[kSortItems];
     int32_t val[kSortItems];
     uint32_t rank[kSortItems];
+    uint32_t dig[kSortItems];
     bool valid[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
@@ -272,11 +366,12 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         valid[r] = i < n;
         key[r] = valid[r] ? kin[i] : 0u;
         val[r] = valid[r] ? vin[i] : 0;
+        dig[r] = valid[r] ? digit_of(key[r], val[r]) : 0u;
     }
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const uint32_t d = (key[r] >> shift) & (kRadix - 1);
+        const uint32_t d = dig[r];
         uint64_t peers = __ballot(valid[r]);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
@@ -313,28 +408,32 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         __hip_atomic_store(my, kFlagAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         int32_t j = (int32_t)tile - 1;
         while (j >= 0) {
-            const uint32_t s = __hip_atomic_load(status + (size_t)j * kRadix + dg, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t flag = s & ~kValMask;
+            const uint32_t sv = __hip_atomic_load(status + (size_t)j * kRadix + dg, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t flag = sv & ~kValMask;
             if (flag == 0u) {
                 __builtin_amdgcn_s_sleep(1);
                 continue;
             }
-            prefix += s & kValMask;
+            prefix += sv & kValMask;
             if (flag == kFlagInc) break;
             --j;
         }
         __hip_atomic_store(my, kFlagInc | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // Global start of this digit: exclusive scan of the whole-array histogram.
-    const uint32_t gexcl = block_exclusive_scan(hist[dg], s_scan, nullptr);
-    s_gofs[dg] = gexcl + prefix;
+    uint32_t gbase;
+    if (SCENE_PASS) {
+        gbase = dg < S ? s_soff[dg] : 0u;
+    } else {
+        gbase = block_exclusive_scan(hist[dg], s_scan, nullptr);
+    }
+    s_gofs[dg] = gbase + prefix;
     __syncthreads();
 
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         if (valid[r]) {
-            const uint32_t d = (key[r] >> shift) & (kRadix - 1);
+            const uint32_t d = dig[r];
             const uint32_t lp = s_excl[d] + s_wave[wave][d] + rank[r];
             s_keys[lp] = key[r];
             s_vals[lp] = val[r];
@@ -344,189 +443,412 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t tile_n = (n - base) < (uint32_t)kSortTile ? (n - base) : (uint32_t)kSortTile;
     for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
         const uint32_t k = s_keys[i];
-        const uint32_t d = (k >> shift) & (kRadix - 1);
+        const int32_t v = s_vals[i];
+        const uint32_t d = digit_of(k, v);
         const uint32_t dst = s_gofs[d] + (i - s_excl[d]);
         kout[dst] = k;
-        vout[dst] = s_vals[i];
+        vout[dst] = v;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Stage 4: heap emission of one tree level (LBVH_Build, LBVH.hlsl:134-226),
-// over the compacted frontier of that level instead of all 2^level slots.
-// Frontier entry: {heap index, first, count, parent} with parent = -1 for the
-// root, else (k << 6) | (side << 5) | split bit of the parent.
-// Leaf rule and axis bits are those of build_recursive (LBVH.cpp:194-198,
-// 231-266, 304-310): a node entered under a split at bit p becomes a leaf when
-// count <= 1, p == 0 or node >= 2^(depth-1), with axis (31 - p) % 3; a node
-// whose codes are all equal is a leaf with axis 1; an internal node split at
-// bit p has axis (30 - p) % 3.
+// Stage 4a: split tables.  For scene s with t = tbits, T[w] = first sorted
+// position whose code is >= w << (30 - t), for w in [0, 2^t]; a split search
+// for a node then starts inside one bucket.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_emit_level(
-    const uint32_t *__restrict__ codes, const int4 *__restrict__ fin, int4 *__restrict__ fout,
-    uint32_t *__restrict__ counters, int level, uint32_t depth, rt_node32 *__restrict__ heap,
-    TravNode *__restrict__ trav, int32_t *__restrict__ trav_parent, uint32_t *__restrict__ flags,
-    int4 *__restrict__ leaves, SceneHdr *__restrict__ hdr)
+__global__ __launch_bounds__(256) void k_split_table(const SceneDesc *__restrict__ desc, uint32_t S,
+                                                     uint64_t total, const uint32_t *__restrict__ codes,
+                                                     uint32_t *__restrict__ table)
 {
-    const uint32_t n_in = counters[level];
-    const uint32_t leaf_level = 1u << (depth - 1);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t b = blockIdx.x * blockDim.x; b < n_in; b += stride) {
-        const uint32_t i = b + threadIdx.x;
-        const bool active = i < n_in;
-        int4 e = active ? fin[i] : make_int4(0, 0, 0, 0);
-        const uint32_t h = (uint32_t)e.x;
-        const int32_t first = e.y, cnt = e.z, pr = e.w;
-        const int32_t last = first + cnt - 1;
-        const int pp = pr < 0 ? 31 : (pr & 31);
-        bool leaf = false, internal = false;
-        int axis = 0, split_bit = 0;
-        uint32_t x = 0;
-        if (active) {
-            const bool at_entry = cnt <= 1 || h >= leaf_level || pp == 0;
-            if (at_entry) {
-                leaf = true;
-                axis = (31 - pp) % 3;
-            } else {
-                x = codes[first] ^ codes[last];
-                if (x == 0u) {
-                    leaf = true;
-                    axis = 1;
-                } else {
-                    internal = true;
-                    split_bit = 31 - __clz(x);
-                    axis = (30 - split_bit) % 3;
-                }
-            }
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = upper_scene(S, (uint32_t)e, [&](uint32_t i) { return (uint32_t)desc[i].table_off; });
+        const SceneDesc D = desc[s];
+        const uint32_t w = (uint32_t)(e - D.table_off);
+        const uint64_t target = (uint64_t)w << (30 - D.tbits);
+        uint32_t lo = D.prim_off, hi = D.prim_off + D.nt;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if ((uint64_t)codes[mid] < target) lo = mid + 1;
+            else hi = mid;
         }
-        int32_t s = 0;
-        if (internal) {
-            int32_t lo = first, hi = last;
-            while (hi - lo > 1) {
-                const int32_t mid = (lo + hi) >> 1;
-                if ((codes[mid] >> split_bit) & 1u) hi = mid;
-                else lo = mid;
-            }
-            s = hi;
+        table[e] = lo;
+    }
+}
+
+// First sorted position in (first, last] whose code is >= X (the split of a
+// node spanning [first, last] at bit p, X = ((code_first >> p) | 1) << p).
+__device__ __forceinline__ uint32_t split_search(const uint32_t *__restrict__ codes,
+                                                 const uint32_t *__restrict__ table,
+                                                 const SceneDesc &D, uint32_t first, uint32_t last,
+                                                 uint32_t X)
+{
+    const uint32_t w = X >> (30 - D.tbits);
+    uint32_t lo = table[D.table_off + w], hi = table[D.table_off + w + 1];
+    lo = lo < first + 1 ? first + 1 : lo;
+    hi = hi > last ? last : hi;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (codes[mid] < X) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 4b: big nodes (LBVH_Build, LBVH.hlsl:134-226, for nodes holding more
+// than kSubtreeMax primitives).  One block per scene walks the big frontier
+// level by level.  Entries are {heap index, first sorted position, count,
+// parent} with parent = -1 for the root, else (k << 6) | (side << 5) | split
+// bit of the parent.  Leaves and small nodes go to the subtree list.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTopThreads) void k_top(
+    const SceneDesc *__restrict__ desc, SceneHdr *__restrict__ hdr, const uint32_t *__restrict__ codes,
+    const uint32_t *__restrict__ table, int4 *__restrict__ big_a, int4 *__restrict__ big_b,
+    int4 *__restrict__ small, uint32_t *__restrict__ counters, int4 *__restrict__ top_nodes,
+    uint32_t *__restrict__ top_counts, rt_node32 *__restrict__ heap, TravNode *__restrict__ trav)
+{
+    __shared__ uint32_t s_n, s_next, s_ntop;
+    const uint32_t s = blockIdx.x;
+    const SceneDesc D = desc[s];
+    const int tid = threadIdx.x;
+    if (D.nt == 0) {
+        if (tid == 0) {
+            hdr[s].root_ref = -1;
+            hdr[s].levels = 0;
         }
-        const uint32_t child_slot = wave_append(&counters[level + 1], internal, 2u);
-        const uint32_t leaf_slot = wave_append(&counters[kCntLeaves], leaf, 1u);
-        if (!active) continue;
-        rt_node32 *hn = &heap[h];
-        int32_t my_ref;
-        if (leaf) {
-            hn->prim_count = (cnt << 3) | 4 | axis;
-            hn->prim_pos = first;
-            my_ref = (int32_t)(kLeafBit | (uint32_t)first);
-            leaves[leaf_slot] = make_int4((int)h, first, cnt, pr);
-            if (h < leaf_level) {  // mark the two unused child slots, as LBVH.hlsl:166-176
-                heap[2 * h].prim_count = kInvalid;
-                heap[2 * h].prim_pos = kInvalid;
-                heap[2 * h + 1].prim_count = kInvalid;
-                heap[2 * h + 1].prim_pos = kInvalid;
-            }
+        return;
+    }
+    const uint32_t leaf_level = 1u << (D.depth - 1);
+    rt_node32 *sheap = heap + D.heap_off;
+    int4 *cur = big_a + D.prim_off, *nxt = big_b + D.prim_off;
+    int4 *tops = top_nodes + D.prim_off;
+    uint32_t *tcounts = top_counts + (size_t)s * kMaxLevels;
+    if (tid == 0) {
+        const int4 root = make_int4(1, (int)D.prim_off, (int)D.nt, -1);
+        if (D.nt > (uint32_t)kSubtreeMax) {
+            cur[0] = root;
+            s_n = 1;
         } else {
-            const int32_t k = s - 1;
-            hn->prim_count = (cnt << 3) | axis;
-            hn->prim_pos = first;
-            my_ref = k;
+            small[atomicAdd(&counters[kCntSmall], 1u)] = root;
+            s_n = 0;
+        }
+        s_ntop = 0;
+        s_next = 0;
+    }
+    __syncthreads();
+    uint32_t level = 0;
+    while (true) {
+        const uint32_t n = s_n;
+        if (n == 0) break;
+        const uint32_t level_start = s_ntop;
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += kTopThreads) {
+            const int4 e = cur[i];
+            const uint32_t h = (uint32_t)e.x, first = (uint32_t)e.y;
+            const int32_t cnt = e.z, pr = e.w;
+            const uint32_t last = first + cnt - 1;
+            const int pp = pr < 0 ? 31 : (pr & 31);
+            const uint32_t cf = codes[first], cl = codes[last];
+            int axis = 0, p = 0;
+            if (classify(h, cnt, pp, leaf_level, cf, cl, &axis, &p)) {
+                small[atomicAdd(&counters[kCntSmall], 1u)] = e;
+                continue;
+            }
+            const uint32_t X = ((cf >> p) | 1u) << p;
+            const uint32_t sp = split_search(codes, table, D, first, last, X);
+            const int32_t k = (int32_t)sp - 1;
+            write_heap_word(&sheap[h], (cnt << 3) | axis, (int32_t)(first - D.prim_off));
             TravNode *tn = &trav[k];
             tn->axis = axis;
             tn->heap = (int32_t)h;
-            trav_parent[k] = pr;
-            flags[k] = 0u;
-            fout[child_slot] = make_int4((int)(2 * h), first, s - first, (k << 6) | split_bit);
-            fout[child_slot + 1] =
-                make_int4((int)(2 * h + 1), s, last - s + 1, (k << 6) | (1 << 5) | split_bit);
+            if (pr < 0) hdr[s].root_ref = k;
+            else trav[pr >> 6].ref[(pr >> 5) & 1] = k;
+            tops[atomicAdd(&s_ntop, 1u)] = make_int4(k, (int)h, pr, (int)first);
+            const int32_t cl_cnt = (int32_t)(sp - first), cr_cnt = (int32_t)(last - sp + 1);
+            const int4 c0 = make_int4((int)(2 * h), (int)first, cl_cnt, (k << 6) | p);
+            const int4 c1 = make_int4((int)(2 * h + 1), (int)sp, cr_cnt, (k << 6) | (1 << 5) | p);
+            if (cl_cnt > kSubtreeMax) nxt[atomicAdd(&s_next, 1u)] = c0;
+            else small[atomicAdd(&counters[kCntSmall], 1u)] = c0;
+            if (cr_cnt > kSubtreeMax) nxt[atomicAdd(&s_next, 1u)] = c1;
+            else small[atomicAdd(&counters[kCntSmall], 1u)] = c1;
         }
-        if (pr < 0) hdr->root_ref = my_ref;
-        else trav[pr >> 6].ref[(pr >> 5) & 1] = my_ref;
+        __syncthreads();
+        if (tid == 0) {
+            tcounts[level] = s_ntop - level_start;
+            s_n = s_next;
+            s_next = 0;
+        }
+        int4 *t = cur;
+        cur = nxt;
+        nxt = t;
+        ++level;
+        __syncthreads();
+    }
+    if (tid == 0) hdr[s].levels = level;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 4c: small subtrees.  A persistent block takes one entry of the
+// subtree list at a time.  A leaf (of any size) is reduced by the whole block;
+// an internal node (<= kSubtreeMax primitives) has its codes staged in LDS,
+// its subtree emitted level by level with the same rules as k_top, then
+// refitted bottom-up (LBVH_BuildAABB, LBVH.hlsl:228-270) from LDS child boxes.
+// ---------------------------------------------------------------------------
+struct SubtreeLds {
+    uint32_t codes[kSubtreeMax];
+    int4 fr[2][kSubtreeMax];
+    int4 inode[kSubtreeMax];      // k, heap index, parent, first
+    int32_t iword[kSubtreeMax];   // (count << 2) | axis
+    float cbox[kSubtreeMax][12];  // child boxes by local internal index
+    int32_t cref[kSubtreeMax][2];
+    uint32_t lvl[kMaxLevels + 2];
+    float red[kSubtreeThreads / 64 * 6];
+    uint32_t item, n, next, nint;
+};
+
+__global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
+    const SceneDesc *__restrict__ desc, uint32_t S, SceneHdr *__restrict__ hdr,
+    const uint32_t *__restrict__ codes, const int32_t *__restrict__ vals,
+    const PrimBox *__restrict__ pbox, const float *__restrict__ xyz,
+    const uint32_t *__restrict__ idx, const int4 *__restrict__ small, uint32_t *__restrict__ counters,
+    rt_node32 *__restrict__ heap, TravNode *__restrict__ trav, TravTri *__restrict__ tris)
+{
+    __shared__ SubtreeLds L;
+    const int tid = threadIdx.x;
+    const uint32_t nsmall = counters[kCntSmall];
+    while (true) {
+        if (tid == 0) L.item = atomicAdd(&counters[kCntWork], 1u);
+        __syncthreads();
+        const uint32_t item = L.item;
+        __syncthreads();
+        if (item >= nsmall) break;
+        const int4 root = small[item];
+        const uint32_t rfirst = (uint32_t)root.y;
+        const int32_t rcnt = root.z, rpr = root.w;
+        const uint32_t s = scene_of_prim(desc, S, rfirst);
+        const SceneDesc D = desc[s];
+        const uint32_t leaf_level = 1u << (D.depth - 1);
+        rt_node32 *sheap = heap + D.heap_off;
+        const uint32_t rlast = rfirst + rcnt - 1;
+        int raxis = 0, rp = 0;
+        const int pp0 = rpr < 0 ? 31 : (rpr & 31);
+        const bool root_leaf =
+            classify((uint32_t)root.x, rcnt, pp0, leaf_level, codes[rfirst], codes[rlast], &raxis, &rp);
+        if (root_leaf) {
+            // one leaf, reduced by the whole block
+            float lo[3] = {INFINITY, INFINITY, INFINITY};
+            float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int32_t j = tid; j < rcnt; j += kSubtreeThreads)
+                leaf_prim(D, vals, pbox, xyz, idx, tris, rfirst + j, j, rcnt, (uint32_t)root.x, lo, hi);
+            block_box_reduce<kSubtreeThreads / 64>(lo, hi, L.red);
+            if (tid == 0) {
+                const uint32_t h = (uint32_t)root.x;
+                write_heap(&sheap[h], lo, hi, (rcnt << 3) | 4 | raxis, (int32_t)(rfirst - D.prim_off));
+                if (h < leaf_level) {
+                    write_heap_word(&sheap[2 * h], kInvalid, kInvalid);
+                    write_heap_word(&sheap[2 * h + 1], kInvalid, kInvalid);
+                }
+                const int32_t ref = (int32_t)(kLeafBit | rfirst);
+                if (rpr < 0) {
+                    hdr[s].root_ref = ref;
+                    for (int k = 0; k < 3; ++k) {
+                        hdr[s].root_box[k] = lo[k];
+                        hdr[s].root_box[3 + k] = hi[k];
+                    }
+                } else {
+                    TravNode *pn = &trav[rpr >> 6];
+                    const int side = (rpr >> 5) & 1;
+                    pn->ref[side] = ref;
+                    for (int k = 0; k < 3; ++k) {
+                        pn->box[side * 6 + k] = lo[k];
+                        pn->box[side * 6 + 3 + k] = hi[k];
+                    }
+                }
+            }
+            __syncthreads();
+            continue;
+        }
+        // internal subtree root: rcnt <= kSubtreeMax
+        for (int32_t j = tid; j < rcnt; j += kSubtreeThreads) L.codes[j] = codes[rfirst + j];
+        if (tid == 0) {
+            L.fr[0][0] = root;
+            L.n = 1;
+            L.next = 0;
+            L.nint = 0;
+            L.lvl[0] = 0;
+        }
+        __syncthreads();
+        int cur = 0;
+        uint32_t level = 0;
+        while (true) {
+            const uint32_t n = L.n;
+            if (n == 0) break;
+            for (uint32_t i = tid; i < n; i += kSubtreeThreads) {
+                const int4 e = L.fr[cur][i];
+                const uint32_t h = (uint32_t)e.x, first = (uint32_t)e.y;
+                const int32_t cnt = e.z, pr = e.w;
+                const uint32_t lf = first - rfirst, ll = lf + cnt - 1;
+                const int pp = pr < 0 ? 31 : (pr & 31);
+                int axis = 0, p = 0;
+                if (classify(h, cnt, pp, leaf_level, L.codes[lf], L.codes[ll], &axis, &p)) {
+                    float lo[3] = {INFINITY, INFINITY, INFINITY};
+                    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                    for (int32_t j = 0; j < cnt; ++j)
+                        leaf_prim(D, vals, pbox, xyz, idx, tris, first + j, j, cnt, h, lo, hi);
+                    write_heap(&sheap[h], lo, hi, (cnt << 3) | 4 | axis, (int32_t)(first - D.prim_off));
+                    if (h < leaf_level) {
+                        write_heap_word(&sheap[2 * h], kInvalid, kInvalid);
+                        write_heap_word(&sheap[2 * h + 1], kInvalid, kInvalid);
+                    }
+                    // the root of this subtree is internal, so a leaf's parent is local
+                    const uint32_t ploc = (uint32_t)(pr >> 6) - rfirst;
+                    const int side = (pr >> 5) & 1;
+                    for (int k = 0; k < 3; ++k) {
+                        L.cbox[ploc][side * 6 + k] = lo[k];
+                        L.cbox[ploc][side * 6 + 3 + k] = hi[k];
+                    }
+                    L.cref[ploc][side] = (int32_t)(kLeafBit | first);
+                    continue;
+                }
+                const uint32_t X = ((L.codes[lf] >> p) | 1u) << p;
+                uint32_t lo = lf + 1, hi = ll;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (L.codes[mid] < X) lo = mid + 1;
+                    else hi = mid;
+                }
+                const uint32_t sp = rfirst + lo;
+                const int32_t k = (int32_t)sp - 1;
+                const uint32_t slot = atomicAdd(&L.nint, 1u);
+                L.inode[slot] = make_int4(k, (int)h, pr, (int)first);
+                L.iword[slot] = (cnt << 2) | axis;
+                if (level == 0) {
+                    if (pr < 0) hdr[s].root_ref = k;
+                    else trav[pr >> 6].ref[(pr >> 5) & 1] = k;
+                } else {
+                    L.cref[(uint32_t)(pr >> 6) - rfirst][(pr >> 5) & 1] = k;
+                }
+                const uint32_t c = atomicAdd(&L.next, 2u);
+                L.fr[cur ^ 1][c] = make_int4((int)(2 * h), (int)first, (int32_t)(sp - first), (k << 6) | p);
+                L.fr[cur ^ 1][c + 1] = make_int4((int)(2 * h + 1), (int)sp,
+                                                 (int32_t)(first + cnt - sp), (k << 6) | (1 << 5) | p);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                L.lvl[level + 1] = L.nint;
+                L.n = L.next;
+                L.next = 0;
+            }
+            cur ^= 1;
+            ++level;
+            __syncthreads();
+        }
+        // refit bottom-up; level l holds inode[lvl[l] .. lvl[l+1])
+        for (int32_t l = (int32_t)level - 1; l >= 0; --l) {
+            for (uint32_t i = L.lvl[l] + tid; i < L.lvl[l + 1]; i += kSubtreeThreads) {
+                const int4 nd = L.inode[i];
+                const int32_t k = nd.x, pr = nd.z;
+                const uint32_t loc = (uint32_t)k - rfirst;
+                const int32_t word = L.iword[i];
+                const float *cb = L.cbox[loc];
+                float lo[3], hi[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    lo[c] = fminf(cb[c], cb[6 + c]);
+                    hi[c] = fmaxf(cb[3 + c], cb[9 + c]);
+                }
+                float4 *tp = reinterpret_cast<float4 *>(&trav[k]);
+                tp[0] = make_float4(cb[0], cb[1], cb[2], cb[3]);
+                tp[1] = make_float4(cb[4], cb[5], cb[6], cb[7]);
+                tp[2] = make_float4(cb[8], cb[9], cb[10], cb[11]);
+                reinterpret_cast<int4 *>(tp)[3] =
+                    make_int4(L.cref[loc][0], L.cref[loc][1], word & 3, nd.y);
+                write_heap(&sheap[nd.y], lo, hi, ((word >> 2) << 3) | (word & 3),
+                           (int32_t)((uint32_t)nd.w - D.prim_off));
+                if (l == 0) {
+                    if (pr < 0) {
+                        for (int c = 0; c < 3; ++c) {
+                            hdr[s].root_box[c] = lo[c];
+                            hdr[s].root_box[3 + c] = hi[c];
+                        }
+                    } else {
+                        TravNode *pn = &trav[pr >> 6];
+                        const int side = (pr >> 5) & 1;
+                        for (int c = 0; c < 3; ++c) {
+                            pn->box[side * 6 + c] = lo[c];
+                            pn->box[side * 6 + 3 + c] = hi[c];
+                        }
+                    }
+                } else {
+                    const uint32_t ploc = (uint32_t)(pr >> 6) - rfirst;
+                    const int side = (pr >> 5) & 1;
+                    for (int c = 0; c < 3; ++c) {
+                        L.cbox[ploc][side * 6 + c] = lo[c];
+                        L.cbox[ploc][side * 6 + 3 + c] = hi[c];
+                    }
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Stage 5: refit (LBVH_BuildAABB, LBVH.hlsl:228-270) bottom-up from the leaves.
-// Each leaf box is the min/max of its primitives' boxes; at every internal
-// node the second child to arrive (device-scope atomic counter) merges both
-// children and climbs on, so each node is written exactly once.  Min/max is
-// exact, so the boxes equal the CPU builder's (LBVH.cpp:200-206, 312-313).
-// The leaf pass also lays the triangles out in sorted order for traversal.
+// Stage 5: refit of the big nodes, bottom-up, one block per scene.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_refit(
-    const int4 *__restrict__ leaves, const uint32_t *__restrict__ counters,
-    const int32_t *__restrict__ ids, const PrimBox *__restrict__ pbox,
-    const float *__restrict__ xyz, const uint32_t *__restrict__ idx, rt_node32 *heap,
-    TravNode *trav, const int32_t *__restrict__ trav_parent, uint32_t *flags,
-    TravTri *__restrict__ tris, SceneHdr *hdr)
+__global__ __launch_bounds__(kTopThreads) void k_top_refit(
+    const SceneDesc *__restrict__ desc, SceneHdr *__restrict__ hdr,
+    const int4 *__restrict__ top_nodes, const uint32_t *__restrict__ top_counts,
+    rt_node32 *__restrict__ heap, TravNode *trav)
 {
-    const uint32_t nleaves = counters[kCntLeaves];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nleaves; i += stride) {
-        const int4 e = leaves[i];
-        const int32_t h = e.x, first = e.y, cnt = e.z;
-        int32_t pr = e.w;
-        float lo[3] = {INFINITY, INFINITY, INFINITY};
-        float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int32_t j = 0; j < cnt; ++j) {
-            const int32_t id = ids[first + j];
-            const PrimBox b = pbox[id];
+    __shared__ uint32_t s_off[kMaxLevels + 1];
+    const uint32_t s = blockIdx.x;
+    const SceneDesc D = desc[s];
+    if (D.nt == 0) return;
+    const uint32_t levels = hdr[s].levels;
+    if (threadIdx.x == 0) {
+        uint32_t o = 0;
+        for (uint32_t l = 0; l < levels; ++l) {
+            s_off[l] = o;
+            o += top_counts[(size_t)s * kMaxLevels + l];
+        }
+        s_off[levels] = o;
+    }
+    __syncthreads();
+    const int4 *tops = top_nodes + D.prim_off;
+    rt_node32 *sheap = heap + D.heap_off;
+    for (int32_t l = (int32_t)levels - 1; l >= 0; --l) {
+        for (uint32_t i = s_off[l] + threadIdx.x; i < s_off[l + 1]; i += kTopThreads) {
+            const int4 nd = tops[i];
+            const int32_t k = nd.x, pr = nd.z;
+            const TravNode *tn = &trav[k];
+            float lo[3], hi[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = fminf(lo[k], b.mn[k]);
-                hi[k] = fmaxf(hi[k], b.mx[k]);
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = fminf(tn->box[c], tn->box[6 + c]);
+                hi[c] = fmaxf(tn->box[3 + c], tn->box[9 + c]);
             }
-            const uint32_t ia = idx[3 * (size_t)id + 0], ib = idx[3 * (size_t)id + 1],
-                           ic = idx[3 * (size_t)id + 2];
-            const float ax = xyz[3 * (size_t)ia + 0], ay = xyz[3 * (size_t)ia + 1],
-                        az = xyz[3 * (size_t)ia + 2];
-            TravTri t;
-            t.a = make_float4(ax, ay, az, __int_as_float(id));
-            t.e1 = make_float4(xyz[3 * (size_t)ib + 0] - ax, xyz[3 * (size_t)ib + 1] - ay,
-                               xyz[3 * (size_t)ib + 2] - az, __int_as_float(j == 0 ? cnt : 0));
-            t.e2 = make_float4(xyz[3 * (size_t)ic + 0] - ax, xyz[3 * (size_t)ic + 1] - ay,
-                               xyz[3 * (size_t)ic + 2] - az, __int_as_float(j == 0 ? h : 0));
-            tris[first + j] = t;
-        }
-        rt_node32 *hn = &heap[h];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            hn->mn[k] = lo[k];
-            hn->mx[k] = hi[k];
-        }
-        while (true) {
+            rt_node32 *hn = &sheap[nd.y];
+            for (int c = 0; c < 3; ++c) {
+                hn->mn[c] = lo[c];
+                hn->mx[c] = hi[c];
+            }
             if (pr < 0) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    hdr->root_box[k] = lo[k];
-                    hdr->root_box[3 + k] = hi[k];
+                for (int c = 0; c < 3; ++c) {
+                    hdr[s].root_box[c] = lo[c];
+                    hdr[s].root_box[3 + c] = hi[c];
                 }
-                break;
+            } else {
+                TravNode *pn = &trav[pr >> 6];
+                const int side = (pr >> 5) & 1;
+                for (int c = 0; c < 3; ++c) {
+                    pn->box[side * 6 + c] = lo[c];
+                    pn->box[side * 6 + 3 + c] = hi[c];
+                }
             }
-            const int32_t k = pr >> 6, side = (pr >> 5) & 1;
-            TravNode *tn = &trav[k];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                tn->box[side * 6 + c] = lo[c];
-                tn->box[side * 6 + 3 + c] = hi[c];
-            }
-            __threadfence();
-            const uint32_t old = atomicAdd(&flags[k], 1u);
-            if (old == 0u) break;
-            __threadfence();
-            const int other = 1 - side;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = fminf(lo[c], tn->box[other * 6 + c]);
-                hi[c] = fmaxf(hi[c], tn->box[other * 6 + 3 + c]);
-            }
-            rt_node32 *pn = &heap[tn->heap];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                pn->mn[c] = lo[c];
-                pn->mx[c] = hi[c];
-            }
-            pr = trav_parent[k];
         }
+        __syncthreads();
     }
 }
 
@@ -544,17 +866,9 @@ __global__ void k_transform(const float *__restrict__ rest, float *__restrict__ 
         const float a = m[3 * r + 0] * dx;
         const float b = m[3 * r + 1] * dy;
         const float c = m[3 * r + 2] * dz;
-        const float s = a + b;
-        const float t = s + c;
+        const float sum = a + b;
+        const float t = sum + c;
         xyz[3 * (size_t)i + r] = t + m[9 + r];
-    }
-}
-
-__global__ void k_empty_scene(SceneHdr *hdr)
-{
-    if (threadIdx.x == 0) {
-        hdr->root_ref = -1;
-        hdr->nleaves = 0;
     }
 }
 
@@ -564,59 +878,75 @@ __global__ void k_empty_scene(SceneHdr *hdr)
 uint32_t sort_tiles(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
 uint32_t bounds_blocks(uint32_t n) { return (n + kBoundsTile - 1) / kBoundsTile; }
 
+uint32_t table_bits(uint32_t max_tris)
+{
+    uint32_t b = 0;
+    while (b < 31 && (1u << b) < max_tris) ++b;
+    const int t = (int)b - 1;
+    return (uint32_t)(t < 4 ? 4 : (t > 18 ? 18 : t));
+}
+
 static inline uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
-hipError_t launch_transform(Scene &s, const Xform &x, hipStream_t st)
+hipError_t launch_transform(Batch &b, uint32_t scene, const Xform &x, hipStream_t st)
 {
-    if (s.nv == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_transform, dim3((s.nv + 255) / 256), dim3(256), 0, st, s.xyz_rest, s.xyz,
-                       s.nv, x);
+    const SceneDesc &D = b.desc[scene];
+    if (D.nv == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_transform, dim3((D.nv + 255) / 256), dim3(256), 0, st,
+                       b.xyz_rest + 3 * (size_t)D.vert_off, b.xyz + 3 * (size_t)D.vert_off, D.nv, x);
     return hipGetLastError();
 }
 
-hipError_t launch_build(Scene &s, hipStream_t st, hipEvent_t *ev)
+hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
 {
-    const uint32_t nt = s.nt;
+    const uint32_t S = b.nscenes;
+    const uint32_t n = b.total_tris;
+    const uint32_t tiles = sort_tiles(n);
+    const uint32_t passes = kCodePasses + (S > 1 ? 1 : 0);
+    const uint32_t nzero = kCodePasses * kRadix + kCounters + passes * tiles * kRadix;
     if (ev) hipEventRecord(ev[0], st);
-    if (nt == 0) {
-        hipLaunchKernelGGL(k_empty_scene, dim3(1), dim3(64), 0, st, s.hdr);
-        if (ev)
-            for (int i = 1; i < 6; ++i) hipEventRecord(ev[i], st);
-        return hipGetLastError();
-    }
-    const uint32_t nb = bounds_blocks(nt);
-    const uint32_t tiles = sort_tiles(nt);
-    // hist | counters | status are one contiguous block of words
-    const uint32_t nzero = kSortPasses * kRadix + kCounters + kSortPasses * tiles * kRadix;
-    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(kBoundsThreads), 0, st, s.xyz, s.idx, nt, s.pbox,
-                       s.cent, s.partials, s.hist, nzero);
+    const uint32_t nb = b.total_blocks > 0 ? b.total_blocks : 1;
+    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(kBoundsThreads), 0, st, b.d_desc, S, b.total_blocks,
+                       b.xyz, b.idx, b.pbox, b.cent, b.partials, b.hist, nzero);
     if (ev) hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_morton, dim3(nb), dim3(kBoundsThreads), 0, st, s.cent, nt, s.partials, nb,
-                       s.keys[0], s.vals[0], s.hist, s.hdr, s.frontier[0], s.counters);
+    const int start = (int)(passes & 1);  // the sorted result ends in buffer 0
+    if (b.total_blocks > 0)
+        hipLaunchKernelGGL(k_morton, dim3(b.total_blocks), dim3(kBoundsThreads), 0, st, b.d_desc, S,
+                           b.cent, b.partials, b.keys[start], b.vals[start], b.hist, b.hdr);
     if (ev) hipEventRecord(ev[2], st);
-    for (int p = 0; p < kSortPasses; ++p) {
-        const int src = p & 1, dst = src ^ 1;
-        hipLaunchKernelGGL(k_onesweep, dim3(tiles), dim3(kSortThreads), 0, st, s.keys[src],
-                           s.vals[src], s.keys[dst], s.vals[dst], nt, (uint32_t)(p * kRadixBits),
-                           s.hist + p * kRadix, s.status + (size_t)p * tiles * kRadix,
-                           s.counters + kCntTiles + p);
+    if (n > 0) {
+        for (uint32_t p = 0; p < passes; ++p) {
+            const int src = (start + p) & 1, dst = src ^ 1;
+            uint32_t *status = b.status + (size_t)p * tiles * kRadix;
+            uint32_t *tc = b.counters + kCntTiles + p;
+            if (p < (uint32_t)kCodePasses)
+                hipLaunchKernelGGL((k_onesweep<false>), dim3(tiles), dim3(kSortThreads), 0, st,
+                                   b.keys[src], b.vals[src], b.keys[dst], b.vals[dst], n,
+                                   p * kRadixBits, b.hist + p * kRadix, b.d_desc, S, status, tc);
+            else
+                hipLaunchKernelGGL((k_onesweep<true>), dim3(tiles), dim3(kSortThreads), 0, st,
+                                   b.keys[src], b.vals[src], b.keys[dst], b.vals[dst], n, 0u,
+                                   b.hist, b.d_desc, S, status, tc);
+        }
     }
     if (ev) hipEventRecord(ev[3], st);
-    for (uint32_t level = 0; level < s.depth; ++level) {
-        const uint32_t width = level < 31 ? min_u32(1u << level, nt) : nt;
-        const uint32_t blocks = min_u32((width + 255) / 256, 2048u);
-        hipLaunchKernelGGL(k_emit_level, dim3(blocks), dim3(256), 0, st, s.keys[0],
-                           s.frontier[level & 1], s.frontier[(level + 1) & 1], s.counters,
-                           (int)level, s.depth, s.heap, s.trav, s.trav_parent, s.flags, s.leaves,
-                           s.hdr);
+    if (b.total_table > 0 && n > 0) {
+        const uint32_t blocks = (uint32_t)min_u32((uint32_t)((b.total_table + 255) / 256), 4096u);
+        hipLaunchKernelGGL(k_split_table, dim3(blocks), dim3(256), 0, st, b.d_desc, S, b.total_table,
+                           b.keys[0], b.table);
     }
+    hipLaunchKernelGGL(k_top, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.keys[0], b.table,
+                       b.big[0], b.big[1], b.small, b.counters, b.top_nodes, b.top_counts, b.heap,
+                       b.trav);
     if (ev) hipEventRecord(ev[4], st);
-    {
-        const uint32_t blocks = min_u32((nt + 255) / 256, 4096u);
-        hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, s.leaves, s.counters, s.vals[0],
-                           s.pbox, s.xyz, s.idx, s.heap, s.trav, s.trav_parent, s.flags, s.tris,
-                           s.hdr);
+    if (n > 0) {
+        const uint32_t blocks = min_u32(2u * (uint32_t)cu_count, (n + 63) / 64);
+        hipLaunchKernelGGL(k_subtree, dim3(blocks > 0 ? blocks : 1), dim3(kSubtreeThreads), 0, st,
+                           b.d_desc, S, b.hdr, b.keys[0], b.vals[0], b.pbox, b.xyz, b.idx, b.small,
+                           b.counters, b.heap, b.trav, b.tris);
     }
+    hipLaunchKernelGGL(k_top_refit, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.top_nodes,
+                       b.top_counts, b.heap, b.trav);
     if (ev) hipEventRecord(ev[5], st);
     return hipGetLastError();
 }

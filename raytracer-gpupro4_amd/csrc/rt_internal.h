@@ -11,21 +11,31 @@
 
 namespace rt {
 
-// Sort geometry: 8-bit digits, four passes over the 30-bit Morton codes.
+// Sort geometry: 8-bit digits, four passes over the 30-bit Morton codes, plus
+// one pass over the scene index when a batch holds more than one scene.
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
-constexpr int kSortPasses = 4;
+constexpr int kCodePasses = 4;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 4;
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kMaxBatchScenes = kRadix;  // the scene pass is one 8-bit digit
 
 // Bounds / Morton kernels: 256 threads x 4 triangles per block.
 constexpr int kBoundsThreads = 256;
 constexpr int kBoundsItems = 4;
 constexpr int kBoundsTile = kBoundsThreads * kBoundsItems;
 
+// Emit: nodes with more than kSubtreeMax primitives are emitted by one block
+// per scene (k_top); every smaller subtree is emitted and refitted inside one
+// block's LDS (k_subtree).
+constexpr int kSubtreeMax = 512;
+constexpr int kSubtreeThreads = 256;
+constexpr int kTopThreads = 1024;
+constexpr int kMaxLevels = 32;
+
 // Leaf references in traversal nodes carry bit 31; the low bits are the first
-// sorted primitive of the leaf.
+// sorted primitive of the leaf (a position in the batch's sorted arrays).
 constexpr uint32_t kLeafBit = 0x80000000u;
 
 constexpr int32_t kInvalid = -2147483647;
@@ -58,48 +68,81 @@ struct PrimBox {
     float mx[3];
 };
 
-// Device-side scene header written by the build.
-struct SceneHdr {
-    int32_t root_ref;     // internal index or kLeafBit | 0; -1 when empty
-    int32_t pad0;
-    float root_box[6];
-    float scene_box[6];
-    uint32_t nleaves;
-    uint32_t pad1[3];
+// Per scene of a batch, device side.
+// Input arrays (vertices, indices) are laid out by capacity; build arrays
+// (boxes, keys, nodes, triangles) by the current triangle counts.
+struct SceneDesc {
+    uint32_t prim_off;     // first triangle / sorted position in the build arrays
+    uint32_t nt;
+    uint32_t in_prim_off;  // first triangle in the index array
+    uint32_t vert_off;     // first vertex in the vertex array
+    uint32_t nv;
+    uint32_t depth;        // heap depth
+    uint32_t tbits;        // bits of the split table
+    uint32_t block_off;    // first bounds block
+    uint32_t nblocks;
+    uint32_t pad;
+    uint64_t heap_off;     // first heap node
+    uint64_t table_off;    // first split-table entry
 };
 
-// Counters region (uint32): [0..31] frontier counts per level,
-// [32] leaf count, [33..36] sort tile counters, [37..63] spare.
-constexpr int kCntLeaves = 32;
-constexpr int kCntTiles = 33;
-constexpr int kCounters = 64;
+// Per scene, written by the build.
+struct SceneHdr {
+    int32_t root_ref;  // internal index or kLeafBit | first; -1 when empty
+    uint32_t levels;   // number of big (k_top) levels
+    float root_box[6];
+    float scene_box[6];
+    uint32_t pad[2];
+};
 
-struct Scene {
-    uint32_t max_tris = 0, max_verts = 0, depth = 0;
-    uint32_t nt = 0, nv = 0;
+// Counters region (uint32): [0] small-subtree count, [1] subtree work
+// counter, [2..2+kCodePasses] sort tile counters, [8..8+kMaxLevels*256) big
+// internal nodes per level per scene are kept in top_counts instead.
+constexpr int kCntSmall = 0;
+constexpr int kCntWork = 1;
+constexpr int kCntTiles = 2;
+constexpr int kCounters = 16;
+
+// A batch: scenes sharing one set of concatenated device arrays, built by one
+// sequence of launches.  A plain scene is a batch of one.
+struct Batch {
+    uint32_t nscenes = 0;
+    std::vector<SceneDesc> desc;      // host copy
+    std::vector<uint32_t> max_tris, max_verts;
+    uint32_t total_tris = 0, total_verts = 0, total_blocks = 0;
+    uint64_t total_heap = 0, total_table = 0;
+    uint32_t max_depth = 0;
     bool built = false;
-    bool have_rest = false;
+    std::vector<uint8_t> rest;       // per scene: rest positions captured
     float *xyz = nullptr;       // current positions (3 floats per vertex)
     float *xyz_rest = nullptr;  // rest positions for rt_scene_transform
-    uint32_t *idx = nullptr;
+    uint32_t *idx = nullptr;    // per-scene vertex indices (3 per triangle)
+    SceneDesc *d_desc = nullptr;
+    SceneHdr *hdr = nullptr;
     PrimBox *pbox = nullptr;
     float *cent = nullptr;      // 3 floats per triangle (3x centroid)
     float *partials = nullptr;  // 6 floats per bounds block
     uint32_t *keys[2] = {nullptr, nullptr};
     int32_t *vals[2] = {nullptr, nullptr};
-    uint32_t *hist = nullptr;      // kSortPasses * kRadix
-    uint32_t *status = nullptr;    // kSortPasses * tiles * kRadix
+    uint32_t *hist = nullptr;      // kCodePasses * kRadix
     uint32_t *counters = nullptr;  // kCounters
-    int4 *frontier[2] = {nullptr, nullptr};
-    int4 *leaves = nullptr;
+    uint32_t *status = nullptr;    // passes * tiles * kRadix
+    uint32_t *table = nullptr;     // split tables
+    int4 *big[2] = {nullptr, nullptr};
+    int4 *small = nullptr;
+    int4 *top_nodes = nullptr;     // big internal nodes, per scene region, by level
+    uint32_t *top_counts = nullptr;  // per scene, kMaxLevels level sizes
     rt_node32 *heap = nullptr;
     TravNode *trav = nullptr;
-    int32_t *trav_parent = nullptr;
-    uint32_t *flags = nullptr;
     TravTri *tris = nullptr;
-    SceneHdr *hdr = nullptr;
     uint64_t bytes = 0;
     std::vector<void *> allocs;
+    int refs = 0;                  // scene handles pointing here
+};
+
+struct SceneRef {
+    Batch *batch = nullptr;
+    uint32_t index = 0;
 };
 
 struct FrameBuffers {
@@ -116,13 +159,14 @@ struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    std::vector<rt::Scene *> scenes;
+    std::vector<rt::SceneRef> scenes;
     rt::FrameBuffers frame;
     unsigned long long *stats = nullptr;  // device counters: 8 x u64
     void *staging = nullptr;              // rays + hits for host-pointer rt_trace
     uint64_t staging_bytes = 0;
     hipEvent_t ev[8] = {};
     uint64_t bytes = 0;
+    int cu_count = 256;
     // profiling window: events come from a pool and are read back at the end
     bool profiling = false;
     std::vector<hipEvent_t> pool;
@@ -137,17 +181,20 @@ struct rt_ctx {
 
 namespace rt {
 
-// Host-side launchers (build.hip / trace.hip).
-hipError_t launch_build(Scene &s, hipStream_t st, hipEvent_t *ev /* 6 or null */);
 struct Xform {
     float m[12];
 };
-hipError_t launch_transform(Scene &s, const Xform &x, hipStream_t st);
-hipError_t launch_trace(const Scene &s, const rt_ray48 *rays, uint32_t n, rt_hit24 *hits,
-                        uint32_t flags, unsigned long long *stats, hipStream_t st);
-hipError_t launch_render_direct(const Scene &s, const rt_render_desc &d, FrameBuffers &fb,
-                                uint32_t flags, unsigned long long *stats, hipStream_t st);
+
+// Host-side launchers (build.hip / trace.hip).
+hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev /* 6 or null */, int cu_count);
+hipError_t launch_transform(Batch &b, uint32_t scene, const Xform &x, hipStream_t st);
+hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, uint32_t n,
+                        rt_hit24 *hits, uint32_t flags, unsigned long long *stats, hipStream_t st);
+hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
+                                FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
+                                hipStream_t st);
 uint32_t sort_tiles(uint32_t n);
 uint32_t bounds_blocks(uint32_t n);
+uint32_t table_bits(uint32_t max_tris);
 
 }  // namespace rt

@@ -361,14 +361,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_direct(SceneView sv, Ren
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-static SceneView view_of(const Scene &s) { return SceneView{s.trav, s.tris, s.hdr}; }
+static SceneView view_of(const Batch &b, uint32_t scene)
+{
+    return SceneView{b.trav, b.tris, b.hdr + scene};
+}
 
-hipError_t launch_trace(const Scene &s, const rt_ray48 *rays, uint32_t n, rt_hit24 *hits,
-                        uint32_t flags, unsigned long long *stats, hipStream_t st)
+hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, uint32_t n,
+                        rt_hit24 *hits, uint32_t flags, unsigned long long *stats, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
-    const SceneView sv = view_of(s);
+    const SceneView sv = view_of(b, scene);
     const bool any = flags & RT_FLAG_ANYHIT, cnt = flags & RT_FLAG_COUNT;
     if (any && cnt) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, st, sv, rays, n, hits, stats);
     else if (any) hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, st, sv, rays, n, hits, stats);
@@ -377,7 +380,8 @@ hipError_t launch_trace(const Scene &s, const rt_ray48 *rays, uint32_t n, rt_hit
     return hipGetLastError();
 }
 
-hipError_t launch_render_direct(const Scene &s, const rt_render_desc &d, FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
+hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
+                                FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 hipStream_t st)
 {
     RenderParams rp;
@@ -391,7 +395,7 @@ hipError_t launch_render_direct(const Scene &s, const rt_render_desc &d, FrameBu
     for (uint32_t i = 0; i < 24; ++i) rp.lights[i] = (i < 3 * d.nlights) ? d.lights[i] : 0.0f;
     const uint32_t tiles = ((d.width + kTileW - 1) / kTileW) * ((d.height + kTileH - 1) / kTileH);
     if (tiles == 0) return hipSuccess;
-    const SceneView sv = view_of(s);
+    const SceneView sv = view_of(b, scene);
     const bool keep = flags & RT_FLAG_KEEP_HITS, cnt = flags & RT_FLAG_COUNT;
     const dim3 grid(tiles), block(kTraceBlock);
     if (keep && cnt)

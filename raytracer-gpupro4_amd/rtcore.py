@@ -111,6 +111,7 @@ def lib() -> C.CDLL:
         "rt_memory_in_use": (I32, [P, C.POINTER(U64)]),
         "rt_scene_create": (I32, [P, U32, U32, U32, C.POINTER(I32)]),
         "rt_scene_destroy": (I32, [P, I32]),
+        "rt_batch_create": (I32, [P, U32, P, P, P, C.POINTER(I32)]),
         "rt_scene_upload": (I32, [P, I32, P, U32, P, U32]),
         "rt_scene_upload_device": (I32, [P, I32, P, U32, P, U32]),
         "rt_scene_transform": (I32, [P, I32, P]),
@@ -263,6 +264,16 @@ class Context:
         self.check(self.L.rt_scene_create(self.h, max_tris, max_verts, depth, C.byref(s)),
                    "rt_scene_create")
         return s.value
+
+    def batch_create(self, max_tris, max_verts, depths) -> list:
+        """Creates a batch of scenes built together; returns their handles."""
+        mt = np.ascontiguousarray(max_tris, np.uint32)
+        mv = np.ascontiguousarray(max_verts, np.uint32)
+        dp = np.ascontiguousarray(depths, np.uint32)
+        first = C.c_int()
+        self.check(self.L.rt_batch_create(self.h, len(mt), _ptr(mt), _ptr(mv), _ptr(dp),
+                                          C.byref(first)), "rt_batch_create")
+        return list(range(first.value, first.value + len(mt)))
 
     def scene_destroy(self, scene: int):
         self.check(self.L.rt_scene_destroy(self.h, scene), "rt_scene_destroy")

@@ -162,3 +162,60 @@ def test_upload_rejects_bad_indices(ctx, rtcore):
         ctx.scene_upload(s, np.zeros((9, 3), np.float32), np.zeros((5, 3), np.uint32))
     assert e.value.code == rtcore.RT_ERR_CAPACITY
     ctx.scene_destroy(s)
+
+
+def test_batch_of_scenes_matches_per_scene_oracle(ctx, oracle, rtcore):
+    """Scenes of uneven sizes (and depths) built by one launch sequence."""
+    rng = np.random.default_rng(77)
+    sizes = [0, 1, 700, 5000, 30000, 123457, 2, 513, 100000]
+    depths = [8, 6, 12, 16, 20, 20, 4, 10, 14]
+    scenes = []
+    for i, n in enumerate(sizes):
+        if n == 0:
+            scenes.append((np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)))
+        elif i % 2:
+            scenes.append(soup(n, sigma=0.01, seed=100 + i))
+        else:
+            scenes.append(rtcore.gen_object_mix(n, seed=200 + i))
+    caps = [max(1, n + int(rng.integers(0, 50))) for n in sizes]
+    vcaps = [max(1, len(x) + 17) for x, _ in scenes]
+    handles = ctx.batch_create(caps, vcaps, depths)
+    print(f"batch of {len(handles)} scenes", flush=True)
+    for h, (xyz, idx) in zip(handles, scenes):
+        ctx.scene_upload(h, xyz, idx)
+    ctx.build(handles[3])
+    for h, (xyz, idx), d in zip(handles, scenes, depths):
+        nodes, codes, ids, box = ctx.read(h)
+        ref = oracle.build_scene(xyz, idx, d)
+        assert np.array_equal(codes, ref["codes"]) and np.array_equal(ids, ref["ids"])
+        if len(idx) == 0:
+            continue
+        assert np.array_equal(box, ref["scene"])
+        reach = oracle.reachable(ref["nodes"])
+        assert np.array_equal(nodes[reach]["prim_count"], ref["nodes"][reach]["prim_count"])
+        assert np.array_equal(nodes[reach]["prim_pos"], ref["nodes"][reach]["prim_pos"])
+        assert np.all(nodes[reach]["mn"] == ref["nodes"][reach]["mn"])
+        assert np.all(nodes[reach]["mx"] == ref["nodes"][reach]["mx"])
+        # traversal inside the batch uses the scene's own root and local ids
+        rays = np.zeros(512, rtcore.RAY_DTYPE)
+        c = xyz.mean(0)
+        rays["o"] = c + rng.normal(size=(512, 3)).astype(np.float32) * 3
+        dvec = (c - rays["o"]) + rng.normal(size=(512, 3)).astype(np.float32) * 0.3
+        rays["d"] = (dvec / np.linalg.norm(dvec, axis=1, keepdims=True)).astype(np.float32)
+        rays["tmax"] = 1e16
+        rays["tri"] = -1
+        g = ctx.trace(h, rays)
+        r = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays)
+        assert np.array_equal(g, r)
+    # rebuilding after a resize of one scene keeps the others intact
+    xyz, idx = soup(4000, seed=999)
+    ctx.scene_upload(handles[2], xyz, idx)
+    ctx.build(handles[0])
+    nodes, codes, ids, box = ctx.read(handles[2])
+    ref = oracle.build_scene(xyz, idx, depths[2])
+    assert np.array_equal(codes, ref["codes"]) and np.array_equal(ids, ref["ids"])
+    nodes, codes, ids, box = ctx.read(handles[5])
+    ref = oracle.build_scene(*scenes[5], depths[5])
+    assert np.array_equal(ids, ref["ids"])
+    for h in handles:
+        ctx.scene_destroy(h)

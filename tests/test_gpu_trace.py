@@ -94,7 +94,14 @@ def test_trace_anyhit_and_dead_rays(ctx, oracle, rtcore, soup262k):
     live = rays["tri"] > -2
     r = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays[live], anyhit=True, nthreads=8)
     assert np.all(g["tri"][~live] == -2)
-    assert np.array_equal(g[live], r)
+    gl = g[live]
+    for f in ("tri", "u", "v", "t", "root"):
+        assert np.array_equal(gl[f], r[f]), f
+    # any-hit stops early; the device tests both child boxes when it enters a
+    # node, so its box counter can include siblings the trail loop never
+    # reaches.  Triangle tests are the same sequence.
+    assert np.array_equal(gl["visited"] >> 16, r["visited"] >> 16)
+    assert np.all((gl["visited"] & 0xFFFF) >= (r["visited"] & 0xFFFF))
 
 
 def test_trace_counts_match_visit_counters(ctx, oracle, rtcore, torus):
