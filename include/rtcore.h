@@ -111,8 +111,38 @@ enum {
     RT_FLAG_ANYHIT = 1u << 0,    /* stop at the first accepted triangle */
     RT_FLAG_COUNT = 1u << 1,     /* count box / triangle tests into stats */
     RT_FLAG_KEEP_HITS = 1u << 2, /* keep per-pixel primary hits + shadow results */
-    RT_FLAG_TIMING = 1u << 3     /* time the call with HIP events (synchronises) */
+    RT_FLAG_TIMING = 1u << 3,    /* time the call with HIP events (synchronises) */
+    RT_FLAG_PERSISTENT = 1u << 4 /* rt_render_jobs: one persistent launch over all tiles */
 };
+
+/* Launch shape of the traversal kernels (rt_autotune_jobs picks one). */
+typedef struct rt_tuning {
+    int32_t block;         /* threads per block: 64, 128 or 256 (tile 16 x block/16) */
+    int32_t lds_stack;     /* stack entries per lane kept in LDS (8..26); deeper ones spill */
+    int32_t blocks_per_cu; /* resident blocks per CU for persistent job launches */
+} rt_tuning;
+
+/* One image of one scene for rt_render_jobs: spp samples per pixel are added
+ * into accum (device memory, width * height * 4 floats: r, g, b, samples).
+ * RT_MODE_DIRECT: primary + shadow rays as rt_render_desc (ambient, albedo,
+ * lights; misses add sky).  RT_MODE_GI: diffuse path tracing, up to
+ * max_bounces cosine-weighted bounces, Lambert albedo, constant sky emitter
+ * (CSComputeColorGI.hlsl:153-230 structure).  Primary rays are jittered in the
+ * pixel; random numbers are PCG32 streams keyed by (seed, frame, sample,
+ * pixel), so a pixel's result does not depend on the other jobs of a launch. */
+enum { RT_MODE_DIRECT = 0, RT_MODE_GI = 1 };
+typedef struct rt_job_desc {
+    int scene;
+    rt_camera cam;
+    uint32_t width, height;
+    uint32_t mode, spp, max_bounces;
+    float albedo, sky, ambient;
+    const float *lights; /* host, 3 floats per light (direct mode), up to 8 */
+    uint32_t nlights;
+    uint64_t seed;
+    uint32_t frame;
+    float *accum;
+} rt_job_desc;
 
 typedef struct rt_ctx rt_ctx;
 
@@ -201,6 +231,35 @@ typedef struct rt_profile {
 } rt_profile;
 int rt_profile_begin(rt_ctx *ctx);
 int rt_profile_end(rt_ctx *ctx, rt_profile *out);
+
+/* ---- batched rendering (no reference counterpart: the reference renders one
+ *      scene per frame, RayTracerCS.cpp:241-311) */
+int rt_tuning_get(rt_ctx *ctx, rt_tuning *t);
+int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t);
+/* Renders all jobs: with RT_FLAG_PERSISTENT one launch whose blocks take tiles
+ * of every job from one queue; without it one launch per job.  Jobs may name
+ * scenes of different batches.  stats (optional, synchronises) reports the
+ * kernel time and ray counts. */
+int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
+                   uint32_t flags, rt_frame_stats *stats);
+/* Times every launch shape on these jobs (into scratch accumulators) and
+ * keeps the fastest in the context.  cache_path (optional) is a text file of
+ * earlier results keyed by the jobs' shape; a hit skips the search. */
+int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
+                     const char *cache_path, rt_tuning *best);
+/* Captures rebuild (optional) + one persistent rt_render_jobs launch of these
+ * jobs as a HIP graph; each rt_graph_launch replays it with frame + n for the
+ * n-th replay.  Jobs must name scenes of the batch of `scene`. */
+int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_desc *jobs,
+                           uint32_t njobs, int *graph);
+int rt_graph_launch(rt_ctx *ctx, int graph);
+int rt_graph_destroy(rt_ctx *ctx, int graph);
+
+/* ---- device buffers for accumulators (Buffer<T> role, DeviceCore/Buffer.hpp:49-77) */
+int rt_device_alloc(rt_ctx *ctx, uint64_t bytes, void **ptr);
+int rt_device_free(rt_ctx *ctx, void *ptr);
+int rt_device_zero(rt_ctx *ctx, void *ptr, uint64_t bytes);
+int rt_device_read(rt_ctx *ctx, void *host, const void *ptr, uint64_t bytes);
 
 /* ---- camera helper (Camera::SetCamera role, Common/Camera.cpp:154-211) */
 /* Fills a pinhole camera at pos looking at target, vertical field of view

@@ -110,6 +110,15 @@ void orc_render_direct(const orc_node *nodes, const int32_t *ids,
                        const float shade[3], float *rgba, uint8_t *occluded,
                        int nthreads);
 
+/* FP64 diffuse path tracer (gi_oracle.c): sums over samples
+ * first_sample .. first_sample + spp - 1 per pixel into out[w*h]; returns the
+ * number of rays traced. */
+uint64_t orc_render_gi(const orc_node *nodes, const int32_t *ids, const float *xyz,
+                       const uint32_t *idx, const float cam[14], uint32_t w, uint32_t h,
+                       uint32_t spp, uint32_t max_bounces, double albedo, double sky,
+                       uint64_t seed, uint32_t frame, uint32_t first_sample, double *out,
+                       int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,9 @@ def lib():
         L.orc_trace.argtypes = [P, P, P, P, P, P, U32, I32, P, I32]
         L.orc_raygen_pinhole.argtypes = [P, U32, U32, P]
         L.orc_render_direct.argtypes = [P, P, P, P, P, P, U32, P, U32, P, P, P, I32]
+        L.orc_render_gi.argtypes = [P, P, P, P, P, U32, U32, U32, U32, C.c_double, C.c_double,
+                                    C.c_uint64, U32, U32, P, I32]
+        L.orc_render_gi.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -144,6 +147,19 @@ def render_direct(nodes, ids, xyz, idx, rays, hits, lights, ambient=0.1, albedo=
                             _p(np.ascontiguousarray(hits, HIT_DTYPE)), n, _p(lights), nl, _p(shade),
                             _p(rgba), _p(occ), nthreads)
     return rgba, occ
+
+
+def render_gi(nodes, ids, xyz, idx, cam14, w, h, spp, max_bounces, albedo, sky, seed, frame=0,
+              first_sample=0, nthreads=1):
+    """FP64 path tracing; returns (per-pixel radiance sums [h, w], rays traced)."""
+    out = np.empty(w * h, np.float64)
+    rays = lib().orc_render_gi(_p(np.ascontiguousarray(nodes, NODE_DTYPE)),
+                               _p(np.ascontiguousarray(ids, np.int32)),
+                               _p(np.ascontiguousarray(xyz, np.float32)),
+                               _p(np.ascontiguousarray(idx, np.uint32)),
+                               _p(np.ascontiguousarray(cam14, np.float32)), w, h, spp, max_bounces,
+                               albedo, sky, seed, frame, first_sample, _p(out), nthreads)
+    return out.reshape(h, w), int(rays)
 
 
 def reachable(nodes: np.ndarray) -> np.ndarray:

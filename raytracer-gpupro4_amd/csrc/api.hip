@@ -271,11 +271,13 @@ int rt_create(rt_ctx **out, int device)
         return RT_ERR_HIP;
     }
     for (auto &e : ctx->ev) hipEventCreate(&e);
-    if (hipMalloc(&ctx->stats, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    hipEventCreateWithFlags(&ctx->jobs_ev, hipEventDisableTiming);
+    if (hipMalloc(&ctx->stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ctx->tile_counter, 256) != hipSuccess) {
         rt_destroy(ctx);
         return RT_ERR_NOMEM;
     }
-    ctx->bytes += 8 * sizeof(unsigned long long);
+    ctx->bytes += 8 * sizeof(unsigned long long) + 256;
     *out = ctx;
     return RT_OK;
 }
@@ -292,8 +294,17 @@ void rt_destroy(rt_ctx *ctx)
     hipFree(ctx->frame.rgba);
     hipFree(ctx->frame.hits);
     hipFree(ctx->frame.occ);
+    for (rt_ctx::Graph &g : ctx->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+        hipFree(g.jobs);
+        hipFree(g.counter);
+    }
     hipFree(ctx->stats);
     hipFree(ctx->staging);
+    hipFree(ctx->tile_counter);
+    hipFree(ctx->jobs_dev);
+    if (ctx->jobs_ev) hipEventDestroy(ctx->jobs_ev);
     for (auto &e : ctx->ev)
         if (e) hipEventDestroy(e);
     for (auto &e : ctx->pool) hipEventDestroy(e);
@@ -542,7 +553,7 @@ static int trace_common(rt_ctx *ctx, Batch *b, uint32_t i, const rt_ray48 *d_ray
     hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
     if (!stats && ctx->profiling) ev = prof_events(ctx, 1, 2);
     if (ev) hipEventRecord(ev[0], st);
-    RT_HIP(ctx, rt::launch_trace(*b, i, d_rays, n, d_hits, flags, ctx->stats, st));
+    RT_HIP(ctx, rt::launch_trace(*b, i, d_rays, n, d_hits, flags, ctx->stats, ctx->tune, st));
     if (ev) hipEventRecord(ev[1], st);
     if (ctx->profiling) ctx->prof_primary += n;
     if (stats) {
@@ -620,7 +631,7 @@ int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, uint32_
     hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
     if (!stats && ctx->profiling) ev = prof_events(ctx, 1, 2);
     if (ev) hipEventRecord(ev[0], st);
-    RT_HIP(ctx, rt::launch_render_direct(*b, i, *desc, ctx->frame, flags, ctx->stats, st));
+    RT_HIP(ctx, rt::launch_render_direct(*b, i, *desc, ctx->frame, flags, ctx->stats, ctx->tune, st));
     if (ev) hipEventRecord(ev[1], st);
     if (ctx->profiling) ctx->prof_primary += (uint64_t)desc->width * desc->height;
     if (stats) {
@@ -635,6 +646,351 @@ int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, uint32_
         stats->tri_tests = c[1];
     }
     ctx->frame.nlights = desc->nlights > 0 ? desc->nlights : 1;
+    return RT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Render jobs, tuning, graphs
+// ---------------------------------------------------------------------------
+static int check_tuning(rt_ctx *ctx, const rt_tuning &t)
+{
+    if (t.block != 64 && t.block != 128 && t.block != 256)
+        return fail(ctx, RT_ERR_ARG, "tuning: block must be 64, 128 or 256");
+    if (t.lds_stack < 1 || t.lds_stack > 26) return fail(ctx, RT_ERR_ARG, "tuning: lds_stack in 1..26");
+    if (t.blocks_per_cu < 1 || t.blocks_per_cu > 32)
+        return fail(ctx, RT_ERR_ARG, "tuning: blocks_per_cu in 1..32");
+    return RT_OK;
+}
+
+// The instantiated launch shapes, in the order of rt::tune_index.
+static const rt_tuning kShapes[] = {{64, 8, 4},   {64, 16, 4},  {128, 8, 4},  {128, 12, 4},
+                                    {128, 16, 4}, {128, 26, 4}, {256, 8, 4},  {256, 12, 4}};
+
+int rt_tuning_get(rt_ctx *ctx, rt_tuning *t)
+{
+    if (!ctx || !t) return RT_ERR_ARG;
+    *t = ctx->tune;
+    return RT_OK;
+}
+
+int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t)
+{
+    if (!ctx || !t) return RT_ERR_ARG;
+    int rc = check_tuning(ctx, *t);
+    if (rc) return rc;
+    // snap to the nearest instantiated shape; keep blocks_per_cu as given
+    rt_tuning s = kShapes[rt::tune_index(*t)];
+    s.blocks_per_cu = t->blocks_per_cu;
+    ctx->tune = s;
+    return RT_OK;
+}
+
+// Builds the device job table for jobs; returns the total tile count.
+static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const rt_tuning &t,
+                      void *dst_dev, std::vector<uint32_t> *tile_offs, uint32_t *total)
+{
+    const size_t jb = rt::render_job_bytes();
+    std::vector<unsigned char> host(jb * njobs);
+    uint32_t off = 0;
+    if (tile_offs) tile_offs->clear();
+    for (uint32_t k = 0; k < njobs; ++k) {
+        const rt_job_desc &d = jobs[k];
+        Batch *b;
+        uint32_t i;
+        if (!get_scene(ctx, d.scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "job %u: unknown scene %d", k, d.scene);
+        if (!b->built) return fail(ctx, RT_ERR_STATE, "job %u: scene %d not built", k, d.scene);
+        if (!d.accum || ((uintptr_t)d.accum & 15))
+            return fail(ctx, RT_ERR_ARG, "job %u: accum must be a 16-byte aligned device pointer", k);
+        if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || d.spp == 0 ||
+            d.spp > 65536 || d.max_bounces > 64 || d.nlights > 8 || (d.nlights && !d.lights) ||
+            (d.mode != RT_MODE_DIRECT && d.mode != RT_MODE_GI))
+            return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights or mode", k);
+        uint32_t tiles = 0;
+        rt::fill_render_job(host.data() + jb * k, *b, i, d, t, off, &tiles);
+        if (tile_offs) tile_offs->push_back(off);
+        off += tiles;
+    }
+    if (tile_offs) tile_offs->push_back(off);
+    *total = off;
+    // the previous launch may still read the table: wait for it before overwriting
+    RT_HIP(ctx, hipEventSynchronize(ctx->jobs_ev));
+    RT_HIP(ctx, hipMemcpy(dst_dev, host.data(), jb * njobs, hipMemcpyHostToDevice));
+    return RT_OK;
+}
+
+static int ensure_jobs(rt_ctx *ctx, uint32_t njobs)
+{
+    const uint64_t need = (uint64_t)rt::render_job_bytes() * (njobs ? njobs : 1);
+    if (ctx->jobs_cap >= need) return RT_OK;
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(ctx->jobs_dev);
+    ctx->bytes -= ctx->jobs_cap;
+    ctx->jobs_dev = nullptr;
+    ctx->jobs_cap = 0;
+    RT_HIP(ctx, hipMalloc(&ctx->jobs_dev, need));
+    ctx->jobs_cap = need;
+    ctx->bytes += need;
+    return RT_OK;
+}
+
+static int run_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32_t flags,
+                    const rt_tuning &t, rt_frame_stats *stats, float *ms_out)
+{
+    int rc = ensure_jobs(ctx, njobs);
+    if (rc) return rc;
+    std::vector<uint32_t> offs;
+    uint32_t total = 0;
+    rc = stage_jobs(ctx, jobs, njobs, t, ctx->jobs_dev, &offs, &total);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    const bool persistent = (flags & RT_FLAG_PERSISTENT) != 0;
+    const bool timing = stats != nullptr || ms_out != nullptr || (flags & RT_FLAG_TIMING);
+    if (stats || !ctx->profiling)
+        RT_HIP(ctx, hipMemsetAsync(ctx->stats, 0, 8 * sizeof(unsigned long long), st));
+    hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
+    if (!timing && ctx->profiling) ev = prof_events(ctx, 1, 2);
+    if (ev) hipEventRecord(ev[0], st);
+    if (persistent) {
+        RT_HIP(ctx, rt::launch_render_jobs(ctx->jobs_dev, njobs, total, 0, total, true,
+                                           ctx->tile_counter, nullptr, ctx->stats, t, ctx->cu_count, st));
+    } else {
+        for (uint32_t k = 0; k < njobs; ++k)
+            RT_HIP(ctx, rt::launch_render_jobs(ctx->jobs_dev, njobs, total, offs[k],
+                                               offs[k + 1] - offs[k], false, ctx->tile_counter,
+                                               nullptr, ctx->stats, t, ctx->cu_count, st));
+    }
+    if (ev) hipEventRecord(ev[1], st);
+    RT_HIP(ctx, hipEventRecord(ctx->jobs_ev, st));
+    if (ctx->profiling) {
+        for (uint32_t k = 0; k < njobs; ++k)
+            ctx->prof_primary += (uint64_t)jobs[k].width * jobs[k].height * jobs[k].spp;
+    }
+    if (timing) {
+        unsigned long long c[8] = {};
+        RT_HIP(ctx, hipMemcpyAsync(c, ctx->stats, sizeof(c), hipMemcpyDeviceToHost, st));
+        RT_HIP(ctx, hipStreamSynchronize(st));
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (ms_out) *ms_out = ms;
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            stats->trace_ms = ms;
+            stats->primary_rays = c[4];
+            stats->shadow_rays = c[2];
+            stats->bounce_rays = c[3];
+            stats->box_tests = c[0];
+            stats->tri_tests = c[1];
+        }
+    }
+    return RT_OK;
+}
+
+int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32_t flags,
+                   rt_frame_stats *stats)
+{
+    if (!ctx || (njobs && !jobs)) return fail(ctx, RT_ERR_ARG, "rt_render_jobs: bad argument");
+    if (njobs == 0) return RT_OK;
+    hipSetDevice(ctx->device);
+    return run_jobs(ctx, jobs, njobs, flags, ctx->tune, stats, nullptr);
+}
+
+static std::string tune_key(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs)
+{
+    uint64_t px = 0, tris = 0, bounces = 0, spp = 0, modes = 0;
+    for (uint32_t k = 0; k < njobs; ++k) {
+        px += (uint64_t)jobs[k].width * jobs[k].height;
+        Batch *b;
+        uint32_t i;
+        if (get_scene(ctx, jobs[k].scene, &b, &i)) tris += b->desc[i].nt;
+        bounces += jobs[k].max_bounces;
+        spp += jobs[k].spp;
+        modes |= 1ull << jobs[k].mode;
+    }
+    char buf[256];
+    snprintf(buf, sizeof(buf), "jobs%u_px%llu_tris%llu_spp%llu_b%llu_m%llu_cu%d", njobs,
+             (unsigned long long)px, (unsigned long long)tris, (unsigned long long)spp,
+             (unsigned long long)bounces, (unsigned long long)modes, ctx->cu_count);
+    return buf;
+}
+
+int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const char *cache_path,
+                     rt_tuning *best)
+{
+    if (!ctx || !jobs || njobs == 0) return fail(ctx, RT_ERR_ARG, "rt_autotune_jobs: bad argument");
+    hipSetDevice(ctx->device);
+    const std::string key = tune_key(ctx, jobs, njobs);
+    if (cache_path) {
+        if (FILE *f = fopen(cache_path, "r")) {
+            char k[256];
+            rt_tuning t;
+            while (fscanf(f, "%255s %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu) == 4) {
+                if (key == k && check_tuning(ctx, t) == RT_OK) {
+                    fclose(f);
+                    ctx->tune = t;
+                    if (best) *best = t;
+                    return RT_OK;
+                }
+            }
+            fclose(f);
+        }
+    }
+    // scratch accumulators so the search leaves the callers' buffers alone
+    std::vector<rt_job_desc> tj(jobs, jobs + njobs);
+    uint64_t need = 0;
+    for (const rt_job_desc &d : tj) need += 16ull * d.width * d.height;
+    void *scratch = nullptr;
+    RT_HIP(ctx, hipMalloc(&scratch, need ? need : 16));
+    uint64_t off = 0;
+    for (rt_job_desc &d : tj) {
+        d.accum = reinterpret_cast<float *>(static_cast<char *>(scratch) + off);
+        off += 16ull * d.width * d.height;
+    }
+    rt_tuning bt = ctx->tune;
+    float bms = 1e30f;
+    const int bpcs[] = {2, 4, 8};
+    int rc = RT_OK;
+    for (const rt_tuning &shape : kShapes) {
+        for (int bpc : bpcs) {
+            rt_tuning t = shape;
+            t.blocks_per_cu = bpc;
+            float ms = 0.0f;
+            rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);  // warm
+            if (rc) break;
+            rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);
+            if (rc) break;
+            if (ms < bms) {
+                bms = ms;
+                bt = t;
+            }
+        }
+        if (rc) break;
+    }
+    hipStreamSynchronize(ctx->stream);
+    hipFree(scratch);
+    if (rc) return rc;
+    ctx->tune = bt;
+    if (best) *best = bt;
+    if (cache_path) {
+        if (FILE *f = fopen(cache_path, "a")) {
+            fprintf(f, "%s %d %d %d\n", key.c_str(), bt.block, bt.lds_stack, bt.blocks_per_cu);
+            fclose(f);
+        }
+    }
+    return RT_OK;
+}
+
+int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_desc *jobs,
+                           uint32_t njobs, int *graph)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i) || !graph || (njobs && !jobs))
+        return fail(ctx, RT_ERR_ARG, "rt_graph_capture_frame: bad argument");
+    for (uint32_t k = 0; k < njobs; ++k) {
+        Batch *bk;
+        uint32_t ik;
+        if (!get_scene(ctx, jobs[k].scene, &bk, &ik) || bk != b)
+            return fail(ctx, RT_ERR_ARG, "rt_graph_capture_frame: job %u is not in the batch of scene %d", k, scene);
+    }
+    hipSetDevice(ctx->device);
+    if (!b->built) {  // the job table needs built scenes; build once outside the graph
+        RT_HIP(ctx, rt::launch_build(*b, ctx->stream, nullptr, ctx->cu_count));
+        b->built = true;
+    }
+    rt_ctx::Graph g;
+    const size_t jb = rt::render_job_bytes();
+    RT_HIP(ctx, hipMalloc(&g.jobs, jb * (njobs ? njobs : 1)));
+    RT_HIP(ctx, hipMalloc(&g.counter, 256));
+    RT_HIP(ctx, hipMemset(g.counter, 0, 256));
+    uint32_t total = 0;
+    int rc = stage_jobs(ctx, jobs, njobs, ctx->tune, g.jobs, nullptr, &total);
+    if (rc) {
+        hipFree(g.jobs);
+        hipFree(g.counter);
+        return rc;
+    }
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipStream_t st = ctx->stream;
+    RT_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    if (rebuild) e = rt::launch_build(*b, st, nullptr, ctx->cu_count);
+    // g.counter[0] = replay index (read by the kernel), g.counter[16] = tile queue
+    if (e == hipSuccess)
+        e = rt::launch_render_jobs(g.jobs, njobs, total, 0, total, true, g.counter + 16, g.counter,
+                                   ctx->stats, ctx->tune, ctx->cu_count, st);
+    if (e == hipSuccess) e = rt::launch_bump_counter(g.counter, st);
+    hipError_t e2 = hipStreamEndCapture(st, &g.graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g.graph) hipGraphDestroy(g.graph);
+        hipFree(g.jobs);
+        hipFree(g.counter);
+        return hip_fail(ctx, e, "rt_graph_capture_frame");
+    }
+    *graph = (int)ctx->graphs.size();
+    ctx->graphs.push_back(g);
+    return RT_OK;
+}
+
+int rt_graph_launch(rt_ctx *ctx, int graph)
+{
+    if (!ctx || graph < 0 || graph >= (int)ctx->graphs.size() || !ctx->graphs[graph].exec)
+        return fail(ctx, RT_ERR_ARG, "rt_graph_launch: unknown graph %d", graph);
+    hipSetDevice(ctx->device);
+    hipEvent_t *ev = ctx->profiling ? prof_events(ctx, 1, 2) : nullptr;
+    if (ev) hipEventRecord(ev[0], ctx->stream);
+    RT_HIP(ctx, hipGraphLaunch(ctx->graphs[graph].exec, ctx->stream));
+    if (ev) hipEventRecord(ev[1], ctx->stream);
+    return RT_OK;
+}
+
+int rt_graph_destroy(rt_ctx *ctx, int graph)
+{
+    if (!ctx || graph < 0 || graph >= (int)ctx->graphs.size() || !ctx->graphs[graph].exec)
+        return fail(ctx, RT_ERR_ARG, "rt_graph_destroy: unknown graph %d", graph);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    rt_ctx::Graph &g = ctx->graphs[graph];
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+    hipFree(g.jobs);
+    hipFree(g.counter);
+    g = rt_ctx::Graph();
+    return RT_OK;
+}
+
+int rt_device_alloc(rt_ctx *ctx, uint64_t bytes, void **ptr)
+{
+    if (!ctx || !ptr) return RT_ERR_ARG;
+    hipSetDevice(ctx->device);
+    RT_HIP(ctx, hipMalloc(ptr, bytes ? bytes : 16));
+    return RT_OK;
+}
+
+int rt_device_free(rt_ctx *ctx, void *ptr)
+{
+    if (!ctx) return RT_ERR_ARG;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    RT_HIP(ctx, hipFree(ptr));
+    return RT_OK;
+}
+
+int rt_device_zero(rt_ctx *ctx, void *ptr, uint64_t bytes)
+{
+    if (!ctx || (bytes && !ptr)) return RT_ERR_ARG;
+    hipSetDevice(ctx->device);
+    RT_HIP(ctx, hipMemsetAsync(ptr, 0, bytes, ctx->stream));
+    return RT_OK;
+}
+
+int rt_device_read(rt_ctx *ctx, void *host, const void *ptr, uint64_t bytes)
+{
+    if (!ctx || (bytes && (!host || !ptr))) return RT_ERR_ARG;
+    hipSetDevice(ctx->device);
+    RT_HIP(ctx, hipMemcpyAsync(host, ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RT_OK;
 }
 

@@ -167,6 +167,18 @@ struct rt_ctx {
     hipEvent_t ev[8] = {};
     uint64_t bytes = 0;
     int cu_count = 256;
+    rt_tuning tune{128, 12, 4};
+    uint32_t *tile_counter = nullptr;     // persistent job launches
+    void *jobs_dev = nullptr;             // render-job table
+    uint64_t jobs_cap = 0;
+    hipEvent_t jobs_ev = nullptr;         // last launch that read jobs_dev
+    struct Graph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        void *jobs = nullptr;
+        uint32_t *counter = nullptr;      // frame counter read by the job kernel
+    };
+    std::vector<Graph> graphs;
     // profiling window: events come from a pool and are read back at the end
     bool profiling = false;
     std::vector<hipEvent_t> pool;
@@ -189,10 +201,21 @@ struct Xform {
 hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev /* 6 or null */, int cu_count);
 hipError_t launch_transform(Batch &b, uint32_t scene, const Xform &x, hipStream_t st);
 hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, uint32_t n,
-                        rt_hit24 *hits, uint32_t flags, unsigned long long *stats, hipStream_t st);
+                        rt_hit24 *hits, uint32_t flags, unsigned long long *stats,
+                        const rt_tuning &tune, hipStream_t st);
 hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
-                                hipStream_t st);
+                                const rt_tuning &tune, hipStream_t st);
+size_t render_job_bytes();
+void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_desc &d,
+                     const rt_tuning &t, uint32_t tile_off, uint32_t *tiles);
+hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,
+                              uint32_t first_tile, uint32_t tile_count, bool persistent,
+                              uint32_t *tile_counter, const uint32_t *frame_counter,
+                              unsigned long long *stats, const rt_tuning &tune, int cu_count,
+                              hipStream_t st);
+hipError_t launch_bump_counter(uint32_t *c, hipStream_t st);
+int tune_index(const rt_tuning &t);
 uint32_t sort_tiles(uint32_t n);
 uint32_t bounds_blocks(uint32_t n);
 uint32_t table_bits(uint32_t max_tris);

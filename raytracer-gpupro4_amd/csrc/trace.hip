@@ -1,24 +1,29 @@
 // Ray traversal on gfx950: closest-hit and any-hit over the LBVH, primary-ray
-// generation and direct shading with shadow rays.
+// generation, direct shading with shadow rays and diffuse path tracing.
 //
 // The visit order is the reference's stackless trail traversal
 // (LBVH_IntersectP, LBVH.hlsl:363-440): depth first, at an internal node the
 // child on side dirIsNeg[axis] first, a box culled when T0 > best t or T1 < 0,
 // leaf triangles tested first to last, a hit taken only with t < best t.
 // Here a node holds both child boxes, so both are tested when the node is
-// entered; the second child is kept on a per-lane LDS stack with its T0 and
+// entered; the second child is kept on a per-lane stack with its T0 and
 // culled on pop against the then-current best t.  That is the same decision,
 // at the same point of the sequence, as the trail algorithm makes when it
 // reaches the sibling, so the hits, the tie-breaking and the visit counters
 // (+1 per box, +65536 per triangle) equal the reference's.
 // Float arithmetic is IEEE single with no contraction (-ffp-contract=off):
 // IntersectBox and getIntersection (Intersection.hlsl:18-64) in source order.
+//
+// The stack keeps its top KLDS entries in LDS (one column per lane) and the
+// deeper ones in private memory; KLDS and the block size are launch-time
+// tunables (occupancy against spill traffic).
+#include <cstring>
+
 #include "rt_internal.h"
 
 namespace rt {
 
-constexpr int kTraceBlock = 128;
-constexpr int kStack = 26;  // >= depth - 1 for depth <= 27
+constexpr int kStackMax = 26;  // >= depth - 1 for depth <= 27
 
 struct TraceOut {
     int32_t tri;
@@ -26,6 +31,24 @@ struct TraceOut {
     uint32_t boxes, tris;
     int32_t leaf_heap;
     int32_t pos;  // sorted slot of the hit triangle, -1 when none
+};
+
+template <int BLOCK, int KLDS>
+struct RayStack {
+    uint2 *lds;  // this lane's LDS column: entry j at lds[j * BLOCK]
+    uint2 spill[(kStackMax - KLDS) > 0 ? (kStackMax - KLDS) : 1];
+    int sp;
+    __device__ __forceinline__ void push(uint2 e)
+    {
+        if (KLDS >= kStackMax || sp < KLDS) lds[sp * BLOCK] = e;
+        else spill[sp - KLDS] = e;
+        ++sp;
+    }
+    __device__ __forceinline__ uint2 pop()
+    {
+        --sp;
+        return (KLDS >= kStackMax || sp < KLDS) ? lds[sp * BLOCK] : spill[sp - KLDS];
+    }
 };
 
 __device__ __forceinline__ void box_test(const float *bx, float ox, float oy, float oz, float ix,
@@ -47,14 +70,12 @@ __device__ __forceinline__ void box_test(const float *bx, float ox, float oy, fl
     t1 = b;
 }
 
-// Traverses one ray.  stk points at this lane's first LDS stack slot; slot j
-// is stk[j * kTraceBlock].
-template <bool ANYHIT>
+template <bool ANYHIT, class Stack>
 __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
                                              const TravTri *__restrict__ tris, int32_t root_ref,
                                              const float *root_box, float ox, float oy, float oz,
                                              float dx, float dy, float dz, float tmax, int32_t self,
-                                             uint2 *stk)
+                                             Stack &stk)
 {
     TraceOut r;
     r.tri = -1;
@@ -73,7 +94,7 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
     r.boxes = 1;
     if (t0 > r.t || t1 < 0.0f) return r;
     int32_t cur = root_ref;
-    int sp = 0;
+    stk.sp = 0;
     while (true) {
         if (cur >= 0) {
             const TravNode *n = &nodes[cur];
@@ -96,10 +117,7 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
             const bool f_ok = !(f0 > r.t || f1 < 0.0f);
             const bool s_live = !(s1 < 0.0f);
             if (f_ok) {
-                if (s_live) {
-                    stk[sp * kTraceBlock] = make_uint2((uint32_t)sref, __float_as_uint(s0));
-                    ++sp;
-                }
+                if (s_live) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
                 cur = fref;
                 continue;
             }
@@ -145,9 +163,8 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
         }
         // pop the next live entry
         bool found = false;
-        while (sp > 0) {
-            --sp;
-            const uint2 e = stk[sp * kTraceBlock];
+        while (stk.sp > 0) {
+            const uint2 e = stk.pop();
             if (!(__uint_as_float(e.y) > r.t)) {
                 cur = (int32_t)e.x;
                 found = true;
@@ -181,14 +198,24 @@ __device__ __forceinline__ void count_add(unsigned long long *stats, int slot, u
     if (__lane_id() == 0 && x) atomicAdd(&stats[slot], (unsigned long long)x);
 }
 
-// rt_trace: one lane per ray.
-template <bool ANYHIT, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sv, const rt_ray48 *__restrict__ rays,
-                                                       uint32_t n, rt_hit24 *__restrict__ hits,
-                                                       unsigned long long *stats)
+__device__ __forceinline__ void store_hit(rt_hit24 *dst, const TraceOut &o)
 {
-    __shared__ uint2 stack[kStack * kTraceBlock];
-    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    int2 *hp = reinterpret_cast<int2 *>(dst);
+    hp[0] = make_int2(o.tri, __float_as_int(o.u));
+    hp[1] = make_int2(__float_as_int(o.v), __float_as_int(o.t));
+    hp[2] = make_int2((int32_t)(o.boxes + 65536u * o.tris), o.leaf_heap);
+}
+
+// rt_trace: one lane per ray.
+template <bool ANYHIT, bool COUNT, int BLOCK, int KLDS>
+__global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const rt_ray48 *__restrict__ rays,
+                                                 uint32_t n, rt_hit24 *__restrict__ hits,
+                                                 unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     int32_t root_ref;
     float root_box[6];
     load_root(sv.hdr, root_ref, root_box);
@@ -199,31 +226,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sv, const rt_ra
         // o = r0.xyz, d = (r0.w, r1.x, r1.y), refl = (r1.z, r1.w, r2.x), tmin = r2.y,
         // tmax = r2.z, tri = r2.w
         const int32_t self = __float_as_int(r2.w);
-        rt_hit24 h;
-        if (self <= -2) {
-            h.tri = -2;
-            h.u = -1.0f;
-            h.v = -1.0f;
-            h.t = r2.z;
-            h.visited = 0;
-            h.root = 1;
+        if (self <= -2) {  // dead ray (CSIntersections.hlsl:23,37)
+            TraceOut d;
+            d.tri = -2;
+            d.u = -1.0f;
+            d.v = -1.0f;
+            d.t = r2.z;
+            d.boxes = 0;
+            d.tris = 0;
+            d.leaf_heap = 1;
+            store_hit(&hits[i], d);
         } else {
             const TraceOut o = traverse<ANYHIT>(sv.nodes, sv.tris, root_ref, root_box, r0.x, r0.y,
-                                                r0.z, r0.w, r1.x, r1.y, r2.z, self,
-                                                &stack[threadIdx.x]);
-            h.tri = o.tri;
-            h.u = o.u;
-            h.v = o.v;
-            h.t = o.t;
-            h.visited = (int32_t)(o.boxes + 65536u * o.tris);
-            h.root = o.leaf_heap;
+                                                r0.z, r0.w, r1.x, r1.y, r2.z, self, stk);
+            store_hit(&hits[i], o);
             boxes = o.boxes;
             ntris = o.tris;
         }
-        int2 *hp = reinterpret_cast<int2 *>(&hits[i]);
-        hp[0] = make_int2(h.tri, __float_as_int(h.u));
-        hp[1] = make_int2(__float_as_int(h.v), __float_as_int(h.t));
-        hp[2] = make_int2(h.visited, h.root);
     }
     if (COUNT) {
         count_add(stats, 0, boxes);
@@ -231,13 +250,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneView sv, const rt_ra
     }
 }
 
-// Pinhole primary ray: pixel-centre rule of CSPrimaryRays.hlsl:37-40 with an
-// explicit camera basis, maxT 1e16 and self id -1 as at :49-53.
+// Pinhole ray through (px + 0.5 + jx/2, py + 0.5 + jy/2): the pixel-centre
+// rule of CSPrimaryRays.hlsl:37-40 (and its GI jitter, :27-32) with an
+// explicit camera basis.  jx = jy = 0 gives the pixel centre.
 __device__ __forceinline__ void primary_ray(const rt_camera &c, uint32_t px, uint32_t py, float fw,
-                                            float fh, float &dx, float &dy, float &dz)
+                                            float fh, float jx, float jy, float &dx, float &dy,
+                                            float &dz)
 {
-    const float x = ((float)(2u * px + 1u) - fw) / fw;
-    const float y = -(((float)(2u * py + 1u) - fh) / fh);
+    const float x = (((float)(2u * px + 1u) + jx) - fw) / fw;
+    const float y = -((((float)(2u * py + 1u) + jy) - fh) / fh);
     const float sx = x * c.tan_x, sy = y * c.tan_y;
     const float ax = c.fwd[0] + sx * c.right[0];
     const float ay = c.fwd[1] + sx * c.right[1];
@@ -251,6 +272,71 @@ __device__ __forceinline__ void primary_ray(const rt_camera &c, uint32_t px, uin
     dz = ddz / len;
 }
 
+// Flat unit normal of a sorted triangle record: normalize(cross(E1, E2))
+// (CSComputeColorRT.hlsl:61-77).
+__device__ __forceinline__ void flat_normal(const TravTri &T, float &nx, float &ny, float &nz)
+{
+    const float4 e1 = T.e1, e2 = T.e2;
+    nx = (e1.y * e2.z) - (e1.z * e2.y);
+    ny = (e1.z * e2.x) - (e1.x * e2.z);
+    nz = (e1.x * e2.y) - (e1.y * e2.x);
+    const float nl = sqrtf((nx * nx + ny * ny) + nz * nz);
+    nx = nx / nl;
+    ny = ny / nl;
+    nz = nz / nl;
+}
+
+// Direct term of one primary hit (CSComputeColorRT.hlsl:125-198, white
+// material of alpha 1): shadow ray {o = hit, d = -Ldir, maxT = |hit - L|,
+// self = hit triangle} per light, cast when dot(n, Ldir) * dot(n, d) > 0,
+// traced any-hit from the root (the occlusion predicate of the reference's
+// closest-hit shadow ray).  Returns (ambient + sum max(0, -n'.Ldir)) * albedo.
+template <class Stack>
+__device__ __forceinline__ float direct_term(const SceneView &sv, int32_t root_ref,
+                                             const float *root_box, const TraceOut &h, float ox,
+                                             float oy, float oz, float dx, float dy, float dz,
+                                             const float *lights, uint32_t nlights, float ambient,
+                                             float albedo, Stack &stk, uint32_t &boxes,
+                                             uint32_t &ntris, uint32_t &nshadow, uint8_t *occ_out)
+{
+    float nx, ny, nz;
+    flat_normal(sv.tris[h.pos], nx, ny, nz);
+    float mx = nx, my = ny, mz = nz;  // normal facing the ray (:125-131)
+    if (((mx * dx + my * dy) + mz * dz) > 0.0f) {
+        mx = -mx;
+        my = -my;
+        mz = -mz;
+    }
+    const float hx = ox + h.t * dx, hy = oy + h.t * dy, hz = oz + h.t * dz;
+    float diffuse = 0.0f;
+    for (uint32_t l = 0; l < nlights; ++l) {
+        float lx = hx - lights[3 * l + 0];
+        float ly = hy - lights[3 * l + 1];
+        float lz = hz - lights[3 * l + 2];
+        const float dist = sqrtf((lx * lx + ly * ly) + lz * lz);
+        lx = lx / dist;
+        ly = ly / dist;
+        lz = lz / dist;
+        const float ls = (nx * lx + ny * ly) + nz * lz;
+        const float rs = (nx * dx + ny * dy) + nz * dz;
+        uint8_t o = 2;
+        if ((ls * rs) > 0.0f) {
+            const TraceOut s = traverse<true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz,
+                                              -lx, -ly, -lz, dist, h.tri, stk);
+            boxes += s.boxes;
+            ntris += s.tris;
+            nshadow += 1;
+            o = s.tri < 0 ? 0 : 1;
+            if (s.tri < 0) {
+                const float dd = -((mx * lx + my * ly) + mz * lz);
+                diffuse = diffuse + (dd > 0.0f ? dd : 0.0f);
+            }
+        }
+        if (occ_out) occ_out[l] = o;
+    }
+    return (ambient + diffuse) * albedo;
+}
+
 struct RenderParams {
     rt_camera cam;
     uint32_t width, height, nlights;
@@ -258,96 +344,51 @@ struct RenderParams {
     float lights[24];  // up to 8 point lights
 };
 
-constexpr int kTileW = 16, kTileH = 8;  // 128 pixels per block
+template <int BLOCK>
+struct Tile {
+    static constexpr int W = 16, H = BLOCK / 16;
+};
 
-// Primary + shadow rays + direct term (CSComputeColorRT.hlsl:54-198 with a
-// white material of alpha 1, flat normals, no textures).  Shadow rays are
-// any-hit from the root: occlusion is the same predicate the reference's
-// closest-hit shadow ray from the hit leaf evaluates (any accepted triangle
-// with t < |hit - L|), without the search for the closest one.
-template <bool KEEP, bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void k_render_direct(SceneView sv, RenderParams rp,
-                                                               float4 *__restrict__ rgba,
-                                                               rt_hit24 *__restrict__ hits,
-                                                               uint8_t *__restrict__ occ,
-                                                               unsigned long long *stats)
+// Primary + shadow rays + direct term for one image, one tile per block.
+template <bool KEEP, bool COUNT, int BLOCK, int KLDS>
+__global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderParams rp,
+                                                         float4 *__restrict__ rgba,
+                                                         rt_hit24 *__restrict__ hits,
+                                                         uint8_t *__restrict__ occ,
+                                                         unsigned long long *stats)
 {
-    __shared__ uint2 stack[kStack * kTraceBlock];
-    const uint32_t tiles_x = (rp.width + kTileW - 1) / kTileW;
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
+    const uint32_t tiles_x = (rp.width + TW - 1) / TW;
     const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-    const uint32_t px = tx * kTileW + (threadIdx.x % kTileW);
-    const uint32_t py = ty * kTileH + (threadIdx.x / kTileW);
+    const uint32_t px = tx * TW + (threadIdx.x % TW);
+    const uint32_t py = ty * TH + (threadIdx.x / TW);
     const bool valid = px < rp.width && py < rp.height;
     int32_t root_ref;
     float root_box[6];
     load_root(sv.hdr, root_ref, root_box);
-    uint2 *stk = &stack[threadIdx.x];
     uint32_t boxes = 0, ntris = 0, nshadow = 0;
     if (valid) {
         const uint32_t pix = py * rp.width + px;
         float dx, dy, dz;
-        primary_ray(rp.cam, px, py, (float)rp.width, (float)rp.height, dx, dy, dz);
+        primary_ray(rp.cam, px, py, (float)rp.width, (float)rp.height, 0.0f, 0.0f, dx, dy, dz);
         const float ox = rp.cam.pos[0], oy = rp.cam.pos[1], oz = rp.cam.pos[2];
         const TraceOut h = traverse<false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz, dx,
                                            dy, dz, 10000000000000000.0f, -1, stk);
         boxes += h.boxes;
         ntris += h.tris;
-        if (KEEP) {
-            int2 *hp = reinterpret_cast<int2 *>(&hits[pix]);
-            hp[0] = make_int2(h.tri, __float_as_int(h.u));
-            hp[1] = make_int2(__float_as_int(h.v), __float_as_int(h.t));
-            hp[2] = make_int2((int32_t)(h.boxes + 65536u * h.tris), h.leaf_heap);
-        }
+        if (KEEP) store_hit(&hits[pix], h);
         float color;
         if (h.tri < 0) {
             color = rp.env;
             if (KEEP)
                 for (uint32_t l = 0; l < rp.nlights; ++l) occ[(size_t)pix * rp.nlights + l] = 2;
         } else {
-            const TravTri &T = sv.tris[h.pos];
-            const float4 e1 = T.e1, e2 = T.e2;
-            // NorT = normalize(cross(Edge1, Edge2)) (CSComputeColorRT.hlsl:61-77)
-            float nx = (e1.y * e2.z) - (e1.z * e2.y);
-            float ny = (e1.z * e2.x) - (e1.x * e2.z);
-            float nz = (e1.x * e2.y) - (e1.y * e2.x);
-            const float nl = sqrtf((nx * nx + ny * ny) + nz * nz);
-            nx = nx / nl;
-            ny = ny / nl;
-            nz = nz / nl;
-            float mx = nx, my = ny, mz = nz;  // normal facing the ray (:125-131)
-            if (((mx * dx + my * dy) + mz * dz) > 0.0f) {
-                mx = -mx;
-                my = -my;
-                mz = -mz;
-            }
-            const float hx = ox + h.t * dx, hy = oy + h.t * dy, hz = oz + h.t * dz;
-            float diffuse = 0.0f;
-            for (uint32_t l = 0; l < rp.nlights; ++l) {
-                float lx = hx - rp.lights[3 * l + 0];
-                float ly = hy - rp.lights[3 * l + 1];
-                float lz = hz - rp.lights[3 * l + 2];
-                const float dist = sqrtf((lx * lx + ly * ly) + lz * lz);
-                lx = lx / dist;
-                ly = ly / dist;
-                lz = lz / dist;
-                const float ls = (nx * lx + ny * ly) + nz * lz;
-                const float rs = (nx * dx + ny * dy) + nz * dz;
-                uint8_t o = 2;
-                if ((ls * rs) > 0.0f) {
-                    const TraceOut s = traverse<true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy,
-                                                      hz, -lx, -ly, -lz, dist, h.tri, stk);
-                    boxes += s.boxes;
-                    ntris += s.tris;
-                    nshadow += 1;
-                    o = s.tri < 0 ? 0 : 1;
-                    if (s.tri < 0) {
-                        const float dd = -((mx * lx + my * ly) + mz * lz);
-                        diffuse = diffuse + (dd > 0.0f ? dd : 0.0f);
-                    }
-                }
-                if (KEEP) occ[(size_t)pix * rp.nlights + l] = o;
-            }
-            color = (rp.ambient + diffuse) * rp.albedo;
+            color = direct_term(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz, rp.lights,
+                                rp.nlights, rp.ambient, rp.albedo, stk, boxes, ntris, nshadow,
+                                KEEP ? &occ[(size_t)pix * rp.nlights] : nullptr);
         }
         rgba[pix] = make_float4(color, color, color, 1.0f);
     }
@@ -359,30 +400,258 @@ __global__ __launch_bounds__(kTraceBlock) void k_render_direct(SceneView sv, Ren
 }
 
 // ---------------------------------------------------------------------------
-// Host launchers
+// Render jobs: many images of many scenes in one launch.  Each job renders
+// spp samples per pixel into its accumulation buffer (+= (r, g, b, samples)),
+// either direct lighting or diffuse path tracing.  Tiles of all jobs form one
+// queue; persistent blocks take tiles from an atomic counter until the queue
+// is empty (every block reaches the exit), or one block per tile.
+//
+// Path tracing (CSComputeColorGI.hlsl:153-230 loop structure): per sample a
+// jittered primary ray, then up to max_bounces diffuse bounces with
+// cosine-weighted directions around the face-forward flat normal, throughput
+// *= albedo per bounce, a miss adds throughput * sky.  Random numbers come from
+// PCG32 (O'Neill 2014) with stream = pixel index and a seed derived from the
+// job seed, frame and sample, so a pixel's samples do not depend on which
+// other pixels or jobs share the launch.
+// ---------------------------------------------------------------------------
+struct Pcg32 {
+    uint64_t state, inc;
+};
+
+__device__ __forceinline__ uint32_t pcg_next(Pcg32 &r)
+{
+    const uint64_t old = r.state;
+    r.state = old * 6364136223846793005ULL + r.inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+
+__device__ __forceinline__ Pcg32 pcg_seed(uint64_t seed, uint64_t stream)
+{
+    Pcg32 r;
+    r.state = 0u;
+    r.inc = (stream << 1u) | 1u;
+    pcg_next(r);
+    r.state += seed;
+    pcg_next(r);
+    return r;
+}
+
+__device__ __forceinline__ float pcg_unit(Pcg32 &r)
+{
+    return (float)(pcg_next(r) >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ uint64_t sample_seed(uint64_t seed, uint32_t frame, uint32_t sample)
+{
+    return seed + 0x9E3779B97F4A7C15ull * ((uint64_t)frame * 65536ull + sample + 1ull);
+}
+
+// Cosine-weighted direction around unit n (orthonormal basis of Duff et al.,
+// "Building an Orthonormal Basis, Revisited", JCGT 2017).
+__device__ __forceinline__ void cosine_dir(float nx, float ny, float nz, float r1, float r2,
+                                           float &dx, float &dy, float &dz)
+{
+    const float phi = 6.28318530717958647692f * r1;
+    const float rr = sqrtf(r2);
+    const float lx = rr * cosf(phi), ly = rr * sinf(phi);
+    const float lz = sqrtf(fmaxf(0.0f, 1.0f - r2));
+    const float sign = nz >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sign + nz);
+    const float b = nx * ny * a;
+    const float tx = 1.0f + sign * nx * nx * a, ty = sign * b, tz = -sign * nx;
+    const float bx = b, by = sign + ny * ny * a, bz = -ny;
+    dx = (lx * tx + ly * bx) + lz * nx;
+    dy = (lx * ty + ly * by) + lz * ny;
+    dz = (lx * tz + ly * bz) + lz * nz;
+}
+
+struct RenderJob {
+    const TravNode *nodes;
+    const TravTri *tris;
+    const SceneHdr *hdr;
+    float4 *accum;
+    rt_camera cam;
+    uint32_t width, height, tiles_x, tile_off;
+    uint32_t mode, spp, max_bounces, nlights;
+    float albedo, sky, ambient, pad;
+    uint64_t seed;
+    uint32_t frame, pad2;
+    float lights[24];
+};
+
+template <int BLOCK, int KLDS, bool PERSISTENT>
+__global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restrict__ jobs,
+                                                       uint32_t njobs, uint32_t total_tiles,
+                                                       uint32_t tile_base, uint32_t *tile_counter,
+                                                       const uint32_t *frame_counter,
+                                                       unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    __shared__ uint32_t s_tile;
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
+    uint32_t boxes = 0, ntris = 0, nshadow = 0, nbounce = 0, nprimary = 0;
+    const uint32_t frame_add = frame_counter ? *frame_counter : 0u;
+    uint32_t tile = tile_base + blockIdx.x;
+    while (true) {
+        if (PERSISTENT) {
+            if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
+            __syncthreads();
+            tile = s_tile;
+            __syncthreads();
+        }
+        if (tile >= total_tiles) break;
+        uint32_t lo = 0, hi = njobs;  // last job with tile_off <= tile
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (jobs[mid].tile_off <= tile) lo = mid;
+            else hi = mid;
+        }
+        const RenderJob &J = jobs[lo];
+        const uint32_t lt = tile - J.tile_off;
+        const uint32_t px = (lt % J.tiles_x) * TW + (threadIdx.x % TW);
+        const uint32_t py = (lt / J.tiles_x) * TH + (threadIdx.x / TW);
+        if (px < J.width && py < J.height) {
+            const SceneView sv{J.nodes, J.tris, J.hdr};
+            int32_t root_ref;
+            float root_box[6];
+            load_root(J.hdr, root_ref, root_box);
+            const uint32_t pix = py * J.width + px;
+            const float fw = (float)J.width, fh = (float)J.height;
+            float sum = 0.0f;
+            for (uint32_t s = 0; s < J.spp; ++s) {
+                Pcg32 rng = pcg_seed(sample_seed(J.seed, J.frame + frame_add, s), pix);
+                const float jx = 2.0f * pcg_unit(rng) - 1.0f;
+                const float jy = 2.0f * pcg_unit(rng) - 1.0f;
+                float dx, dy, dz;
+                primary_ray(J.cam, px, py, fw, fh, jx, jy, dx, dy, dz);
+                float ox = J.cam.pos[0], oy = J.cam.pos[1], oz = J.cam.pos[2];
+                if (J.mode == 0) {
+                    const TraceOut h = traverse<false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy,
+                                                       oz, dx, dy, dz, 10000000000000000.0f, -1, stk);
+                    nprimary += 1;
+                    boxes += h.boxes;
+                    ntris += h.tris;
+                    sum += h.tri < 0 ? J.sky
+                                     : direct_term(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz,
+                                                   J.lights, J.nlights, J.ambient, J.albedo, stk,
+                                                   boxes, ntris, nshadow, nullptr);
+                    continue;
+                }
+                float thr = 1.0f, L = 0.0f;
+                int32_t self = -1;
+                for (uint32_t b = 0; b <= J.max_bounces; ++b) {
+                    const TraceOut h = traverse<false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy,
+                                                       oz, dx, dy, dz, 10000000000000000.0f, self, stk);
+                    if (b == 0) nprimary += 1;
+                    else nbounce += 1;
+                    boxes += h.boxes;
+                    ntris += h.tris;
+                    if (h.tri < 0) {
+                        L = L + thr * J.sky;
+                        break;
+                    }
+                    if (b == J.max_bounces) break;
+                    float nx, ny, nz;
+                    flat_normal(sv.tris[h.pos], nx, ny, nz);
+                    if (((nx * dx + ny * dy) + nz * dz) > 0.0f) {
+                        nx = -nx;
+                        ny = -ny;
+                        nz = -nz;
+                    }
+                    ox = ox + h.t * dx;
+                    oy = oy + h.t * dy;
+                    oz = oz + h.t * dz;
+                    const float r1 = pcg_unit(rng), r2 = pcg_unit(rng);
+                    cosine_dir(nx, ny, nz, r1, r2, dx, dy, dz);
+                    thr = thr * J.albedo;
+                    self = h.tri;
+                }
+                sum += L;
+            }
+            float4 a = J.accum[pix];
+            a.x += sum;
+            a.y += sum;
+            a.z += sum;
+            a.w += (float)J.spp;
+            J.accum[pix] = a;
+        }
+        if (!PERSISTENT) break;
+    }
+    count_add(stats, 0, boxes);
+    count_add(stats, 1, ntris);
+    count_add(stats, 2, nshadow);
+    count_add(stats, 3, nbounce);
+    count_add(stats, 4, nprimary);
+}
+
+__global__ void k_reset_counter(uint32_t *c)
+{
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+__global__ void k_bump_counter(uint32_t *c)
+{
+    if (threadIdx.x == 0) *c += 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.  The tuning picks one of the instantiated (block, LDS stack
+// entries) pairs.
 // ---------------------------------------------------------------------------
 static SceneView view_of(const Batch &b, uint32_t scene)
 {
     return SceneView{b.trav, b.tris, b.hdr + scene};
 }
 
+#define RT_TUNED(MACRO)                                  \
+    switch (tune_index(tune)) {                          \
+        case 0: MACRO(64, 8); break;                     \
+        case 1: MACRO(64, 16); break;                    \
+        case 2: MACRO(128, 8); break;                    \
+        case 3: MACRO(128, 12); break;                   \
+        case 4: MACRO(128, 16); break;                   \
+        case 5: MACRO(128, 26); break;                   \
+        case 6: MACRO(256, 8); break;                    \
+        default: MACRO(256, 12); break;                  \
+    }
+
+int tune_index(const rt_tuning &t)
+{
+    if (t.block == 64) return t.lds_stack <= 8 ? 0 : 1;
+    if (t.block == 256) return t.lds_stack <= 8 ? 6 : 7;
+    if (t.lds_stack <= 8) return 2;
+    if (t.lds_stack <= 12) return 3;
+    if (t.lds_stack <= 16) return 4;
+    return 5;
+}
+
 hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, uint32_t n,
-                        rt_hit24 *hits, uint32_t flags, unsigned long long *stats, hipStream_t st)
+                        rt_hit24 *hits, uint32_t flags, unsigned long long *stats,
+                        const rt_tuning &tune, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    const dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
     const SceneView sv = view_of(b, scene);
     const bool any = flags & RT_FLAG_ANYHIT, cnt = flags & RT_FLAG_COUNT;
-    if (any && cnt) hipLaunchKernelGGL((k_trace<true, true>), grid, block, 0, st, sv, rays, n, hits, stats);
-    else if (any) hipLaunchKernelGGL((k_trace<true, false>), grid, block, 0, st, sv, rays, n, hits, stats);
-    else if (cnt) hipLaunchKernelGGL((k_trace<false, true>), grid, block, 0, st, sv, rays, n, hits, stats);
-    else hipLaunchKernelGGL((k_trace<false, false>), grid, block, 0, st, sv, rays, n, hits, stats);
+#define RT_L(B, K)                                                                                 \
+    {                                                                                              \
+        const dim3 grid((n + B - 1) / B), block(B);                                                \
+        if (any && cnt) hipLaunchKernelGGL((k_trace<true, true, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        else if (any) hipLaunchKernelGGL((k_trace<true, false, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        else if (cnt) hipLaunchKernelGGL((k_trace<false, true, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        else hipLaunchKernelGGL((k_trace<false, false, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
+    }
+    RT_TUNED(RT_L)
+#undef RT_L
     return hipGetLastError();
 }
 
 hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
-                                hipStream_t st)
+                                const rt_tuning &tune, hipStream_t st)
 {
     RenderParams rp;
     rp.cam = d.cam;
@@ -393,19 +662,85 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     rp.albedo = d.albedo;
     rp.env = d.env;
     for (uint32_t i = 0; i < 24; ++i) rp.lights[i] = (i < 3 * d.nlights) ? d.lights[i] : 0.0f;
-    const uint32_t tiles = ((d.width + kTileW - 1) / kTileW) * ((d.height + kTileH - 1) / kTileH);
-    if (tiles == 0) return hipSuccess;
     const SceneView sv = view_of(b, scene);
     const bool keep = flags & RT_FLAG_KEEP_HITS, cnt = flags & RT_FLAG_COUNT;
-    const dim3 grid(tiles), block(kTraceBlock);
-    if (keep && cnt)
-        hipLaunchKernelGGL((k_render_direct<true, true>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats);
-    else if (keep)
-        hipLaunchKernelGGL((k_render_direct<true, false>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats);
-    else if (cnt)
-        hipLaunchKernelGGL((k_render_direct<false, true>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats);
-    else
-        hipLaunchKernelGGL((k_render_direct<false, false>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats);
+#define RT_L(B, K)                                                                                 \
+    {                                                                                              \
+        constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                            \
+        const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);             \
+        const dim3 grid(tiles), block(B);                                                          \
+        if (keep && cnt) hipLaunchKernelGGL((k_render_direct<true, true, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        else if (keep) hipLaunchKernelGGL((k_render_direct<true, false, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        else if (cnt) hipLaunchKernelGGL((k_render_direct<false, true, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        else hipLaunchKernelGGL((k_render_direct<false, false, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+    }
+    RT_TUNED(RT_L)
+#undef RT_L
+    return hipGetLastError();
+}
+
+uint32_t job_tile_pixels(const rt_tuning &t) { return (uint32_t)t.block; }
+
+void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_desc &d,
+                     const rt_tuning &t, uint32_t tile_off, uint32_t *tiles)
+{
+    RenderJob *j = static_cast<RenderJob *>(dst);
+    memset(j, 0, sizeof(*j));
+    const uint32_t TW = 16, TH = (uint32_t)t.block / 16;
+    j->nodes = b.trav;
+    j->tris = b.tris;
+    j->hdr = b.hdr + scene;
+    j->accum = reinterpret_cast<float4 *>(d.accum);
+    j->cam = d.cam;
+    j->width = d.width;
+    j->height = d.height;
+    j->tiles_x = (d.width + TW - 1) / TW;
+    j->tile_off = tile_off;
+    j->mode = d.mode == RT_MODE_GI ? 1u : 0u;
+    j->spp = d.spp;
+    j->max_bounces = d.max_bounces;
+    j->nlights = d.nlights;
+    j->albedo = d.albedo;
+    j->sky = d.sky;
+    j->ambient = d.ambient;
+    j->seed = d.seed;
+    j->frame = d.frame;
+    for (uint32_t i = 0; i < 24; ++i) j->lights[i] = (i < 3 * d.nlights) ? d.lights[i] : 0.0f;
+    *tiles = j->tiles_x * ((d.height + TH - 1) / TH);
+}
+
+size_t render_job_bytes() { return sizeof(RenderJob); }
+
+hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,
+                              uint32_t first_tile, uint32_t tile_count, bool persistent,
+                              uint32_t *tile_counter, const uint32_t *frame_counter,
+                              unsigned long long *stats, const rt_tuning &tune, int cu_count,
+                              hipStream_t st)
+{
+    if (total_tiles == 0 || njobs == 0) return hipSuccess;
+    const RenderJob *jobs = static_cast<const RenderJob *>(d_jobs);
+#define RT_L(B, K)                                                                                  \
+    {                                                                                               \
+        if (persistent) {                                                                           \
+            hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, tile_counter);            \
+            uint32_t blocks = (uint32_t)cu_count * (uint32_t)(tune.blocks_per_cu > 0 ? tune.blocks_per_cu : 4); \
+            if (blocks > total_tiles) blocks = total_tiles;                                         \
+            hipLaunchKernelGGL((k_render_jobs<B, K, true>), dim3(blocks), dim3(B), 0, st, jobs, njobs, \
+                               total_tiles, 0u, tile_counter, frame_counter, stats);                \
+        } else {                                                                                    \
+            hipLaunchKernelGGL((k_render_jobs<B, K, false>), dim3(tile_count), dim3(B), 0, st, jobs, \
+                               njobs, first_tile + tile_count, first_tile, tile_counter,            \
+                               frame_counter, stats);                                               \
+        }                                                                                           \
+    }
+    RT_TUNED(RT_L)
+#undef RT_L
+    return hipGetLastError();
+}
+
+hipError_t launch_bump_counter(uint32_t *c, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bump_counter, dim3(1), dim3(64), 0, st, c);
     return hipGetLastError();
 }
 

@@ -29,6 +29,9 @@ FLAG_ANYHIT = 1 << 0
 FLAG_COUNT = 1 << 1
 FLAG_KEEP_HITS = 1 << 2
 FLAG_TIMING = 1 << 3
+FLAG_PERSISTENT = 1 << 4
+MODE_DIRECT = 0
+MODE_GI = 1
 
 # Byte layouts of RayTracerCS/Shaders/Structures.hlsl (Node 32 B, Ray 48 B,
 # Intersection 24 B) as numpy dtypes.
@@ -74,6 +77,21 @@ class RenderDesc(C.Structure):
     _fields_ = [("cam", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("lights", C.POINTER(C.c_float)), ("nlights", C.c_uint32), ("ambient", C.c_float),
                 ("albedo", C.c_float), ("env", C.c_float)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+    def as_tuple(self):
+        return (self.block, self.lds_stack, self.blocks_per_cu)
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_int), ("cam", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("mode", C.c_uint32), ("spp", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("albedo", C.c_float), ("sky", C.c_float), ("ambient", C.c_float),
+                ("lights", C.POINTER(C.c_float)), ("nlights", C.c_uint32), ("seed", C.c_uint64),
+                ("frame", C.c_uint32), ("accum", C.c_void_p)]
 
 
 class FrameStats(C.Structure):
@@ -124,6 +142,17 @@ def lib() -> C.CDLL:
         "rt_frame_read": (I32, [P, P, P, P]),
         "rt_profile_begin": (I32, [P]),
         "rt_profile_end": (I32, [P, C.POINTER(Profile)]),
+        "rt_tuning_get": (I32, [P, C.POINTER(Tuning)]),
+        "rt_tuning_set": (I32, [P, C.POINTER(Tuning)]),
+        "rt_render_jobs": (I32, [P, P, U32, U32, C.POINTER(FrameStats)]),
+        "rt_autotune_jobs": (I32, [P, P, U32, C.c_char_p, C.POINTER(Tuning)]),
+        "rt_graph_capture_frame": (I32, [P, I32, I32, P, U32, C.POINTER(I32)]),
+        "rt_graph_launch": (I32, [P, I32]),
+        "rt_graph_destroy": (I32, [P, I32]),
+        "rt_device_alloc": (I32, [P, U64, C.POINTER(P)]),
+        "rt_device_free": (I32, [P, P]),
+        "rt_device_zero": (I32, [P, P, U64]),
+        "rt_device_read": (I32, [P, P, P, U64]),
         "rt_camera_look_at": (None, [C.POINTER(Camera), P, P, P, C.c_float, C.c_float]),
         "rt_gen_torus": (I32, [C.c_float, C.c_float, U32, U32, C.c_float, U64, P, C.POINTER(U32), P,
                                C.POINTER(U32)]),
@@ -257,6 +286,75 @@ class Context:
         p = Profile()
         self.check(self.L.rt_profile_end(self.h, C.byref(p)), "rt_profile_end")
         return p.as_dict()
+
+    # tuning, jobs, graphs, device buffers
+    def tuning(self):
+        t = Tuning()
+        self.check(self.L.rt_tuning_get(self.h, C.byref(t)), "rt_tuning_get")
+        return t.as_tuple()
+
+    def set_tuning(self, block, lds_stack, blocks_per_cu=4):
+        t = Tuning(block, lds_stack, blocks_per_cu)
+        self.check(self.L.rt_tuning_set(self.h, C.byref(t)), "rt_tuning_set")
+
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self.check(self.L.rt_device_alloc(self.h, nbytes, C.byref(p)), "rt_device_alloc")
+        return p.value
+
+    def free(self, ptr: int):
+        self.check(self.L.rt_device_free(self.h, C.c_void_p(ptr)), "rt_device_free")
+
+    def zero(self, ptr: int, nbytes: int):
+        self.check(self.L.rt_device_zero(self.h, C.c_void_p(ptr), nbytes), "rt_device_zero")
+
+    def read_accum(self, ptr: int, width: int, height: int) -> np.ndarray:
+        out = np.empty((height, width, 4), np.float32)
+        self.check(self.L.rt_device_read(self.h, _ptr(out), C.c_void_p(ptr), out.nbytes), "rt_device_read")
+        return out
+
+    @staticmethod
+    def job(scene, cam, width, height, accum, mode=MODE_GI, spp=1, max_bounces=4, albedo=0.7,
+            sky=1.0, ambient=0.1, lights=None, seed=1, frame=0):
+        """Builds a JobDesc; keeps the light array alive on the object."""
+        j = JobDesc()
+        j.scene, j.cam, j.width, j.height = scene, cam, width, height
+        j.mode, j.spp, j.max_bounces = mode, spp, max_bounces
+        j.albedo, j.sky, j.ambient = albedo, sky, ambient
+        arr = np.ascontiguousarray(np.asarray(lights if lights is not None else [], np.float32).reshape(-1))
+        j._lights = arr
+        j.lights = arr.ctypes.data_as(C.POINTER(C.c_float)) if arr.size else None
+        j.nlights = arr.size // 3
+        j.seed, j.frame, j.accum = seed, frame, accum
+        return j
+
+    def render_jobs(self, jobs, persistent=True, want_stats=True):
+        arr = (JobDesc * len(jobs))(*jobs)
+        st = FrameStats()
+        flags = FLAG_PERSISTENT if persistent else 0
+        self.check(self.L.rt_render_jobs(self.h, arr, len(jobs), flags,
+                                         C.byref(st) if want_stats else None), "rt_render_jobs")
+        return st.as_dict() if want_stats else None
+
+    def autotune(self, jobs, cache_path=None):
+        arr = (JobDesc * len(jobs))(*jobs)
+        t = Tuning()
+        cp = cache_path.encode() if cache_path else None
+        self.check(self.L.rt_autotune_jobs(self.h, arr, len(jobs), cp, C.byref(t)), "rt_autotune_jobs")
+        return t.as_tuple()
+
+    def graph_capture(self, scene, jobs, rebuild=True) -> int:
+        arr = (JobDesc * len(jobs))(*jobs)
+        g = C.c_int()
+        self.check(self.L.rt_graph_capture_frame(self.h, scene, 1 if rebuild else 0, arr, len(jobs),
+                                                 C.byref(g)), "rt_graph_capture_frame")
+        return g.value
+
+    def graph_launch(self, g: int):
+        self.check(self.L.rt_graph_launch(self.h, g), "rt_graph_launch")
+
+    def graph_destroy(self, g: int):
+        self.check(self.L.rt_graph_destroy(self.h, g), "rt_graph_destroy")
 
     # scenes
     def scene_create(self, max_tris: int, max_verts: int, depth: int) -> int:

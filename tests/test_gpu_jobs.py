@@ -1,0 +1,194 @@
+"""GPU render jobs: diffuse path tracing against the FP64 CPU path tracer,
+result invariance across launch modes, mixes and launch shapes, graph replay,
+and the autotuner.
+
+GI tolerance: the device computes in fp32 (device sin/cos, IEEE div/sqrt);
+with the same PCG streams it follows the same paths as the FP64 oracle except
+where a path grazes an edge, so the same-stream image is compared by RMSE of
+the per-pixel mean radiance (bar 0.02 for radiance in [0, 1]) and >= 98 % of
+pixels within 1e-3; the converged comparison (1024 device samples against
+4096 independent FP64 samples) reports the RMSE and bounds it by the Monte
+Carlo noise of both estimates (bar 0.03)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mix(ctx, oracle, rtcore):
+    xyz, idx = rtcore.gen_object_mix(20000, seed=11)
+    s = ctx.add_scene(xyz, idx, 18)
+    ctx.build(s)
+    ref = oracle.build_scene(xyz, idx, 18)
+    c = xyz.mean(0)
+    cam = rtcore.camera_look_at(tuple(c + [0.0, 2.0, -9.0]), tuple(c), (0, 1, 0), 60.0, 64 / 48)
+    yield s, xyz, idx, ref, cam
+    ctx.scene_destroy(s)
+
+
+def gpu_gi(ctx, rtcore, scene, cam, w, h, spp, seed=5, frame=0, persistent=True):
+    acc = ctx.alloc(16 * w * h)
+    ctx.zero(acc, 16 * w * h)
+    st = ctx.render_jobs([ctx.job(scene, cam, w, h, acc, rtcore.MODE_GI, spp, 4, 0.7, 1.0, seed=seed,
+                                  frame=frame)], persistent=persistent)
+    img = ctx.read_accum(acc, w, h)
+    ctx.free(acc)
+    return img, st
+
+
+def test_gi_same_streams_match_fp64(ctx, oracle, rtcore, mix):
+    s, xyz, idx, ref, cam = mix
+    W, H, spp = 64, 48, 4
+    img, st = gpu_gi(ctx, rtcore, s, cam, W, H, spp)
+    assert np.all(img[..., 3] == spp)
+    g = img[..., 0] / spp
+    r, rays = oracle.render_gi(ref["nodes"], ref["ids"], xyz, idx, cam.as_array(), W, H, spp, 4, 0.7,
+                               1.0, 5, nthreads=8)
+    r = r / spp
+    rmse = float(np.sqrt(np.mean((g - r) ** 2)))
+    close = float(np.mean(np.abs(g - r) <= 1e-3))
+    print(f"GI same streams: rays gpu={st['primary_rays'] + st['bounce_rays']} cpu={rays} "
+          f"RMSE={rmse:.5f} within1e-3={close:.4f}", flush=True)
+    assert abs((st["primary_rays"] + st["bounce_rays"]) - rays) <= 0.01 * rays
+    assert rmse < 0.02 and close >= 0.98
+
+
+def test_gi_converged_against_fp64(ctx, oracle, rtcore, mix):
+    s, xyz, idx, ref, cam = mix
+    W, H = 32, 24
+    img, _ = gpu_gi(ctx, rtcore, s, cam, W, H, 1024, seed=77)
+    g = img[..., 0] / 1024
+    r, _ = oracle.render_gi(ref["nodes"], ref["ids"], xyz, idx, cam.as_array(), W, H, 4096, 4, 0.7, 1.0,
+                            77, first_sample=100000, nthreads=16)
+    r = r / 4096
+    rmse = float(np.sqrt(np.mean((g - r) ** 2)))
+    print(f"GI 1024 spp vs FP64 4096 spp (independent samples): RMSE={rmse:.5f}", flush=True)
+    assert rmse < 0.03
+
+
+def test_jobs_mix_and_launch_mode_invariance(ctx, rtcore):
+    """A job's image is the same alone, in a mix of scenes from two batches,
+    per-job launches or one persistent launch, under every launch shape."""
+    a = rtcore.gen_object_mix(5000, seed=1)
+    b = rtcore.gen_triangle_soup(30000, 0.01, seed=2)
+    c = rtcore.gen_torus()
+    hs = ctx.batch_create([len(a[1]), len(b[1])], [len(a[0]), len(b[0])], [16, 18])
+    ctx.scene_upload(hs[0], *a)
+    ctx.scene_upload(hs[1], *b)
+    ctx.build(hs[0])
+    sc = ctx.add_scene(*c, 20)
+    ctx.build(sc)
+    specs = [(hs[0], a, (48, 40), rtcore.MODE_GI, 3), (hs[1], b, (64, 32), rtcore.MODE_DIRECT, 2),
+             (sc, c, (40, 40), rtcore.MODE_GI, 2)]
+    bufs, jobs = [], []
+    for scene, (xyz, _), (w, h), mode, spp in specs:
+        ctr = xyz.mean(0)
+        cam = rtcore.camera_look_at(tuple(ctr + [0.3, 1.0, -3.0]), tuple(ctr), (0, 1, 0), 60.0, w / h)
+        acc = ctx.alloc(16 * w * h)
+        bufs.append((acc, w, h))
+        jobs.append(ctx.job(scene, cam, w, h, acc, mode, spp, 3, 0.7, 1.0, 0.1,
+                            lights=[tuple(ctr + [0, 3, 0])], seed=11, frame=2))
+
+    def run(job_list, persistent):
+        for acc, w, h in bufs:
+            ctx.zero(acc, 16 * w * h)
+        ctx.render_jobs(job_list, persistent=persistent, want_stats=False)
+        return [ctx.read_accum(acc, w, h) for acc, w, h in bufs]
+
+    alone = [run([j], False)[k] for k, j in enumerate(jobs)]
+    saved = ctx.tuning()
+    try:
+        for shape in [(64, 8), (128, 8), (128, 12), (128, 26), (256, 12)]:
+            ctx.set_tuning(shape[0], shape[1], 2)
+            for persistent in (False, True):
+                together = run(jobs, persistent)
+                for k in range(3):
+                    assert np.array_equal(together[k], alone[k]), (shape, persistent, k)
+    finally:
+        ctx.set_tuning(*saved)
+    for acc, _, _ in bufs:
+        ctx.free(acc)
+    for h in hs + [sc]:
+        ctx.scene_destroy(h)
+
+
+def test_trace_parity_under_every_launch_shape(ctx, oracle, rtcore):
+    xyz, idx = rtcore.gen_triangle_soup(100000, 0.006, seed=3)
+    s = ctx.add_scene(xyz, idx, 20)
+    ctx.build(s)
+    ref = oracle.build_scene(xyz, idx, 20)
+    cam = rtcore.camera_look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0, 1, 0), 60.0, 1.0)
+    rays = oracle.raygen_pinhole(cam.as_array(), 160, 160)
+    r = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays, nthreads=8)
+    saved = ctx.tuning()
+    try:
+        for block, lds in [(64, 8), (64, 16), (128, 8), (128, 12), (128, 16), (128, 26), (256, 8), (256, 12)]:
+            ctx.set_tuning(block, lds, 4)
+            assert np.array_equal(ctx.trace(s, rays), r), (block, lds)
+            st = ctx.render_direct(s, cam, 160, 160, [(0.5, 2.5, -1.0)], flags=rtcore.FLAG_KEEP_HITS)
+            _, hits, _ = ctx.frame_read(160, 160, 1, hits=True)
+            assert np.array_equal(hits, r), (block, lds)
+    finally:
+        ctx.set_tuning(*saved)
+    ctx.scene_destroy(s)
+
+
+def test_graph_replay_equals_eager(ctx, rtcore):
+    xyz, idx = rtcore.gen_object_mix(8000, seed=21)
+    hs = ctx.batch_create([len(idx), len(idx)], [len(xyz), len(xyz)], [16, 16])
+    for h in hs:
+        ctx.scene_upload(h, xyz, idx)
+    ctx.build(hs[0])
+    c = xyz.mean(0)
+    cam = rtcore.camera_look_at(tuple(c + [0, 1, -6]), tuple(c), (0, 1, 0), 60.0, 1.0)
+    W = H = 32
+    accs = [ctx.alloc(16 * W * H) for _ in hs]
+    jobs = [ctx.job(h, cam, W, H, a, rtcore.MODE_GI, 2, 4, 0.7, 1.0, seed=3, frame=10)
+            for h, a in zip(hs, accs)]
+    for a in accs:
+        ctx.zero(a, 16 * W * H)
+    g = ctx.graph_capture(hs[0], jobs, rebuild=True)
+    for _ in range(3):  # replays with eager work and syncs in between
+        ctx.graph_launch(g)
+        ctx.sync()
+        ctx.build(hs[1])
+        ctx.sync()
+    replayed = [ctx.read_accum(a, W, H) for a in accs]
+    for a in accs:
+        ctx.zero(a, 16 * W * H)
+    for f in range(3):
+        eager = [ctx.job(h, cam, W, H, a, rtcore.MODE_GI, 2, 4, 0.7, 1.0, seed=3, frame=10 + f)
+                 for h, a in zip(hs, accs)]
+        ctx.render_jobs(eager, want_stats=False)
+    expect = [ctx.read_accum(a, W, H) for a in accs]
+    for r, e in zip(replayed, expect):
+        assert np.array_equal(r, e)
+        assert np.all(r[..., 3] == 6)
+    ctx.graph_destroy(g)
+    for a in accs:
+        ctx.free(a)
+    for h in hs:
+        ctx.scene_destroy(h)
+
+
+def test_autotuner_picks_and_caches(ctx, rtcore, tmp_path):
+    xyz, idx = rtcore.gen_object_mix(20000, seed=4)
+    s = ctx.add_scene(xyz, idx, 18)
+    ctx.build(s)
+    c = xyz.mean(0)
+    cam = rtcore.camera_look_at(tuple(c + [0, 1, -8]), tuple(c), (0, 1, 0), 60.0, 1.0)
+    acc = ctx.alloc(16 * 128 * 128)
+    ctx.zero(acc, 16 * 128 * 128)
+    jobs = [ctx.job(s, cam, 128, 128, acc, rtcore.MODE_GI, 1, 2, 0.7, 1.0)]
+    cache = str(tmp_path / "tune.txt")
+    best = ctx.autotune(jobs, cache)
+    assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] in (2, 4, 8)
+    assert ctx.tuning() == best
+    assert open(cache).read().strip().endswith(f"{best[0]} {best[1]} {best[2]}")
+    ctx.set_tuning(128, 26, 4)
+    assert ctx.autotune(jobs, cache) == best  # cache hit
+    # the search rendered into scratch buffers, not into the caller's
+    assert np.all(ctx.read_accum(acc, 128, 128) == 0)
+    ctx.free(acc)
+    ctx.scene_destroy(s)

@@ -376,3 +376,65 @@ def test_render_direct_analytic_shadow(oracle):
         expect = (0.1 + (0.0 if in_shadow else -ld[2])) * 0.8
         assert occ[i, 0] == (1 if in_shadow else 0)
         assert abs(rgba[i, 0] - expect) < 1e-5
+
+
+# ------------------------------------------------------------------------- GI
+def _form_factor_parallel_rect(px, py, x0, x1, y0, y1, c):
+    """Form factor from a differential area at (px, py, 0) facing +z to the
+    rectangle [x0, x1] x [y0, y1] at height c (corner superposition of the
+    differential-area-to-parallel-rectangle closed form)."""
+    def f(a, b):
+        A, B = a / c, b / c
+        return (A / np.sqrt(1 + A * A) * np.arctan(B / np.sqrt(1 + A * A)) +
+                B / np.sqrt(1 + B * B) * np.arctan(A / np.sqrt(1 + B * B))) / (2 * np.pi)
+
+    def g(x, y):
+        return np.sign(x) * np.sign(y) * f(np.abs(x), np.abs(y))
+
+    return g(x1 - px, y1 - py) - g(x0 - px, y1 - py) - g(x1 - px, y0 - py) + g(x0 - px, y0 - py)
+
+
+def _floor_and_ceiling():
+    floor = [[-50, -50, 0], [50, -50, 0], [50, 50, 0], [-50, 50, 0]]
+    ceil = [[-0.5, -0.5, 1], [0.5, -0.5, 1], [0.5, 0.5, 1], [-0.5, 0.5, 1]]
+    xyz = np.array(floor + ceil, np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]], np.uint32)
+    return xyz, idx
+
+
+def test_gi_oracle_sky_and_zero_bounce(oracle, rtcore):
+    xyz, idx = _floor_and_ceiling()
+    b = oracle.build_scene(xyz, idx, 6)
+    down = rtcore.camera_look_at((0, 0, 0.5), (0, 0, -3), (0, 1, 0), 30.0, 1.0)
+    img, rays = oracle.render_gi(b["nodes"], b["ids"], xyz, idx, down.as_array(), 8, 8, 5, 0, 0.7, 1.5, 9)
+    assert np.all(img == 0.0) and rays == 8 * 8 * 5  # floor hit, no bounce allowed
+    side = rtcore.camera_look_at((0, 0, 0.5), (0, 100, 30), (0, 0, 1), 10.0, 1.0)  # above the horizon, past the ceiling
+    img, rays = oracle.render_gi(b["nodes"], b["ids"], xyz, idx, side.as_array(), 8, 8, 4, 3, 0.7, 1.5, 9)
+    assert np.all(img == 4 * 1.5) and rays == 8 * 8 * 4  # every primary ray escapes
+
+
+def test_gi_oracle_form_factor(oracle, rtcore):
+    """Floor under a 1 x 1 ceiling at height 1, one bounce: a floor point at p
+    returns albedo * sky * (1 - F(p)), F the closed-form form factor.  Pins the
+    cosine-weighted sampling, the face-forward normal and the path loop."""
+    xyz, idx = _floor_and_ceiling()
+    b = oracle.build_scene(xyz, idx, 6)
+    cam = rtcore.camera_look_at((0.1, -0.05, 0.5), (0.1, -0.05, 0.0), (0, 1, 0), 90.0, 1.0)
+    W = H = 12
+    spp = 4096
+    img, _ = oracle.render_gi(b["nodes"], b["ids"], xyz, idx, cam.as_array(), W, H, spp, 1, 0.7, 1.0,
+                              31, nthreads=8)
+    est = img / spp
+    c = cam.as_array()
+    py, px = np.mgrid[0:H, 0:W]
+    x = ((2 * px + 1) - W) / W * c[12]
+    y = -((2 * py + 1) - H) / H * c[13]
+    d = c[3:6] + x[..., None] * c[6:9] + y[..., None] * c[9:12]
+    t = -c[2] / d[..., 2]
+    fx, fy = c[0] + t * d[..., 0], c[1] + t * d[..., 1]
+    expect = 0.7 * (1 - _form_factor_parallel_rect(fx, fy, -0.5, 0.5, -0.5, 0.5, 1.0))
+    sigma = 0.7 * np.sqrt(expect / 0.7 * (1 - expect / 0.7) / spp)
+    z = (est - expect) / sigma
+    # the pixel footprint varies F by far less than sigma at this resolution
+    assert np.abs(z).max() < 5.0
+    assert abs(z.mean()) < 5.0 / np.sqrt(W * H)
