@@ -142,6 +142,9 @@ typedef struct rt_job_desc {
     uint64_t seed;
     uint32_t frame;
     float *accum;
+    /* optional tile range (tiles of 16 x block/16 pixels, row-major):
+     * tile_count == 0 renders the whole image */
+    uint32_t tile_begin, tile_count;
 } rt_job_desc;
 
 typedef struct rt_ctx rt_ctx;
@@ -254,6 +257,71 @@ int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_des
                            uint32_t njobs, int *graph);
 int rt_graph_launch(rt_ctx *ctx, int graph);
 int rt_graph_destroy(rt_ctx *ctx, int graph);
+
+/* ---- render server (no reference counterpart; it keeps the reference's
+ *      progressive accumulation, RayTracerCS.cpp:246-250 with
+ *      CSPrimaryRays.hlsl:62-63, per stream of requests).
+ * Scenes are registered once by key and kept resident (built) while the
+ * device-memory budget allows; least recently used idle scenes and
+ * accumulators are evicted.  A request renders spp samples per pixel of one
+ * view into the accumulator of its stream (scene_key, view, width, height,
+ * mode): the stream's n-th request uses frame = n, so successive requests
+ * refine one image.  Each rt_server_step admits arrived requests in arrival
+ * order (one in flight per stream) and renders a share of every in-flight
+ * request's tiles in one persistent rt_render_jobs launch; a request's pixels
+ * are the same alone or in any mix.  dry_run performs no device work (no
+ * context needed): a step takes estimated rays / dry_rate seconds. */
+typedef struct rt_server rt_server;
+typedef struct rt_server_config {
+    uint64_t hbm_budget;     /* device bytes for scenes + accumulators */
+    uint32_t max_inflight;   /* requests rendered concurrently */
+    uint32_t tiles_per_step; /* tiles per launch, shared by in-flight requests */
+    int32_t dry_run;
+    double dry_rate;         /* rays per second assumed by dry_run */
+} rt_server_config;
+typedef struct rt_request {
+    uint32_t scene_key, view;
+    rt_camera cam;
+    uint32_t width, height;
+    uint32_t mode, spp, max_bounces;
+    float albedo, sky, ambient;
+    const float *lights;     /* host, 3 floats per light, up to 8; copied at submit */
+    uint32_t nlights;
+    uint64_t seed;
+    double arrival;          /* seconds, caller's clock; admitted once now >= arrival */
+} rt_request;
+typedef struct rt_request_result {
+    uint64_t id;
+    double arrival, admitted, completed; /* completed = step's now + step duration */
+    uint64_t rays;           /* rays of this request (share of its launches) */
+    uint32_t frame;          /* pass index of the request in its stream */
+    uint32_t samples;        /* samples per pixel in the stream after this request */
+    int32_t status;          /* RT_OK, RT_ERR_BUDGET (never fits) or the failing call's code */
+} rt_request_result;
+typedef struct rt_server_stats {
+    uint64_t submitted, completed, rejected, steps, builds, evictions, rays;
+    uint64_t bytes_used;
+    uint32_t pending, inflight, resident_scenes, accumulators;
+    double device_s;         /* summed device time of the step launches (dry run: simulated) */
+} rt_server_stats;
+int rt_server_create(rt_ctx *ctx, const rt_server_config *cfg, rt_server **out);
+void rt_server_destroy(rt_server *srv);
+const char *rt_server_last_error(const rt_server *srv);
+/* Registers a scene (copied to the host side of the server; built on demand). */
+int rt_server_add_scene(rt_server *srv, uint32_t key, const float *xyz, uint32_t nv,
+                        const uint32_t *idx, uint32_t nt, uint32_t depth);
+int rt_server_submit(rt_server *srv, const rt_request *req, uint64_t *id);
+/* One scheduling step at time `now` (seconds); *elapsed (optional) = the
+ * step's duration: wall time of admission, builds and the synchronised
+ * launch (dry run: the simulated launch time).  Requests retired by the step
+ * complete at now + *elapsed; a step with nothing to render takes 0. */
+int rt_server_step(rt_server *srv, double now, double *elapsed);
+/* Moves up to cap finished results into out; *n = how many. */
+int rt_server_poll(rt_server *srv, rt_request_result *out, uint32_t cap, uint32_t *n);
+int rt_server_status(rt_server *srv, rt_server_stats *out);
+/* Copies a resident stream's accumulator (width * height * 4 floats). */
+int rt_server_read_stream(rt_server *srv, uint32_t scene_key, uint32_t view, uint32_t width,
+                          uint32_t height, uint32_t mode, float *rgba_sum, uint32_t *samples);
 
 /* ---- device buffers for accumulators (Buffer<T> role, DeviceCore/Buffer.hpp:49-77) */
 int rt_device_alloc(rt_ctx *ctx, uint64_t bytes, void **ptr);

@@ -154,6 +154,27 @@ hipEvent_t *prof_events(rt_ctx *ctx, int kind, size_t n)
     return &ctx->pool[r.first];
 }
 
+}  // namespace
+
+namespace rt {
+
+// Device bytes create_batch allocates for one scene (serving admission).
+uint64_t scene_bytes(uint32_t max_tris, uint32_t max_verts, uint32_t depth)
+{
+    const uint64_t nt = max_tris ? max_tris : 1, nv = max_verts ? max_verts : 1;
+    const uint64_t tiles = sort_tiles((uint32_t)nt);
+    const uint64_t words = kCodePasses * kRadix + kCounters + (uint64_t)kCodePasses * tiles * kRadix;
+    const uint64_t table = ((uint64_t)1 << table_bits(max_tris)) + 1;
+    return 12 * nv + 12 * nt + sizeof(SceneDesc) + sizeof(SceneHdr) + sizeof(PrimBox) * nt + 12 * nt +
+           24 * (uint64_t)bounds_blocks((uint32_t)nt) + 16 * nt + 4 * words + 4 * table + 16 * 4 * nt +
+           4 * kMaxLevels + sizeof(rt_node32) * ((uint64_t)1 << depth) + sizeof(TravNode) * nt +
+           sizeof(TravTri) * nt;
+}
+
+}  // namespace rt
+
+namespace {
+
 int create_batch(rt_ctx *ctx, uint32_t n, const uint32_t *max_tris, const uint32_t *max_verts,
                  const uint32_t *depths, int *first)
 {

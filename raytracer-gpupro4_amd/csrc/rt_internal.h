@@ -219,5 +219,6 @@ int tune_index(const rt_tuning &t);
 uint32_t sort_tiles(uint32_t n);
 uint32_t bounds_blocks(uint32_t n);
 uint32_t table_bits(uint32_t max_tris);
+uint64_t scene_bytes(uint32_t max_tris, uint32_t max_verts, uint32_t depth);
 
 }  // namespace rt

@@ -474,7 +474,7 @@ struct RenderJob {
     float4 *accum;
     rt_camera cam;
     uint32_t width, height, tiles_x, tile_off;
-    uint32_t mode, spp, max_bounces, nlights;
+    uint32_t local_base, mode, spp, max_bounces, nlights, pad1;
     float albedo, sky, ambient, pad;
     uint64_t seed;
     uint32_t frame, pad2;
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restri
             else hi = mid;
         }
         const RenderJob &J = jobs[lo];
-        const uint32_t lt = tile - J.tile_off;
+        const uint32_t lt = tile - J.tile_off + J.local_base;
         const uint32_t px = (lt % J.tiles_x) * TW + (threadIdx.x % TW);
         const uint32_t py = (lt / J.tiles_x) * TH + (threadIdx.x / TW);
         if (px < J.width && py < J.height) {
@@ -697,6 +697,8 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
     j->tiles_x = (d.width + TW - 1) / TW;
     j->tile_off = tile_off;
     j->mode = d.mode == RT_MODE_GI ? 1u : 0u;
+    const uint32_t all = j->tiles_x * ((d.height + TH - 1) / TH);
+    j->local_base = d.tile_count ? (d.tile_begin < all ? d.tile_begin : all) : 0u;
     j->spp = d.spp;
     j->max_bounces = d.max_bounces;
     j->nlights = d.nlights;
@@ -706,7 +708,7 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
     j->seed = d.seed;
     j->frame = d.frame;
     for (uint32_t i = 0; i < 24; ++i) j->lights[i] = (i < 3 * d.nlights) ? d.lights[i] : 0.0f;
-    *tiles = j->tiles_x * ((d.height + TH - 1) / TH);
+    *tiles = d.tile_count ? (d.tile_count < all - j->local_base ? d.tile_count : all - j->local_base) : all;
 }
 
 size_t render_job_bytes() { return sizeof(RenderJob); }
@@ -717,7 +719,7 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
                               unsigned long long *stats, const rt_tuning &tune, int cu_count,
                               hipStream_t st)
 {
-    if (total_tiles == 0 || njobs == 0) return hipSuccess;
+    if (total_tiles == 0 || njobs == 0 || (!persistent && tile_count == 0)) return hipSuccess;
     const RenderJob *jobs = static_cast<const RenderJob *>(d_jobs);
 #define RT_L(B, K)                                                                                  \
     {                                                                                               \

@@ -91,7 +91,40 @@ class JobDesc(C.Structure):
                 ("mode", C.c_uint32), ("spp", C.c_uint32), ("max_bounces", C.c_uint32),
                 ("albedo", C.c_float), ("sky", C.c_float), ("ambient", C.c_float),
                 ("lights", C.POINTER(C.c_float)), ("nlights", C.c_uint32), ("seed", C.c_uint64),
-                ("frame", C.c_uint32), ("accum", C.c_void_p)]
+                ("frame", C.c_uint32), ("accum", C.c_void_p), ("tile_begin", C.c_uint32),
+                ("tile_count", C.c_uint32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("tiles_per_step", C.c_uint32),
+                ("dry_run", C.c_int32), ("dry_rate", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene_key", C.c_uint32), ("view", C.c_uint32), ("cam", Camera), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("mode", C.c_uint32), ("spp", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("albedo", C.c_float), ("sky", C.c_float),
+                ("ambient", C.c_float), ("lights", C.POINTER(C.c_float)), ("nlights", C.c_uint32),
+                ("seed", C.c_uint64), ("arrival", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("arrival", C.c_double), ("admitted", C.c_double),
+                ("completed", C.c_double), ("rays", C.c_uint64), ("frame", C.c_uint32),
+                ("samples", C.c_uint32), ("status", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "rejected", "steps", "builds",
+                                          "evictions", "rays", "bytes_used")] + \
+               [(k, C.c_uint32) for k in ("pending", "inflight", "resident_scenes", "accumulators")] + \
+               [("device_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class FrameStats(C.Structure):
@@ -153,6 +186,15 @@ def lib() -> C.CDLL:
         "rt_device_free": (I32, [P, P]),
         "rt_device_zero": (I32, [P, P, U64]),
         "rt_device_read": (I32, [P, P, P, U64]),
+        "rt_server_create": (I32, [P, C.POINTER(ServerConfig), C.POINTER(P)]),
+        "rt_server_destroy": (None, [P]),
+        "rt_server_last_error": (C.c_char_p, [P]),
+        "rt_server_add_scene": (I32, [P, U32, P, U32, P, U32, U32]),
+        "rt_server_submit": (I32, [P, C.POINTER(Request), C.POINTER(U64)]),
+        "rt_server_step": (I32, [P, C.c_double, C.POINTER(C.c_double)]),
+        "rt_server_poll": (I32, [P, P, U32, C.POINTER(U32)]),
+        "rt_server_status": (I32, [P, C.POINTER(ServerStats)]),
+        "rt_server_read_stream": (I32, [P, U32, U32, U32, U32, U32, P, C.POINTER(U32)]),
         "rt_camera_look_at": (None, [C.POINTER(Camera), P, P, P, C.c_float, C.c_float]),
         "rt_gen_torus": (I32, [C.c_float, C.c_float, U32, U32, C.c_float, U64, P, C.POINTER(U32), P,
                                C.POINTER(U32)]),
@@ -315,7 +357,7 @@ class Context:
 
     @staticmethod
     def job(scene, cam, width, height, accum, mode=MODE_GI, spp=1, max_bounces=4, albedo=0.7,
-            sky=1.0, ambient=0.1, lights=None, seed=1, frame=0):
+            sky=1.0, ambient=0.1, lights=None, seed=1, frame=0, tile_begin=0, tile_count=0):
         """Builds a JobDesc; keeps the light array alive on the object."""
         j = JobDesc()
         j.scene, j.cam, j.width, j.height = scene, cam, width, height
@@ -326,6 +368,7 @@ class Context:
         j.lights = arr.ctypes.data_as(C.POINTER(C.c_float)) if arr.size else None
         j.nlights = arr.size // 3
         j.seed, j.frame, j.accum = seed, frame, accum
+        j.tile_begin, j.tile_count = tile_begin, tile_count
         return j
 
     def render_jobs(self, jobs, persistent=True, want_stats=True):
@@ -455,3 +498,111 @@ class Context:
         occ = np.empty((height * width, max(nlights, 1)), np.uint8) if hits else None
         self.check(self.L.rt_frame_read(self.h, _ptr(rgba), _ptr(h), _ptr(occ)), "rt_frame_read")
         return rgba, h, occ
+
+
+class Server:
+    """Render server (rt_server_*): resident scenes and progressive streams
+    under a device-memory budget, continuous batching of requests.
+    ctx=None requires dry_run (scheduling only, no device work)."""
+
+    def __init__(self, ctx: Context | None, hbm_budget: int, max_inflight: int = 16,
+                 tiles_per_step: int = 4096, dry_run: bool = False, dry_rate: float = 1e9):
+        self.L = lib()
+        self.ctx = ctx
+        cfg = ServerConfig(hbm_budget, max_inflight, tiles_per_step, 1 if dry_run else 0, dry_rate)
+        h = C.c_void_p()
+        rc = self.L.rt_server_create(ctx.h if ctx else None, C.byref(cfg), C.byref(h))
+        if rc != RT_OK:
+            raise RtError(rc, "rt_server_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.rt_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int, what: str):
+        if rc != RT_OK:
+            raise RtError(rc, f"{what}: {self.L.rt_server_last_error(self.h).decode()}")
+
+    def add_scene(self, key: int, xyz: np.ndarray, idx: np.ndarray, depth: int):
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        self.check(self.L.rt_server_add_scene(self.h, key, _ptr(xyz), xyz.size // 3, _ptr(idx),
+                                              idx.size // 3, depth), "rt_server_add_scene")
+
+    def submit(self, scene_key, view, cam, width, height, mode=MODE_GI, spp=1, max_bounces=4,
+               albedo=0.7, sky=1.0, ambient=0.1, lights=None, seed=1, arrival=0.0) -> int:
+        r = Request()
+        r.scene_key, r.view, r.cam, r.width, r.height = scene_key, view, cam, width, height
+        r.mode, r.spp, r.max_bounces = mode, spp, max_bounces
+        r.albedo, r.sky, r.ambient = albedo, sky, ambient
+        arr = np.ascontiguousarray(np.asarray(lights if lights is not None else [], np.float32).reshape(-1))
+        r.lights = arr.ctypes.data_as(C.POINTER(C.c_float)) if arr.size else None
+        r.nlights = arr.size // 3
+        r.seed, r.arrival = seed, arrival
+        rid = C.c_uint64()
+        self.check(self.L.rt_server_submit(self.h, C.byref(r), C.byref(rid)), "rt_server_submit")
+        return rid.value
+
+    def step(self, now: float) -> float:
+        """One scheduling step at `now`; returns its duration in seconds."""
+        dt = C.c_double()
+        self.check(self.L.rt_server_step(self.h, now, C.byref(dt)), "rt_server_step")
+        return dt.value
+
+    def poll(self, cap: int = 1024) -> list:
+        out = (RequestResult * cap)()
+        n = C.c_uint32()
+        self.check(self.L.rt_server_poll(self.h, out, cap, C.byref(n)), "rt_server_poll")
+        return [out[k].as_dict() for k in range(n.value)]
+
+    def status(self) -> dict:
+        st = ServerStats()
+        self.check(self.L.rt_server_status(self.h, C.byref(st)), "rt_server_status")
+        return st.as_dict()
+
+    def read_stream(self, scene_key, view, width, height, mode):
+        out = np.empty((height, width, 4), np.float32)
+        n = C.c_uint32()
+        self.check(self.L.rt_server_read_stream(self.h, scene_key, view, width, height, mode, _ptr(out),
+                                                C.byref(n)), "rt_server_read_stream")
+        return out, n.value
+
+    def run(self, arrivals, now: float = 0.0, max_steps: int = 10_000_000) -> tuple:
+        """Steps on a simulated clock until every submitted request is
+        finished: each step advances the clock by its duration, an idle server
+        jumps to the next arrival.  arrivals: the submitted arrival times.
+        Returns (results, end time)."""
+        pending_t = sorted(arrivals)
+        results, t, k = [], now, 0
+        for _ in range(max_steps):
+            st = self.status()
+            if st["pending"] == 0 and st["inflight"] == 0:
+                break
+            dt = self.step(t)
+            results += self.poll()
+            if dt > 0.0:
+                t += dt
+                continue
+            st = self.status()
+            if st["inflight"] == 0:
+                while k < len(pending_t) and pending_t[k] <= t:
+                    k += 1
+                if k == len(pending_t):
+                    break  # nothing left that could arrive (rejected or waiting for memory)
+                t = pending_t[k]
+        results += self.poll()
+        return results, t

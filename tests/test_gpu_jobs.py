@@ -192,3 +192,26 @@ def test_autotuner_picks_and_caches(ctx, rtcore, tmp_path):
     assert np.all(ctx.read_accum(acc, 128, 128) == 0)
     ctx.free(acc)
     ctx.scene_destroy(s)
+
+
+def test_tile_ranges_compose_to_whole_image(ctx, rtcore, mix):
+    """Jobs over disjoint tile ranges of one image (one of them empty, one
+    running past the end) add up to the whole-image job, in one launch or many."""
+    s, xyz, idx, ref, cam = mix
+    W, H = 50, 30
+    block = ctx.tuning()[0]
+    ntiles = ((W + 15) // 16) * ((H + block // 16 - 1) // (block // 16))
+    acc = ctx.alloc(16 * W * H)
+    ctx.zero(acc, 16 * W * H)
+    ctx.render_jobs([ctx.job(s, cam, W, H, acc, rtcore.MODE_GI, 2, 3, seed=8)], want_stats=False)
+    whole = ctx.read_accum(acc, W, H)
+    cuts = [(0, 3), (3, 0), (3, ntiles // 2), (ntiles // 2, ntiles + 7)]
+    for persistent in (True, False):
+        ctx.zero(acc, 16 * W * H)
+        jobs = [ctx.job(s, cam, W, H, acc, rtcore.MODE_GI, 2, 3, seed=8, tile_begin=b, tile_count=e - b)
+                for b, e in cuts if e > b]
+        jobs.insert(1, ctx.job(s, cam, W, H, acc, rtcore.MODE_GI, 2, 3, seed=8, tile_begin=ntiles + 5,
+                               tile_count=4))  # past the end: renders nothing
+        ctx.render_jobs(jobs, persistent=persistent, want_stats=False)
+        assert np.array_equal(ctx.read_accum(acc, W, H), whole), persistent
+    ctx.free(acc)
