@@ -1,27 +1,43 @@
-"""Benchmark: per-frame LBVH rebuild + primary/shadow traversal on one GPU.
+"""Benchmarks of the LBVH build + traversal core on one GPU.
 
-Workload (one step = one frame): 262,144 triangles (centroids U[0,1]^3, edges
-N(0, 0.004^2)), rigidly rotated 1 degree per frame about the axis (1, 2, 3)
-through (0.5, 0.5, 0.5); the LBVH is rebuilt every frame (bounds, Morton,
-onesweep sort, emit, refit), then a 1920x1080 frame traces one primary ray per
-pixel and one shadow ray per lit-side hit to a point light.
+Default workload "frame" (one step = one frame): 262,144 triangles
+(centroids U[0,1]^3, edges N(0, 0.004^2)), rigidly rotated 1 degree per frame
+about the axis (1, 2, 3) through (0.5, 0.5, 0.5); the LBVH is rebuilt every
+frame (bounds, Morton, onesweep sort, emit, refit), then a 1920x1080 frame
+traces one primary ray per pixel and one shadow ray per lit-side hit to a
+point light.  value = (primary + shadow rays) / summed device time of the
+render kernel (HIP events on the library's stream); build_ms is the device
+time of one rebuild.  Scene data is resident in device memory before the
+timed region; nothing is copied per frame.
 
-value = (primary + shadow rays) / summed device time of the trace kernel
-(HIP events on the library's stream), the Mrays/s definition that excludes
-the build; build_ms is the device time of one rebuild.  Scene data is resident
-in device memory before the timed region; nothing is copied per frame.
+Other workloads (--workload), each printing one JSON line of the same shape:
+  sphere   2,097,152-triangle noise-displaced sphere, 8 orbit cameras, 4 lights
+           (one step = the 8 frames); value = Mrays/s, plus HBM GB/s.
+  batched  64 object-mix scenes (1,024..200,000 triangles, log-uniform), images
+           of 256^2/512^2/1024^2, diffuse GI at 4 spp x 4 bounces; one step =
+           rebuild all 64 + render all 64, batched (one build, one persistent
+           launch; also as a replayed graph) against per-scene builds and
+           launches; images compared bit for bit.
+  serve    Poisson 200 requests/s for 60 s over a pool of 500 scenes
+           (1k..500k triangles), 512^2 or 1920x1080, direct or 2-bounce GI at
+           1 spp with progressive accumulation, 200 GB device budget; p50/p99
+           latency and aggregate Mrays/s against one request per launch.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "raytracer-gpupro4_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "Mrays/s + LBVH build ms @262k tris; batched vs plain; p50/p99 latency"
+CPU_SHARE = 16  # host threads a one-GPU box grants a process
 
 NTRI, SIGMA, SEED, DEPTH = 262144, 0.004, 2, 20
 W, H = 1920, 1080
@@ -32,39 +48,86 @@ AXIS, CENTER = (1.0, 2.0, 3.0), (0.5, 0.5, 0.5)
 # 32 B node box per box test, 4 B id + 12 B indices + 36 B vertices per
 # triangle test)
 RAY_BYTES, BOX_BYTES, TRI_BYTES = 48 + 24, 32, 52
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "render_direct_soup262k_1080p_pmc.md")
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def depth_for(nt):
+    """Heap depth for nt triangles: two levels below a balanced binary tree."""
+    return int(min(24, max(8, math.ceil(math.log2(max(nt, 2))) + 2)))
+
+
+def view_camera(rtcore, xyz, view, w, h, dist=1.6):
+    """Camera on a circle around the scene's box, view k at 30 + 90k degrees."""
+    lo, hi = xyz.min(0), xyz.max(0)
+    c = 0.5 * (lo + hi)
+    r = 0.5 * float(np.linalg.norm(hi - lo))
+    a = math.radians(30.0 + 90.0 * view)
+    pos = c + dist * r * np.array([math.cos(a), 0.45, math.sin(a)], np.float32)
+    return rtcore.camera_look_at(tuple(pos), tuple(c), (0, 1, 0), 60.0, w / h)
+
+
+def result(metric, value, unit, steps, warmup, wall, config, dtype="f32", **extra):
+    out = {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": round(wall * 1e3 / max(steps, 1), 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def traffic_from_profile():
+    """HBM bytes per k_render_direct launch measured by rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by their copy-kernel calibration),
+    read from the committed summary; None when it is absent."""
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+# ------------------------------------------------------------------ frame
 def frame(ctx, rtcore, s, cam, f, flags=0, stats=False):
     ctx.scene_transform(s, rtcore.rotation_about(AXIS, float(f), CENTER))
     ctx.build(s)
     return ctx.render_direct(s, cam, W, H, LIGHTS, flags=flags, want_stats=stats)
 
 
-def cpu_baseline(rtcore, xyz, idx, cam):
-    """Oracle build (1 thread, as the reference builds on the CPU) and oracle
-    traversal (all cores in the affinity mask) of a bounded sample of frame 0:
-    every 8th pixel's primary ray plus its shadow ray (reference semantics)."""
+def cpu_baseline(rtcore, xyz, idx, cam, budget_s=12.0):
+    """The oracle doing the same per-frame work on the host: build of the
+    rotated scene on 1 thread (the reference builds on one CPU thread) and
+    closest-hit primary + shadow traversal of every pixel on up to CPU_SHARE
+    threads, frame after frame until budget_s seconds have been spent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-
     import oracle
 
-    cores = len(os.sched_getaffinity(0))
-    m = rtcore.rotation_about(AXIS, 0.0, CENTER)
-    f = np.float32
-    d = (xyz - m[9:12]).astype(f)
-    R = m[:9].reshape(3, 3)
-    rot = np.stack([((((R[r, 0] * d[:, 0]).astype(f) + (R[r, 1] * d[:, 1]).astype(f)).astype(f)
-                      + (R[r, 2] * d[:, 2]).astype(f)).astype(f) + m[9 + r]).astype(f) for r in range(3)], 1)
-    t0 = time.perf_counter()
-    ref = oracle.build_scene(rot, idx, DEPTH, presorted=False)
-    build_s = time.perf_counter() - t0
-    rays = oracle.raygen_pinhole(cam.as_array(), W, H)[::8].copy()
-    t0 = time.perf_counter()
-    hits = oracle.trace(ref["nodes"], ref["ids"], rot, idx, rays, nthreads=cores)
-    _, occ = oracle.render_direct(ref["nodes"], ref["ids"], rot, idx, rays, hits, LIGHTS, nthreads=cores)
-    trace_s = time.perf_counter() - t0
-    nrays = len(rays) + int((occ != 2).sum())
+    cores = min(CPU_SHARE, len(os.sched_getaffinity(0)))
+    f32 = np.float32
+    rays0 = oracle.raygen_pinhole(cam.as_array(), W, H)
+    build_s = trace_s = 0.0
+    nrays = frames = 0
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s and frames < 64:
+        m = rtcore.rotation_about(AXIS, float(frames), CENTER)
+        d = (xyz - m[9:12]).astype(f32)
+        R = m[:9].reshape(3, 3)
+        rot = np.stack([((((R[r, 0] * d[:, 0]).astype(f32) + (R[r, 1] * d[:, 1]).astype(f32)).astype(f32)
+                          + (R[r, 2] * d[:, 2]).astype(f32)).astype(f32) + m[9 + r]).astype(f32)
+                        for r in range(3)], 1)
+        t0 = time.perf_counter()
+        ref = oracle.build_scene(rot, idx, DEPTH, presorted=False)
+        build_s += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        hits = oracle.trace(ref["nodes"], ref["ids"], rot, idx, rays0, nthreads=cores)
+        _, occ = oracle.render_direct(ref["nodes"], ref["ids"], rot, idx, rays0, hits, LIGHTS, nthreads=cores)
+        trace_s += time.perf_counter() - t0
+        nrays += len(rays0) + int((occ != 2).sum())
+        frames += 1
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -73,49 +136,35 @@ def cpu_baseline(rtcore, xyz, idx, cam):
                 break
     except OSError:
         pass
-    return {"value": nrays / trace_s / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"frame 0, every 8th pixel of {W}x{H}: {len(rays)} primary + "
-                      f"{nrays - len(rays)} shadow rays (closest hit from the hit leaf), "
-                      f"oracle build of {NTRI} tris on 1 thread",
-            "build_ms_1thread": build_s * 1e3, "trace_s": trace_s, "cpu_model": model}
+    return {"value": round(nrays / trace_s / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"{frames} full frames of the same workload ({W}x{H} primary + shadow rays, "
+                      f"closest hit, shadow rays from the hit leaf): {nrays} rays in {trace_s:.1f} s "
+                      f"on {cores} threads; oracle rebuild of {NTRI} tris per frame on 1 thread",
+            "build_ms_1thread": round(build_s * 1e3 / frames, 2), "cpu_model": model}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the traversal shards by "
-              f"image tile / scene across GPUs, which this build does not run", file=sys.stderr)
-        sys.exit(2)
-    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)
-        sys.exit(2)
-
+def device_sync(ctx):
+    """Waits for the library's stream, then for the whole device."""
     import torch
 
-    import rtcore
+    ctx.sync()
+    torch.cuda.synchronize()
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    rtcore.lib()
+
+def workload_frame(args, rtcore):
     xyz, idx = rtcore.gen_triangle_soup(NTRI, SIGMA, SEED)
     cam = rtcore.camera_look_at(CAM_POS, CAM_TGT, (0, 1, 0), FOV, W / H)
     ctx = rtcore.Context(0)
     s = ctx.add_scene(xyz, idx, DEPTH)
     for f in range(args.warmup):
         frame(ctx, rtcore, s, cam, f)
-    ctx.sync()
-    torch.cuda.synchronize()
+    device_sync(ctx)
 
     ctx.profile_begin()
     t0 = time.perf_counter()
     for f in range(args.steps):
         frame(ctx, rtcore, s, cam, f)
-    ctx.sync()
-    torch.cuda.synchronize()
+    device_sync(ctx)
     wall = time.perf_counter() - t0
     prof = ctx.profile_end()
 
@@ -127,6 +176,7 @@ def main():
         boxes += st["box_tests"]
         tris += st["tri_tests"]
         rays_counted += st["primary_rays"] + st["shadow_rays"]
+    ctx.close()
 
     rays = prof["primary_rays"] + prof["shadow_rays"]
     assert rays == rays_counted, (rays, rays_counted)
@@ -137,31 +187,320 @@ def main():
     per_launch_ms = trace_ms / prof["launches"]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
     build = {k: v / prof["builds"] for k, v in prof["build_ms"].items()}
-
-    out = {
-        "metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{NTRI}-triangle soup rebuilt every frame (1 deg/frame rotation), "
-                               f"{W}x{H} primary + shadow rays, 1 point light",
-                   "tris": NTRI, "depth": DEPTH, "width": W, "height": H, "lights": len(LIGHTS)},
-        "build_ms": round(build["total"], 4),
-        "build_ms_stages": {k: round(v, 4) for k, v in build.items()},
-        "trace_ms_per_frame": round(per_launch_ms, 4),
-        "rays_per_frame": rays / args.steps,
-        "shadow_rays_per_frame": prof["shadow_rays"] / args.steps,
-        "frame_mrays_s_wall": round(rays / wall / 1e6, 2),
-        "tests_per_ray": {"box": boxes / rays, "tri": tris / rays},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_render_direct", "bytes_per_launch": per_launch_bytes},
-        "cpu_baseline": None,
-    }
-    ctx.close()
+    traffic = traffic_from_profile()
+    out = result(METRIC, mrays, "Mrays/s", args.steps, args.warmup, wall,
+                 {"workload": f"{NTRI}-triangle soup rebuilt every frame (1 deg/frame rotation), "
+                              f"{W}x{H} primary + shadow rays, 1 point light",
+                  "tris": NTRI, "depth": DEPTH, "width": W, "height": H, "lights": len(LIGHTS)},
+                 build_ms=round(build["total"], 4),
+                 build_ms_stages={k: round(v, 4) for k, v in build.items()},
+                 trace_ms_per_frame=round(per_launch_ms, 4),
+                 rays_per_frame=rays / args.steps,
+                 shadow_rays_per_frame=prof["shadow_rays"] / args.steps,
+                 frame_mrays_s_wall=round(rays / wall / 1e6, 2),
+                 tests_per_ray={"box": boxes / rays, "tri": tris / rays},
+                 roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": traffic, "kernel": "k_render_direct",
+                           "bytes_per_launch": per_launch_bytes},
+                 cpu_baseline=None)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rtcore, xyz, idx, cam)
-    print(json.dumps(out))
+    return out
+
+
+# ------------------------------------------------------------------ sphere
+def workload_sphere(args, rtcore):
+    xyz, idx = rtcore.gen_noisy_sphere(1025, 1024, 0.05, 3)
+    nt = len(idx)
+    depth = depth_for(nt)
+    lights = [(3.0, 3.0, 3.0), (-3.0, 3.0, 3.0), (3.0, 3.0, -3.0), (-3.0, -3.0, -3.0)]
+    cams = []
+    for k in range(8):
+        a = 2.0 * math.pi * k / 8
+        cams.append(rtcore.camera_look_at((2.5 * math.cos(a), 0.0, 2.5 * math.sin(a)), (0, 0, 0), (0, 1, 0),
+                                          60.0, W / H))
+    ctx = rtcore.Context(0)
+    s = ctx.add_scene(xyz, idx, depth)
+    bt = ctx.build(s, timed=True)
+    for _ in range(args.warmup):
+        for c in cams:
+            ctx.render_direct(s, c, W, H, lights, want_stats=False)
+    ctx.sync()
+    ctx.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for c in cams:
+            ctx.render_direct(s, c, W, H, lights, want_stats=False)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_end()
+    boxes = tris = 0
+    for c in cams:
+        st = ctx.render_direct(s, c, W, H, lights, flags=rtcore.FLAG_COUNT)
+        boxes += st["box_tests"]
+        tris += st["tri_tests"]
+    ctx.close()
+    rays = prof["primary_rays"] + prof["shadow_rays"]
+    per_step_rays = rays / args.steps
+    mrays = rays / (prof["trace_ms"] * 1e-3) / 1e6
+    alg = RAY_BYTES * per_step_rays + BOX_BYTES * boxes + TRI_BYTES * tris  # per step (8 frames)
+    per_launch_ms = prof["trace_ms"] / prof["launches"]
+    achieved = (alg / 8) / (per_launch_ms * 1e-3) / 1e9
+    return result(METRIC, mrays, "Mrays/s", args.steps, args.warmup, wall,
+                  {"workload": f"{nt}-triangle noise-displaced sphere (static), 8 orbit cameras "
+                               f"(radius 2.5) at {W}x{H}, primary + shadow rays to 4 point lights",
+                   "tris": nt, "depth": depth, "cameras": 8, "lights": 4},
+                  build_ms=round(bt["total_ms"], 4), trace_ms_per_frame=round(per_launch_ms, 4),
+                  rays_per_frame=per_step_rays / 8,
+                  tests_per_ray={"box": boxes / per_step_rays, "tri": tris / per_step_rays},
+                  roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                            "kernel": "k_render_direct", "bytes_per_launch": alg / 8},
+                  cpu_baseline=None)
+
+
+# ------------------------------------------------------------------ batched
+def gi_rmse_vs_fp64(ctx, rtcore, xyz, idx, depth, cam_fn, seed=7):
+    """One scene at 48x48: 1024 device samples against 2048 independent FP64
+    oracle samples per pixel; RMSE of the per-pixel mean radiance."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    w = h = 48
+    s = ctx.add_scene(xyz, idx, depth)
+    ctx.build(s)
+    cam = cam_fn(w, h)
+    acc = ctx.alloc(16 * w * h)
+    ctx.zero(acc, 16 * w * h)
+    ctx.render_jobs([ctx.job(s, cam, w, h, acc, rtcore.MODE_GI, 1024, 4, 0.7, 1.0, seed=seed)],
+                    want_stats=False)
+    g = ctx.read_accum(acc, w, h)[..., 0] / 1024
+    ctx.free(acc)
+    ctx.scene_destroy(s)
+    ref = oracle.build_scene(xyz, idx, depth)
+    r, _ = oracle.render_gi(ref["nodes"], ref["ids"], xyz, idx, cam.as_array(), w, h, 2048, 4, 0.7, 1.0,
+                            seed, first_sample=1 << 20, nthreads=min(CPU_SHARE, len(os.sched_getaffinity(0))))
+    return float(np.sqrt(np.mean((g - r / 2048) ** 2)))
+
+
+def workload_batched(args, rtcore):
+    n = 64
+    rng = np.random.default_rng(44)
+    sizes = np.exp(rng.uniform(math.log(1024), math.log(200000), n)).astype(np.int64)
+    res = rng.choice([256, 512, 1024], n)
+    scenes = [rtcore.gen_object_mix(int(sizes[k]), seed=1000 + k) for k in range(n)]
+    depths = [depth_for(len(i)) for _, i in scenes]
+    cams = [view_camera(rtcore, x, 0, int(r), int(r)) for (x, _), r in zip(scenes, res)]
+    ctx = rtcore.Context(0)
+    hs = ctx.batch_create([len(i) for _, i in scenes], [len(x) for x, _ in scenes], depths)
+    for h_, (x, i) in zip(hs, scenes):
+        ctx.scene_upload(h_, x, i)
+    ps = [ctx.add_scene(x, i, d) for (x, i), d in zip(scenes, depths)]
+    acc_b = [ctx.alloc(16 * int(r) * int(r)) for r in res]
+    acc_p = [ctx.alloc(16 * int(r) * int(r)) for r in res]
+
+    def jobs(handles, accs, frame):
+        return [ctx.job(handles[k], cams[k], int(res[k]), int(res[k]), accs[k], rtcore.MODE_GI, 4, 4, 0.7, 1.0,
+                        seed=7, frame=frame) for k in range(n)]
+
+    ctx.build(hs[0])
+    t0 = time.perf_counter()
+    tuned = ctx.autotune(jobs(hs, acc_b, 0))
+    tune_s = time.perf_counter() - t0
+
+    def zero_all():
+        for a, r in zip(acc_b + acc_p, list(res) + list(res)):
+            ctx.zero(a, 16 * int(r) * int(r))
+
+    def run_batched(frame0, steps):
+        for f in range(frame0, frame0 + steps):
+            ctx.build(hs[0])
+            ctx.render_jobs(jobs(hs, acc_b, f), persistent=True, want_stats=False)
+
+    def run_plain(frame0, steps):
+        for f in range(frame0, frame0 + steps):
+            js = jobs(ps, acc_p, f)
+            for k in range(n):
+                ctx.build(ps[k])
+                ctx.render_jobs([js[k]], persistent=False, want_stats=False)
+
+    def timed(fn):
+        fn(0, args.warmup)
+        ctx.sync()
+        ctx.profile_begin()
+        t0 = time.perf_counter()
+        fn(args.warmup, args.steps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        return ctx.profile_end(), wall
+
+    log(f"batched: autotuned {tuned} in {tune_s:.1f} s")
+    zero_all()
+    pb, wall_b = timed(run_batched)
+    log(f"batched: {wall_b:.2f} s for {args.steps} steps")
+    pp, wall_p = timed(run_plain)
+    log(f"plain: {wall_p:.2f} s for {args.steps} steps")
+    same = all(np.array_equal(ctx.read_accum(acc_b[k], int(res[k]), int(res[k])),
+                              ctx.read_accum(acc_p[k], int(res[k]), int(res[k]))) for k in range(n))
+    # the batched frame as a captured graph (rebuild + persistent launch),
+    # replayed; frame indices advance per replay inside the graph
+    g = ctx.graph_capture(hs[0], jobs(hs, acc_b, 0), rebuild=True)
+    ctx.graph_launch(g)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.graph_launch(g)
+    ctx.sync()
+    wall_g = time.perf_counter() - t0
+    ctx.graph_destroy(g)
+    rays_b = pb["primary_rays"] + pb["bounce_rays"]
+    rays_p = pp["primary_rays"] + pp["bounce_rays"]
+    mr_b = rays_b / (pb["trace_ms"] * 1e-3) / 1e6
+    mr_p = rays_p / (pp["trace_ms"] * 1e-3) / 1e6
+    k_mid = int(np.argsort(sizes)[n // 2])
+    rmse = None
+    if not args.no_cpu:
+        x, i = scenes[k_mid]
+        rmse = gi_rmse_vs_fp64(ctx, rtcore, x, i, depths[k_mid],
+                               lambda w, h: view_camera(rtcore, x, 0, w, h))
+    ctx.close()
+    return result(METRIC, mr_b, "Mrays/s", args.steps, args.warmup, wall_b,
+                  {"workload": "64 object-mix scenes (1,024..200,000 tris, log-uniform), 256^2/512^2/1024^2, "
+                               "GI 4 spp x 4 bounces, albedo 0.7; step = rebuild all + render all",
+                   "scenes": n, "tris_total": int(sizes.sum()), "pixels_total": int((res * res).sum())},
+                  plain={"mrays_s": round(mr_p, 2), "ms_per_step": round(wall_p * 1e3 / args.steps, 3),
+                         "build_ms_per_step": round(pp["build_ms"]["total"] / args.steps, 4),
+                         "trace_ms_per_step": round(pp["trace_ms"] / args.steps, 3),
+                         "launches_per_step": pp["launches"] / args.steps},
+                  batched={"mrays_s": round(mr_b, 2), "ms_per_step": round(wall_b * 1e3 / args.steps, 3),
+                           "build_ms_per_step": round(pb["build_ms"]["total"] / args.steps, 4),
+                           "trace_ms_per_step": round(pb["trace_ms"] / args.steps, 3),
+                           "graph_ms_per_step": round(wall_g * 1e3 / args.steps, 3)},
+                  speedup_trace=round(mr_b / mr_p, 3), speedup_step=round(wall_p / wall_b, 3),
+                  identical_images=bool(same), tuning=list(tuned), autotune_s=round(tune_s, 2),
+                  rays_per_step=rays_b / args.steps,
+                  gi_rmse_vs_fp64={"value": rmse, "scene_tris": int(sizes[k_mid]), "res": "48x48",
+                                   "gpu_spp": 1024, "fp64_spp": 2048}, cpu_baseline=None)
+
+
+# ------------------------------------------------------------------ serve
+def workload_serve(args, rtcore):
+    rng = np.random.default_rng(55)
+    npool = 500
+    sizes = np.exp(rng.uniform(math.log(1000), math.log(500000), npool)).astype(np.int64)
+    duration, rate = args.duration, 200.0
+    gaps = rng.exponential(1.0 / rate, int(rate * duration * 1.5))
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < duration]
+    nreq = len(arrivals)
+    scene_of = rng.integers(0, npool, nreq)
+    big = rng.random(nreq) < 0.5
+    gi = rng.random(nreq) < 0.5
+    view_of = rng.integers(0, 4, nreq)
+    pool = {}
+    for k in np.unique(scene_of):
+        pool[int(k)] = rtcore.gen_object_mix(int(sizes[k]), seed=5000 + int(k))
+    log(f"serve: {nreq} requests over {len(pool)} scenes generated")
+
+    def submit_all(srv, count):
+        for r in range(count):
+            k = int(scene_of[r])
+            xyz = pool[k][0]
+            w, h = (1920, 1080) if big[r] else (512, 512)
+            cam = view_camera(rtcore, xyz, int(view_of[r]), w, h)
+            light = tuple(xyz.max(0) + 0.5 * (xyz.max(0) - xyz.min(0)))
+            if gi[r]:
+                srv.submit(k, int(view_of[r]), cam, w, h, rtcore.MODE_GI, 1, 2, 0.7, 1.0, seed=3,
+                           arrival=float(arrivals[r]))
+            else:
+                srv.submit(k, int(view_of[r]), cam, w, h, rtcore.MODE_DIRECT, 1, 0, 0.8, 0.0, 0.1,
+                           lights=[light], seed=3, arrival=float(arrivals[r]))
+
+    def serve(ctx, count, inflight, tiles):
+        srv = rtcore.Server(ctx, 200 << 30, max_inflight=inflight, tiles_per_step=tiles)
+        for k, (x, i) in pool.items():
+            srv.add_scene(k, x, i, depth_for(len(i)))
+        submit_all(srv, count)
+        t0 = time.perf_counter()
+        res, t_end = srv.run(list(arrivals[:count]),
+                             progress=lambda t, st: log(f"serve: clock {t:.1f} s, {st['completed']} done, "
+                                                        f"{st['pending']} pending"))
+        wall = time.perf_counter() - t0
+        st = srv.status()
+        srv.close()
+        ok = [r for r in res if r["status"] == rtcore.RT_OK]
+        lat = np.array([r["completed"] - r["arrival"] for r in ok])
+        return {"requests": count, "completed": len(ok), "rejected": st["rejected"],
+                "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                "mrays_s_device": round(st["rays"] / st["device_s"] / 1e6, 2),
+                "mrays_s_clock": round(st["rays"] / t_end / 1e6, 2),
+                "device_busy_frac": round(st["device_s"] / t_end, 4), "clock_s": round(t_end, 3),
+                "wall_s": round(wall, 2), "steps": st["steps"], "builds": st["builds"],
+                "evictions": st["evictions"], "rays": st["rays"]}
+
+    ctx = rtcore.Context(0)
+    # warm-up: compile paths, first allocations
+    srv = rtcore.Server(ctx, 1 << 34, 4, 4096)
+    k0 = int(scene_of[0])
+    srv.add_scene(k0, *pool[k0], depth_for(len(pool[k0][1])))
+    srv.submit(k0, 0, view_camera(rtcore, pool[k0][0], 0, 256, 256), 256, 256, arrival=0.0)
+    srv.run([0.0])
+    srv.close()
+    served = serve(ctx, nreq, 32, 32768)
+    log(f"serve: continuous batching done: {served}")
+    nplain = min(nreq, int(rate * args.plain_seconds))
+    plain = serve(ctx, nplain, 1, 1 << 30)
+    log(f"serve: one request per launch done: {plain}")
+    ctx.close()
+    return result(METRIC, served["mrays_s_device"], "Mrays/s", nreq, 1, served["wall_s"],
+                  {"workload": f"Poisson {rate:.0f} req/s for {duration:.0f} s, pool of {npool} object-mix "
+                               "scenes (1k..500k tris, log-uniform), 512^2 or 1920x1080, direct (1 light) or "
+                               "2-bounce GI at 1 spp, progressive accumulation per (scene, view, size, mode), "
+                               "200 GB device budget",
+                   "requests": nreq, "pool": npool},
+                  served=served, plain_one_request_per_launch=plain,
+                  speedup_device=round(served["mrays_s_device"] / plain["mrays_s_device"], 3),
+                  p50_ms=served["p50_ms"], p99_ms=served["p99_ms"],
+                  clock="simulated: arrivals on a virtual clock; each step advances it by the step's measured "
+                        "wall time (admission, builds, synchronised launch); idle gaps jump to the next arrival",
+                  cpu_baseline=None)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", choices=["frame", "sphere", "batched", "serve"], default="frame")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
+    ap.add_argument("--plain-seconds", type=float, default=10.0,
+                    help="serve: seconds of the same arrivals replayed one request per launch")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the traversal shards by "
+              f"image tile / scene across GPUs, which this build does not run", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+    defaults = {"frame": (100, 5), "sphere": (10, 1), "batched": (3, 1), "serve": (0, 0)}
+    ds, dw = defaults[args.workload]
+    args.steps = ds if args.steps is None else args.steps
+    args.warmup = dw if args.warmup is None else args.warmup
+
+    # torch first: its HIP runtime is then the one librtcore.so binds to
+    import torch
+
+    import rtcore
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    rtcore.lib()
+    fn = {"frame": workload_frame, "sphere": workload_sphere, "batched": workload_batched,
+          "serve": workload_serve}[args.workload]
+    print(json.dumps(fn(args, rtcore)), flush=True)
 
 
 if __name__ == "__main__":

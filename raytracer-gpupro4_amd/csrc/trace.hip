@@ -70,6 +70,134 @@ __device__ __forceinline__ void box_test(const float *bx, float ox, float oy, fl
     t1 = b;
 }
 
+// Traversal state of one lane.  trav_begin tests the root box; each
+// trav_step then handles one internal node (both child boxes) or one leaf and
+// pops the stack.  A ray is finished when either call returns true; its
+// result is in r.
+struct Trav {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+    int32_t self, cur;
+    uint32_t neg;  // bit 0: 1/dx < 0, bit 1: 1/dy < 0, bit 2: 1/dz < 0
+    bool anyhit;
+    TraceOut r;
+};
+
+template <class Stack>
+__device__ __forceinline__ bool trav_begin(Trav &T, int32_t root_ref, const float *root_box, float ox,
+                                           float oy, float oz, float dx, float dy, float dz, float tmax,
+                                           int32_t self, bool anyhit, Stack &stk)
+{
+    T.r.tri = -1;
+    T.r.u = -1.0f;
+    T.r.v = -1.0f;
+    T.r.t = tmax;
+    T.r.boxes = 0;
+    T.r.tris = 0;
+    T.r.leaf_heap = 1;
+    T.r.pos = -1;
+    T.ox = ox;
+    T.oy = oy;
+    T.oz = oz;
+    T.dx = dx;
+    T.dy = dy;
+    T.dz = dz;
+    T.self = self;
+    T.anyhit = anyhit;
+    if (root_ref == -1) return true;
+    T.ix = 1.0f / dx;
+    T.iy = 1.0f / dy;
+    T.iz = 1.0f / dz;
+    T.neg = (T.ix < 0.0f ? 1u : 0u) | (T.iy < 0.0f ? 2u : 0u) | (T.iz < 0.0f ? 4u : 0u);
+    float t0, t1;
+    box_test(root_box, ox, oy, oz, T.ix, T.iy, T.iz, t0, t1);
+    T.r.boxes = 1;
+    if (t0 > T.r.t || t1 < 0.0f) return true;
+    T.cur = root_ref;
+    stk.sp = 0;
+    return false;
+}
+
+// MODE 0: closest hit, 1: any hit, 2: as T.anyhit says.
+template <int MODE, class Stack>
+__device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ nodes,
+                                          const TravTri *__restrict__ tris, Stack &stk)
+{
+    TraceOut &r = T.r;
+    if (T.cur >= 0) {
+        const TravNode *n = &nodes[T.cur];
+        const float4 q0 = *reinterpret_cast<const float4 *>(&n->box[0]);
+        const float4 q1 = *reinterpret_cast<const float4 *>(&n->box[4]);
+        const float4 q2 = *reinterpret_cast<const float4 *>(&n->box[8]);
+        const int4 q3 = *reinterpret_cast<const int4 *>(&n->ref[0]);
+        const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+        const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+        float a0, a1, b0, b1;
+        box_test(ba, T.ox, T.oy, T.oz, T.ix, T.iy, T.iz, a0, a1);
+        box_test(bb, T.ox, T.oy, T.oz, T.ix, T.iy, T.iz, b0, b1);
+        r.boxes += 2;
+        const int first_is_b = (T.neg >> q3.z) & 1u;
+        const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
+        const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+        const int32_t fref = first_is_b ? q3.y : q3.x;
+        const int32_t sref = first_is_b ? q3.x : q3.y;
+        const bool f_ok = !(f0 > r.t || f1 < 0.0f);
+        const bool s_live = !(s1 < 0.0f);
+        if (f_ok) {
+            if (s_live) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+            T.cur = fref;
+            return false;
+        }
+        if (s_live && !(s0 > r.t)) {
+            T.cur = sref;
+            return false;
+        }
+    } else {
+        const uint32_t pos = (uint32_t)T.cur & ~kLeafBit;
+        const TravTri *tp = &tris[pos];
+        const float dx = T.dx, dy = T.dy, dz = T.dz;
+        int32_t cnt = 1;
+        for (int32_t j = 0; j < cnt; ++j) {
+            const float4 a = tp[j].a;
+            const float4 e1 = tp[j].e1;
+            const float4 e2 = tp[j].e2;
+            if (j == 0) cnt = __float_as_int(e1.w);
+            r.tris += 1;
+            const int32_t id = __float_as_int(a.w);
+            // getIntersection (Intersection.hlsl:47-64)
+            const float px = (dy * e2.z) - (dz * e2.y);
+            const float py = (dz * e2.x) - (dx * e2.z);
+            const float pz = (dx * e2.y) - (dy * e2.x);
+            const float dd = (e1.x * px + e1.y * py) + e1.z * pz;
+            const float det = 1.0f / dd;
+            const float tx = T.ox - a.x, ty = T.oy - a.y, tz = T.oz - a.z;
+            const float u = ((tx * px + ty * py) + tz * pz) * det;
+            const float qx = (ty * e1.z) - (tz * e1.y);
+            const float qy = (tz * e1.x) - (tx * e1.z);
+            const float qz = (tx * e1.y) - (ty * e1.x);
+            const float v = ((dx * qx + dy * qy) + dz * qz) * det;
+            const float t = ((e2.x * qx + e2.y * qy) + e2.z * qz) * det;
+            if (id != T.self && u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f && t >= 0.0f && t < r.t) {
+                r.tri = id;
+                r.u = u;
+                r.v = v;
+                r.t = t;
+                r.leaf_heap = __float_as_int(tp[0].e2.w);
+                r.pos = (int32_t)pos + j;
+                if (MODE == 1 || (MODE == 2 && T.anyhit)) return true;
+            }
+        }
+    }
+    // pop the next live entry
+    while (stk.sp > 0) {
+        const uint2 e = stk.pop();
+        if (!(__uint_as_float(e.y) > r.t)) {
+            T.cur = (int32_t)e.x;
+            return false;
+        }
+    }
+    return true;
+}
+
 template <bool ANYHIT, class Stack>
 __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
                                              const TravTri *__restrict__ tris, int32_t root_ref,
@@ -77,103 +205,11 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
                                              float dx, float dy, float dz, float tmax, int32_t self,
                                              Stack &stk)
 {
-    TraceOut r;
-    r.tri = -1;
-    r.u = -1.0f;
-    r.v = -1.0f;
-    r.t = tmax;
-    r.boxes = 0;
-    r.tris = 0;
-    r.leaf_heap = 1;
-    r.pos = -1;
-    if (root_ref == -1) return r;
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-    const int negx = ix < 0.0f, negy = iy < 0.0f, negz = iz < 0.0f;
-    float t0, t1;
-    box_test(root_box, ox, oy, oz, ix, iy, iz, t0, t1);
-    r.boxes = 1;
-    if (t0 > r.t || t1 < 0.0f) return r;
-    int32_t cur = root_ref;
-    stk.sp = 0;
-    while (true) {
-        if (cur >= 0) {
-            const TravNode *n = &nodes[cur];
-            const float4 q0 = *reinterpret_cast<const float4 *>(&n->box[0]);
-            const float4 q1 = *reinterpret_cast<const float4 *>(&n->box[4]);
-            const float4 q2 = *reinterpret_cast<const float4 *>(&n->box[8]);
-            const int4 q3 = *reinterpret_cast<const int4 *>(&n->ref[0]);
-            const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-            const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-            float a0, a1, b0, b1;
-            box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
-            box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
-            r.boxes += 2;
-            const int axis = q3.z;
-            const int first_is_b = axis == 0 ? negx : (axis == 1 ? negy : negz);
-            const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
-            const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
-            const int32_t fref = first_is_b ? q3.y : q3.x;
-            const int32_t sref = first_is_b ? q3.x : q3.y;
-            const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-            const bool s_live = !(s1 < 0.0f);
-            if (f_ok) {
-                if (s_live) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
-                cur = fref;
-                continue;
-            }
-            if (s_live && !(s0 > r.t)) {
-                cur = sref;
-                continue;
-            }
-        } else {
-            const uint32_t pos = (uint32_t)cur & ~kLeafBit;
-            const TravTri *tp = &tris[pos];
-            int32_t cnt = 1;
-            for (int32_t j = 0; j < cnt; ++j) {
-                const float4 a = tp[j].a;
-                const float4 e1 = tp[j].e1;
-                const float4 e2 = tp[j].e2;
-                if (j == 0) cnt = __float_as_int(e1.w);
-                r.tris += 1;
-                const int32_t id = __float_as_int(a.w);
-                // getIntersection (Intersection.hlsl:47-64)
-                const float px = (dy * e2.z) - (dz * e2.y);
-                const float py = (dz * e2.x) - (dx * e2.z);
-                const float pz = (dx * e2.y) - (dy * e2.x);
-                const float dd = (e1.x * px + e1.y * py) + e1.z * pz;
-                const float det = 1.0f / dd;
-                const float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
-                const float u = ((tx * px + ty * py) + tz * pz) * det;
-                const float qx = (ty * e1.z) - (tz * e1.y);
-                const float qy = (tz * e1.x) - (tx * e1.z);
-                const float qz = (tx * e1.y) - (ty * e1.x);
-                const float v = ((dx * qx + dy * qy) + dz * qz) * det;
-                const float t = ((e2.x * qx + e2.y * qy) + e2.z * qz) * det;
-                if (id != self && u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f && t >= 0.0f &&
-                    t < r.t) {
-                    r.tri = id;
-                    r.u = u;
-                    r.v = v;
-                    r.t = t;
-                    r.leaf_heap = __float_as_int(tp[0].e2.w);
-                    r.pos = (int32_t)pos + j;
-                    if (ANYHIT) return r;
-                }
-            }
+    Trav T;
+    if (!trav_begin(T, root_ref, root_box, ox, oy, oz, dx, dy, dz, tmax, self, ANYHIT, stk))
+        while (!trav_step<ANYHIT ? 1 : 0>(T, nodes, tris, stk)) {
         }
-        // pop the next live entry
-        bool found = false;
-        while (stk.sp > 0) {
-            const uint2 e = stk.pop();
-            if (!(__uint_as_float(e.y) > r.t)) {
-                cur = (int32_t)e.x;
-                found = true;
-                break;
-            }
-        }
-        if (!found) break;
-    }
-    return r;
+    return T.r;
 }
 
 struct SceneView {
@@ -344,9 +380,23 @@ struct RenderParams {
     float lights[24];  // up to 8 point lights
 };
 
+// A block renders a tile of 16 x BLOCK/16 pixels.  With two or more waves
+// per block each wave takes an 8 x 8 square of it (a smaller footprint than a
+// 16 x 4 strip, so its rays stay together deeper into the tree).
 template <int BLOCK>
 struct Tile {
     static constexpr int W = 16, H = BLOCK / 16;
+    static __device__ __forceinline__ void offset(uint32_t t, uint32_t &x, uint32_t &y)
+    {
+        if (BLOCK >= 128) {
+            const uint32_t w = t >> 6, l = t & 63u;
+            x = (w & 1u) * 8u + (l & 7u);
+            y = (w >> 1) * 8u + (l >> 3);
+        } else {
+            x = t % W;
+            y = t / W;
+        }
+    }
 };
 
 // Primary + shadow rays + direct term for one image, one tile per block.
@@ -363,8 +413,10 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
     constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
     const uint32_t tiles_x = (rp.width + TW - 1) / TW;
     const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-    const uint32_t px = tx * TW + (threadIdx.x % TW);
-    const uint32_t py = ty * TH + (threadIdx.x / TW);
+    uint32_t ox_, oy_;
+    Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
+    const uint32_t px = tx * TW + ox_;
+    const uint32_t py = ty * TH + oy_;
     const bool valid = px < rp.width && py < rp.height;
     int32_t root_ref;
     float root_box[6];
@@ -512,8 +564,10 @@ __global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restri
         }
         const RenderJob &J = jobs[lo];
         const uint32_t lt = tile - J.tile_off + J.local_base;
-        const uint32_t px = (lt % J.tiles_x) * TW + (threadIdx.x % TW);
-        const uint32_t py = (lt / J.tiles_x) * TH + (threadIdx.x / TW);
+        uint32_t ox_, oy_;
+        Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
+        const uint32_t px = (lt % J.tiles_x) * TW + ox_;
+        const uint32_t py = (lt / J.tiles_x) * TH + oy_;
         if (px < J.width && py < J.height) {
             const SceneView sv{J.nodes, J.tris, J.hdr};
             int32_t root_ref;

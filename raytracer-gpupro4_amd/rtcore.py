@@ -581,15 +581,22 @@ class Server:
                                                 C.byref(n)), "rt_server_read_stream")
         return out, n.value
 
-    def run(self, arrivals, now: float = 0.0, max_steps: int = 10_000_000) -> tuple:
+    def run(self, arrivals, now: float = 0.0, max_steps: int = 10_000_000, progress=None) -> tuple:
         """Steps on a simulated clock until every submitted request is
         finished: each step advances the clock by its duration, an idle server
         jumps to the next arrival.  arrivals: the submitted arrival times.
-        Returns (results, end time)."""
+        progress (optional) is called with (clock, status) about every 10 s of
+        wall time.  Returns (results, end time)."""
+        import time as _time
+
         pending_t = sorted(arrivals)
         results, t, k = [], now, 0
+        next_report = _time.perf_counter() + 10.0
         for _ in range(max_steps):
             st = self.status()
+            if progress is not None and _time.perf_counter() >= next_report:
+                progress(t, st)
+                next_report += 10.0
             if st["pending"] == 0 and st["inflight"] == 0:
                 break
             dt = self.step(t)
