@@ -156,6 +156,8 @@ def workload_frame(args, rtcore):
     cam = rtcore.camera_look_at(CAM_POS, CAM_TGT, (0, 1, 0), FOV, W / H)
     ctx = rtcore.Context(0)
     s = ctx.add_scene(xyz, idx, DEPTH)
+    ctx.build(s)
+    tuned = ctx.autotune_direct(s, cam, W, H, LIGHTS)  # launch shape for this frame, before timing
     for f in range(args.warmup):
         frame(ctx, rtcore, s, cam, f)
     device_sync(ctx)
@@ -199,6 +201,7 @@ def workload_frame(args, rtcore):
                  shadow_rays_per_frame=prof["shadow_rays"] / args.steps,
                  frame_mrays_s_wall=round(rays / wall / 1e6, 2),
                  tests_per_ray={"box": boxes / rays, "tri": tris / rays},
+                 tuning=list(tuned),
                  roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                            "traffic": traffic, "kernel": "k_render_direct",
@@ -223,6 +226,7 @@ def workload_sphere(args, rtcore):
     ctx = rtcore.Context(0)
     s = ctx.add_scene(xyz, idx, depth)
     bt = ctx.build(s, timed=True)
+    tuned = ctx.autotune_direct(s, cams[0], W, H, lights)
     for _ in range(args.warmup):
         for c in cams:
             ctx.render_direct(s, c, W, H, lights, want_stats=False)
@@ -252,6 +256,7 @@ def workload_sphere(args, rtcore):
                                f"(radius 2.5) at {W}x{H}, primary + shadow rays to 4 point lights",
                    "tris": nt, "depth": depth, "cameras": 8, "lights": 4},
                   build_ms=round(bt["total_ms"], 4), trace_ms_per_frame=round(per_launch_ms, 4),
+                  tuning=list(tuned),
                   rays_per_frame=per_step_rays / 8,
                   tests_per_ray={"box": boxes / per_step_rays, "tri": tris / per_step_rays},
                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -403,68 +408,119 @@ def workload_serve(args, rtcore):
         pool[int(k)] = rtcore.gen_object_mix(int(sizes[k]), seed=5000 + int(k))
     log(f"serve: {nreq} requests over {len(pool)} scenes generated")
 
-    def submit_all(srv, count):
-        for r in range(count):
-            k = int(scene_of[r])
-            xyz = pool[k][0]
-            w, h = (1920, 1080) if big[r] else (512, 512)
-            cam = view_camera(rtcore, xyz, int(view_of[r]), w, h)
-            light = tuple(xyz.max(0) + 0.5 * (xyz.max(0) - xyz.min(0)))
-            if gi[r]:
-                srv.submit(k, int(view_of[r]), cam, w, h, rtcore.MODE_GI, 1, 2, 0.7, 1.0, seed=3,
-                           arrival=float(arrivals[r]))
-            else:
-                srv.submit(k, int(view_of[r]), cam, w, h, rtcore.MODE_DIRECT, 1, 0, 0.8, 0.0, 0.1,
-                           lights=[light], seed=3, arrival=float(arrivals[r]))
+    def request(r):
+        """(scene, view, w, h, mode, bounces, camera, lights) of request r."""
+        k = int(scene_of[r])
+        xyz = pool[k][0]
+        w, h = (1920, 1080) if big[r] else (512, 512)
+        cam = view_camera(rtcore, xyz, int(view_of[r]), w, h)
+        if gi[r]:
+            return k, int(view_of[r]), w, h, rtcore.MODE_GI, 2, cam, None
+        light = tuple(xyz.max(0) + 0.5 * (xyz.max(0) - xyz.min(0)))
+        return k, int(view_of[r]), w, h, rtcore.MODE_DIRECT, 0, cam, [light]
 
-    def serve(ctx, count, inflight, tiles):
-        srv = rtcore.Server(ctx, 200 << 30, max_inflight=inflight, tiles_per_step=tiles)
+    def served(ctx, arr):
+        srv = rtcore.Server(ctx, 200 << 30, max_inflight=32, tiles_per_step=32768)
         for k, (x, i) in pool.items():
             srv.add_scene(k, x, i, depth_for(len(i)))
-        submit_all(srv, count)
+        for r, t in enumerate(arr):
+            k, v, w, h, mode, nb, cam, lights = request(r)
+            if mode == rtcore.MODE_GI:
+                srv.submit(k, v, cam, w, h, mode, 1, nb, 0.7, 1.0, seed=3, arrival=float(t))
+            else:
+                srv.submit(k, v, cam, w, h, mode, 1, 0, 0.8, 0.0, 0.1, lights=lights, seed=3, arrival=float(t))
         t0 = time.perf_counter()
-        res, t_end = srv.run(list(arrivals[:count]),
-                             progress=lambda t, st: log(f"serve: clock {t:.1f} s, {st['completed']} done, "
-                                                        f"{st['pending']} pending"))
+        res, t_end = srv.run(list(arr), progress=lambda t, st: log(
+            f"serve: clock {t:.1f} s, {st['completed']} done, {st['pending']} pending"))
         wall = time.perf_counter() - t0
         st = srv.status()
         srv.close()
         ok = [r for r in res if r["status"] == rtcore.RT_OK]
         lat = np.array([r["completed"] - r["arrival"] for r in ok])
-        return {"requests": count, "completed": len(ok), "rejected": st["rejected"],
+        return {"requests": len(arr), "completed": len(ok), "rejected": int(st["rejected"]),
                 "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                 "mrays_s_device": round(st["rays"] / st["device_s"] / 1e6, 2),
                 "mrays_s_clock": round(st["rays"] / t_end / 1e6, 2),
                 "device_busy_frac": round(st["device_s"] / t_end, 4), "clock_s": round(t_end, 3),
-                "wall_s": round(wall, 2), "steps": st["steps"], "builds": st["builds"],
-                "evictions": st["evictions"], "rays": st["rays"]}
+                "wall_s": round(wall, 2), "steps": int(st["steps"]), "builds": int(st["builds"]),
+                "evictions": int(st["evictions"]), "rays": int(st["rays"])}
+
+    def plain(ctx, arr):
+        """The plain per-frame path: requests in arrival order, one at a time,
+        each image rendered whole by its own launch (one block per tile),
+        scenes built on first use and kept, one accumulator per stream."""
+        scenes, accs = {}, {}
+        t = dev = 0.0
+        rays = 0
+        lat = []
+        for r, ta in enumerate(arr):
+            k, v, w, h, mode, nb, cam, lights = request(r)
+            t = max(t, float(ta))
+            t0 = time.perf_counter()
+            if k not in scenes:
+                x, i = pool[k]
+                scenes[k] = ctx.add_scene(x, i, depth_for(len(i)))
+                ctx.build(scenes[k])
+            key = (k, v, w, h, mode)
+            if key not in accs:
+                accs[key] = [ctx.alloc(16 * w * h), 0]
+                ctx.zero(accs[key][0], 16 * w * h)
+            acc = accs[key]
+            if mode == rtcore.MODE_GI:
+                job = ctx.job(scenes[k], cam, w, h, acc[0], mode, 1, nb, 0.7, 1.0, seed=3, frame=acc[1])
+            else:
+                job = ctx.job(scenes[k], cam, w, h, acc[0], mode, 1, 0, 0.8, 0.0, 0.1, lights=lights, seed=3,
+                              frame=acc[1])
+            st = ctx.render_jobs([job], persistent=False)
+            acc[1] += 1
+            t += time.perf_counter() - t0
+            dev += st["trace_ms"] * 1e-3
+            rays += st["primary_rays"] + st["shadow_rays"] + st["bounce_rays"]
+            lat.append(t - float(ta))
+        for a, _ in accs.values():
+            ctx.free(a)
+        for hnd in scenes.values():
+            ctx.scene_destroy(hnd)
+        lat = np.array(lat)
+        return {"requests": len(arr), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                "mrays_s_device": round(rays / dev / 1e6, 2), "mrays_s_clock": round(rays / t / 1e6, 2),
+                "device_busy_frac": round(dev / t, 4), "clock_s": round(t, 3), "rays": int(rays)}
 
     ctx = rtcore.Context(0)
-    # warm-up: compile paths, first allocations
+    # warm-up: first launches of every kernel shape used below
     srv = rtcore.Server(ctx, 1 << 34, 4, 4096)
     k0 = int(scene_of[0])
     srv.add_scene(k0, *pool[k0], depth_for(len(pool[k0][1])))
     srv.submit(k0, 0, view_camera(rtcore, pool[k0][0], 0, 256, 256), 256, 256, arrival=0.0)
     srv.run([0.0])
     srv.close()
-    served = serve(ctx, nreq, 32, 32768)
-    log(f"serve: continuous batching done: {served}")
+    s_poisson = served(ctx, arrivals)
+    log(f"serve: continuous batching, Poisson arrivals: {s_poisson}")
     nplain = min(nreq, int(rate * args.plain_seconds))
-    plain = serve(ctx, nplain, 1, 1 << 30)
-    log(f"serve: one request per launch done: {plain}")
+    p_poisson = plain(ctx, arrivals[:nplain])
+    log(f"serve: plain path, Poisson arrivals: {p_poisson}")
+    nburst = min(nreq, args.burst)
+    s_burst = served(ctx, np.zeros(nburst))
+    log(f"serve: continuous batching, burst: {s_burst}")
+    p_burst = plain(ctx, np.zeros(nburst))
+    log(f"serve: plain path, burst: {p_burst}")
     ctx.close()
-    return result(METRIC, served["mrays_s_device"], "Mrays/s", nreq, 1, served["wall_s"],
+    return result(METRIC, s_poisson["mrays_s_device"], "Mrays/s", nreq, 1, s_poisson["wall_s"],
                   {"workload": f"Poisson {rate:.0f} req/s for {duration:.0f} s, pool of {npool} object-mix "
                                "scenes (1k..500k tris, log-uniform), 512^2 or 1920x1080, direct (1 light) or "
                                "2-bounce GI at 1 spp, progressive accumulation per (scene, view, size, mode), "
                                "200 GB device budget",
                    "requests": nreq, "pool": npool},
-                  served=served, plain_one_request_per_launch=plain,
-                  speedup_device=round(served["mrays_s_device"] / plain["mrays_s_device"], 3),
-                  p50_ms=served["p50_ms"], p99_ms=served["p99_ms"],
-                  clock="simulated: arrivals on a virtual clock; each step advances it by the step's measured "
-                        "wall time (admission, builds, synchronised launch); idle gaps jump to the next arrival",
+                  p50_ms=s_poisson["p50_ms"], p99_ms=s_poisson["p99_ms"],
+                  served=s_poisson, plain=p_poisson,
+                  burst={"requests": nburst, "served": s_burst, "plain": p_burst,
+                         "speedup_clock": round(s_burst["mrays_s_clock"] / p_burst["mrays_s_clock"], 3),
+                         "speedup_device": round(s_burst["mrays_s_device"] / p_burst["mrays_s_device"], 3)},
+                  clock="virtual: each step (served) or request (plain) advances it by the measured wall "
+                        "time of its library calls (admission, builds, synchronised launch); an idle "
+                        "server jumps to the next arrival",
                   cpu_baseline=None)
 
 
@@ -477,6 +533,8 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--plain-seconds", type=float, default=10.0,
                     help="serve: seconds of the same arrivals replayed one request per launch")
+    ap.add_argument("--burst", type=int, default=2000,
+                    help="serve: requests offered at once to compare saturated throughput")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -250,6 +250,12 @@ int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
  * earlier results keyed by the jobs' shape; a hit skips the search. */
 int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
                      const char *cache_path, rt_tuning *best);
+/* Times every launch shape on rt_render_direct of this scene and frame
+ * (renders into the context's frame buffer) and keeps the fastest; the
+ * blocks_per_cu field is left as it is.  cache_path as for rt_autotune_jobs,
+ * keyed by triangle count, depth, image size and light count. */
+int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc,
+                       const char *cache_path, rt_tuning *best);
 /* Captures rebuild (optional) + one persistent rt_render_jobs launch of these
  * jobs as a HIP graph; each rt_graph_launch replays it with frame + n for the
  * n-th replay.  Jobs must name scenes of the batch of `scene`. */

@@ -900,6 +900,63 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
     return RT_OK;
 }
 
+int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const char *cache_path,
+                       rt_tuning *best)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i) || !desc) return fail(ctx, RT_ERR_ARG, "rt_autotune_direct: bad argument");
+    if (ctx->profiling) return fail(ctx, RT_ERR_STATE, "rt_autotune_direct: inside a profiling window");
+    char key[160];
+    snprintf(key, sizeof(key), "direct:cu%d:t%u:d%u:%ux%u:l%u", ctx->cu_count, b->desc[i].nt, b->desc[i].depth,
+             desc->width, desc->height, desc->nlights);
+    if (cache_path) {
+        if (FILE *f = fopen(cache_path, "r")) {
+            char k[256];
+            rt_tuning t;
+            while (fscanf(f, "%255s %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu) == 4) {
+                if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK) {
+                    fclose(f);
+                    ctx->tune = t;
+                    if (best) *best = t;
+                    return RT_OK;
+                }
+            }
+            fclose(f);
+        }
+    }
+    const rt_tuning saved = ctx->tune;
+    rt_tuning bt = saved;
+    float bms = 1e30f;
+    int rc = RT_OK;
+    for (const rt_tuning &shape : kShapes) {
+        rt_tuning t = shape;
+        t.blocks_per_cu = saved.blocks_per_cu;
+        ctx->tune = t;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3 && rc == RT_OK; ++rep) {  // first run warms caches
+            rt_frame_stats st;
+            rc = rt_render_direct(ctx, scene, desc, 0, &st);
+            if (rep > 0 && st.trace_ms < ms) ms = st.trace_ms;
+        }
+        if (rc) break;
+        if (ms < bms) {
+            bms = ms;
+            bt = t;
+        }
+    }
+    ctx->tune = rc ? saved : bt;
+    if (rc) return rc;
+    if (best) *best = bt;
+    if (cache_path) {
+        if (FILE *f = fopen(cache_path, "a")) {
+            fprintf(f, "%s %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu);
+            fclose(f);
+        }
+    }
+    return RT_OK;
+}
+
 int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_desc *jobs,
                            uint32_t njobs, int *graph)
 {

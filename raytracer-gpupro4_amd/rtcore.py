@@ -179,6 +179,7 @@ def lib() -> C.CDLL:
         "rt_tuning_set": (I32, [P, C.POINTER(Tuning)]),
         "rt_render_jobs": (I32, [P, P, U32, U32, C.POINTER(FrameStats)]),
         "rt_autotune_jobs": (I32, [P, P, U32, C.c_char_p, C.POINTER(Tuning)]),
+        "rt_autotune_direct": (I32, [P, I32, C.POINTER(RenderDesc), C.c_char_p, C.POINTER(Tuning)]),
         "rt_graph_capture_frame": (I32, [P, I32, I32, P, U32, C.POINTER(I32)]),
         "rt_graph_launch": (I32, [P, I32]),
         "rt_graph_destroy": (I32, [P, I32]),
@@ -384,6 +385,21 @@ class Context:
         t = Tuning()
         cp = cache_path.encode() if cache_path else None
         self.check(self.L.rt_autotune_jobs(self.h, arr, len(jobs), cp, C.byref(t)), "rt_autotune_jobs")
+        return t.as_tuple()
+
+    def autotune_direct(self, scene, cam, width, height, lights, ambient=0.1, albedo=0.8, env=0.0,
+                        cache_path=None):
+        """Picks the fastest launch shape for rt_render_direct of this frame."""
+        lights = np.ascontiguousarray(np.asarray(lights, np.float32).reshape(-1))
+        d = RenderDesc()
+        d.cam = cam
+        d.width, d.height = width, height
+        d.lights = lights.ctypes.data_as(C.POINTER(C.c_float))
+        d.nlights = lights.size // 3
+        d.ambient, d.albedo, d.env = ambient, albedo, env
+        t = Tuning()
+        cp = cache_path.encode() if cache_path else None
+        self.check(self.L.rt_autotune_direct(self.h, scene, C.byref(d), cp, C.byref(t)), "rt_autotune_direct")
         return t.as_tuple()
 
     def graph_capture(self, scene, jobs, rebuild=True) -> int:

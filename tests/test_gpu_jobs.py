@@ -215,3 +215,18 @@ def test_tile_ranges_compose_to_whole_image(ctx, rtcore, mix):
         ctx.render_jobs(jobs, persistent=persistent, want_stats=False)
         assert np.array_equal(ctx.read_accum(acc, W, H), whole), persistent
     ctx.free(acc)
+
+
+def test_autotune_direct_picks_and_caches(ctx, rtcore, mix, tmp_path):
+    s, xyz, idx, ref, cam = mix
+    cache = str(tmp_path / "tune_direct.txt")
+    saved = ctx.tuning()
+    try:
+        best = ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache)
+        assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] == saved[2]
+        assert ctx.tuning() == best
+        assert open(cache).read().split()[-3:] == [str(v) for v in best]
+        ctx.set_tuning(64, 8, saved[2])
+        assert ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache) == best  # cache hit
+    finally:
+        ctx.set_tuning(*saved)
