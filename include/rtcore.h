@@ -120,6 +120,8 @@ typedef struct rt_tuning {
     int32_t block;         /* threads per block: 64, 128 or 256 (tile 16 x block/16) */
     int32_t lds_stack;     /* stack entries per lane kept in LDS (8..26); deeper ones spill */
     int32_t blocks_per_cu; /* resident blocks per CU for persistent job launches */
+    int32_t traversal;     /* 0: loop form, 1: step form (same visit order and results;
+                              the faster one depends on the scene) */
 } rt_tuning;
 
 /* One image of one scene for rt_render_jobs: spp samples per pixel are added
@@ -250,9 +252,9 @@ int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
  * earlier results keyed by the jobs' shape; a hit skips the search. */
 int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
                      const char *cache_path, rt_tuning *best);
-/* Times every launch shape on rt_render_direct of this scene and frame
- * (renders into the context's frame buffer) and keeps the fastest; the
- * blocks_per_cu field is left as it is.  cache_path as for rt_autotune_jobs,
+/* Times every launch shape and traversal form on rt_render_direct of this
+ * scene and frame (renders into the context's frame buffer) and keeps the
+ * fastest; the blocks_per_cu field is left as it is.  cache_path as for rt_autotune_jobs,
  * keyed by triangle count, depth, image size and light count. */
 int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc,
                        const char *cache_path, rt_tuning *best);

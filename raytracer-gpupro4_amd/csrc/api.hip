@@ -680,12 +680,13 @@ static int check_tuning(rt_ctx *ctx, const rt_tuning &t)
     if (t.lds_stack < 1 || t.lds_stack > 26) return fail(ctx, RT_ERR_ARG, "tuning: lds_stack in 1..26");
     if (t.blocks_per_cu < 1 || t.blocks_per_cu > 32)
         return fail(ctx, RT_ERR_ARG, "tuning: blocks_per_cu in 1..32");
+    if (t.traversal != 0 && t.traversal != 1) return fail(ctx, RT_ERR_ARG, "tuning: traversal 0 or 1");
     return RT_OK;
 }
 
 // The instantiated launch shapes, in the order of rt::tune_index.
-static const rt_tuning kShapes[] = {{64, 8, 4},   {64, 16, 4},  {128, 8, 4},  {128, 12, 4},
-                                    {128, 16, 4}, {128, 26, 4}, {256, 8, 4},  {256, 12, 4}};
+static const rt_tuning kShapes[] = {{64, 8, 4, 0},   {64, 16, 4, 0},  {128, 8, 4, 0},  {128, 12, 4, 0},
+                                    {128, 16, 4, 0}, {128, 26, 4, 0}, {256, 8, 4, 0},  {256, 12, 4, 0}};
 
 int rt_tuning_get(rt_ctx *ctx, rt_tuning *t)
 {
@@ -702,6 +703,7 @@ int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t)
     // snap to the nearest instantiated shape; keep blocks_per_cu as given
     rt_tuning s = kShapes[rt::tune_index(*t)];
     s.blocks_per_cu = t->blocks_per_cu;
+    s.traversal = t->traversal;
     ctx->tune = s;
     return RT_OK;
 }
@@ -844,7 +846,7 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
         if (FILE *f = fopen(cache_path, "r")) {
             char k[256];
             rt_tuning t;
-            while (fscanf(f, "%255s %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu) == 4) {
+            while (fscanf(f, "%255s %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal) == 5) {
                 if (key == k && check_tuning(ctx, t) == RT_OK) {
                     fclose(f);
                     ctx->tune = t;
@@ -870,21 +872,24 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
     float bms = 1e30f;
     const int bpcs[] = {2, 4, 8};
     int rc = RT_OK;
-    for (const rt_tuning &shape : kShapes) {
-        for (int bpc : bpcs) {
-            rt_tuning t = shape;
-            t.blocks_per_cu = bpc;
-            float ms = 0.0f;
-            rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);  // warm
-            if (rc) break;
-            rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);
-            if (rc) break;
-            if (ms < bms) {
-                bms = ms;
-                bt = t;
+    for (int form = 0; form < 2 && rc == RT_OK; ++form) {
+        for (const rt_tuning &shape : kShapes) {
+            for (int bpc : bpcs) {
+                rt_tuning t = shape;
+                t.blocks_per_cu = bpc;
+                t.traversal = form;
+                float ms = 0.0f;
+                rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);  // warm
+                if (rc) break;
+                rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);
+                if (rc) break;
+                if (ms < bms) {
+                    bms = ms;
+                    bt = t;
+                }
             }
+            if (rc) break;
         }
-        if (rc) break;
     }
     hipStreamSynchronize(ctx->stream);
     hipFree(scratch);
@@ -893,7 +898,7 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
     if (best) *best = bt;
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "a")) {
-            fprintf(f, "%s %d %d %d\n", key.c_str(), bt.block, bt.lds_stack, bt.blocks_per_cu);
+            fprintf(f, "%s %d %d %d %d\n", key.c_str(), bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal);
             fclose(f);
         }
     }
@@ -914,7 +919,7 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
         if (FILE *f = fopen(cache_path, "r")) {
             char k[256];
             rt_tuning t;
-            while (fscanf(f, "%255s %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu) == 4) {
+            while (fscanf(f, "%255s %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal) == 5) {
                 if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK) {
                     fclose(f);
                     ctx->tune = t;
@@ -929,20 +934,23 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     rt_tuning bt = saved;
     float bms = 1e30f;
     int rc = RT_OK;
-    for (const rt_tuning &shape : kShapes) {
-        rt_tuning t = shape;
-        t.blocks_per_cu = saved.blocks_per_cu;
-        ctx->tune = t;
-        float ms = 1e30f;
-        for (int rep = 0; rep < 3 && rc == RT_OK; ++rep) {  // first run warms caches
-            rt_frame_stats st;
-            rc = rt_render_direct(ctx, scene, desc, 0, &st);
-            if (rep > 0 && st.trace_ms < ms) ms = st.trace_ms;
-        }
-        if (rc) break;
-        if (ms < bms) {
-            bms = ms;
-            bt = t;
+    for (int form = 0; form < 2 && rc == RT_OK; ++form) {
+        for (const rt_tuning &shape : kShapes) {
+            rt_tuning t = shape;
+            t.blocks_per_cu = saved.blocks_per_cu;
+            t.traversal = form;
+            ctx->tune = t;
+            float ms = 1e30f;
+            for (int rep = 0; rep < 3 && rc == RT_OK; ++rep) {  // first run warms caches
+                rt_frame_stats st;
+                rc = rt_render_direct(ctx, scene, desc, 0, &st);
+                if (rep > 0 && st.trace_ms < ms) ms = st.trace_ms;
+            }
+            if (rc) break;
+            if (ms < bms) {
+                bms = ms;
+                bt = t;
+            }
         }
     }
     ctx->tune = rc ? saved : bt;
@@ -950,7 +958,7 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     if (best) *best = bt;
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "a")) {
-            fprintf(f, "%s %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu);
+            fprintf(f, "%s %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal);
             fclose(f);
         }
     }

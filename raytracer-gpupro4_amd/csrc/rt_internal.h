@@ -167,7 +167,7 @@ struct rt_ctx {
     hipEvent_t ev[8] = {};
     uint64_t bytes = 0;
     int cu_count = 256;
-    rt_tuning tune{128, 26, 8};
+    rt_tuning tune{128, 26, 8, 0};
     uint32_t *tile_counter = nullptr;     // persistent job launches
     void *jobs_dev = nullptr;             // render-job table
     uint64_t jobs_cap = 0;

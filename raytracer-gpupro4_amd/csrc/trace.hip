@@ -70,10 +70,119 @@ __device__ __forceinline__ void box_test(const float *bx, float ox, float oy, fl
     t1 = b;
 }
 
-// Traversal state of one lane.  trav_begin tests the root box; each
-// trav_step then handles one internal node (both child boxes) or one leaf and
-// pops the stack.  A ray is finished when either call returns true; its
-// result is in r.
+template <bool ANYHIT, class Stack>
+__device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
+                                             const TravTri *__restrict__ tris, int32_t root_ref,
+                                             const float *root_box, float ox, float oy, float oz,
+                                             float dx, float dy, float dz, float tmax, int32_t self,
+                                             Stack &stk)
+{
+    TraceOut r;
+    r.tri = -1;
+    r.u = -1.0f;
+    r.v = -1.0f;
+    r.t = tmax;
+    r.boxes = 0;
+    r.tris = 0;
+    r.leaf_heap = 1;
+    r.pos = -1;
+    if (root_ref == -1) return r;
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const uint32_t neg = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
+    float t0, t1;
+    box_test(root_box, ox, oy, oz, ix, iy, iz, t0, t1);
+    r.boxes = 1;
+    if (t0 > r.t || t1 < 0.0f) return r;
+    int32_t cur = root_ref;
+    stk.sp = 0;
+    while (true) {
+        if (cur >= 0) {
+            const TravNode *n = &nodes[cur];
+            const float4 q0 = *reinterpret_cast<const float4 *>(&n->box[0]);
+            const float4 q1 = *reinterpret_cast<const float4 *>(&n->box[4]);
+            const float4 q2 = *reinterpret_cast<const float4 *>(&n->box[8]);
+            const int4 q3 = *reinterpret_cast<const int4 *>(&n->ref[0]);
+            const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+            const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            float a0, a1, b0, b1;
+            box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
+            box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+            r.boxes += 2;
+            const int first_is_b = (neg >> q3.z) & 1u;  // the child on side dirIsNeg[axis]
+            const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
+            const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+            const int32_t fref = first_is_b ? q3.y : q3.x;
+            const int32_t sref = first_is_b ? q3.x : q3.y;
+            const bool f_ok = !(f0 > r.t || f1 < 0.0f);
+            const bool s_live = !(s1 < 0.0f);
+            if (f_ok) {
+                if (s_live) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+                cur = fref;
+                continue;
+            }
+            if (s_live && !(s0 > r.t)) {
+                cur = sref;
+                continue;
+            }
+        } else {
+            const uint32_t pos = (uint32_t)cur & ~kLeafBit;
+            const TravTri *tp = &tris[pos];
+            int32_t cnt = 1;
+            for (int32_t j = 0; j < cnt; ++j) {
+                const float4 a = tp[j].a;
+                const float4 e1 = tp[j].e1;
+                const float4 e2 = tp[j].e2;
+                if (j == 0) cnt = __float_as_int(e1.w);
+                r.tris += 1;
+                const int32_t id = __float_as_int(a.w);
+                // getIntersection (Intersection.hlsl:47-64)
+                const float px = (dy * e2.z) - (dz * e2.y);
+                const float py = (dz * e2.x) - (dx * e2.z);
+                const float pz = (dx * e2.y) - (dy * e2.x);
+                const float dd = (e1.x * px + e1.y * py) + e1.z * pz;
+                const float det = 1.0f / dd;
+                const float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
+                const float u = ((tx * px + ty * py) + tz * pz) * det;
+                const float qx = (ty * e1.z) - (tz * e1.y);
+                const float qy = (tz * e1.x) - (tx * e1.z);
+                const float qz = (tx * e1.y) - (ty * e1.x);
+                const float v = ((dx * qx + dy * qy) + dz * qz) * det;
+                const float t = ((e2.x * qx + e2.y * qy) + e2.z * qz) * det;
+                if (id != self && u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f && t >= 0.0f &&
+                    t < r.t) {
+                    r.tri = id;
+                    r.u = u;
+                    r.v = v;
+                    r.t = t;
+                    r.leaf_heap = __float_as_int(tp[0].e2.w);
+                    r.pos = (int32_t)pos + j;
+                    if (ANYHIT) return r;
+                }
+            }
+        }
+        // pop the next live entry
+        bool found = false;
+        while (stk.sp > 0) {
+            const uint2 e = stk.pop();
+            if (!(__uint_as_float(e.y) > r.t)) {
+                cur = (int32_t)e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return r;
+}
+
+// The same traversal written as a step function: trav_begin tests the root
+// box; each trav_step handles one internal node (both child boxes) or one
+// leaf and pops the stack; a ray is finished when either returns true.  It
+// visits the same nodes in the same order as traverse(), but the compiler
+// lays the loop out differently, and which layout is faster depends on the
+// scene (deep leaves of many triangles favour traverse(), incoherent any-hit
+// rays through thin geometry favour this one), so the launch shape's
+// traversal field picks one per launch.
 struct Trav {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
     int32_t self, cur;
@@ -199,7 +308,7 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
 }
 
 template <bool ANYHIT, class Stack>
-__device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
+__device__ __forceinline__ TraceOut traverse_steps(const TravNode *__restrict__ nodes,
                                              const TravTri *__restrict__ tris, int32_t root_ref,
                                              const float *root_box, float ox, float oy, float oz,
                                              float dx, float dy, float dz, float tmax, int32_t self,
@@ -210,6 +319,19 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
         while (!trav_step<ANYHIT ? 1 : 0>(T, nodes, tris, stk)) {
         }
     return T.r;
+}
+
+
+template <int FORM, bool ANYHIT, class Stack>
+__device__ __forceinline__ TraceOut traverse_form(const TravNode *__restrict__ nodes,
+                                                  const TravTri *__restrict__ tris, int32_t root_ref,
+                                                  const float *root_box, float ox, float oy, float oz,
+                                                  float dx, float dy, float dz, float tmax, int32_t self,
+                                                  Stack &stk)
+{
+    if (FORM == 1)
+        return traverse_steps<ANYHIT>(nodes, tris, root_ref, root_box, ox, oy, oz, dx, dy, dz, tmax, self, stk);
+    return traverse<ANYHIT>(nodes, tris, root_ref, root_box, ox, oy, oz, dx, dy, dz, tmax, self, stk);
 }
 
 struct SceneView {
@@ -243,7 +365,7 @@ __device__ __forceinline__ void store_hit(rt_hit24 *dst, const TraceOut &o)
 }
 
 // rt_trace: one lane per ray.
-template <bool ANYHIT, bool COUNT, int BLOCK, int KLDS>
+template <bool ANYHIT, bool COUNT, int BLOCK, int KLDS, int FORM>
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const rt_ray48 *__restrict__ rays,
                                                  uint32_t n, rt_hit24 *__restrict__ hits,
                                                  unsigned long long *stats)
@@ -273,7 +395,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const rt_ray48 *_
             d.leaf_heap = 1;
             store_hit(&hits[i], d);
         } else {
-            const TraceOut o = traverse<ANYHIT>(sv.nodes, sv.tris, root_ref, root_box, r0.x, r0.y,
+            const TraceOut o = traverse_form<FORM, ANYHIT>(sv.nodes, sv.tris, root_ref, root_box, r0.x, r0.y,
                                                 r0.z, r0.w, r1.x, r1.y, r2.z, self, stk);
             store_hit(&hits[i], o);
             boxes = o.boxes;
@@ -327,7 +449,7 @@ __device__ __forceinline__ void flat_normal(const TravTri &T, float &nx, float &
 // self = hit triangle} per light, cast when dot(n, Ldir) * dot(n, d) > 0,
 // traced any-hit from the root (the occlusion predicate of the reference's
 // closest-hit shadow ray).  Returns (ambient + sum max(0, -n'.Ldir)) * albedo.
-template <class Stack>
+template <int FORM, class Stack>
 __device__ __forceinline__ float direct_term(const SceneView &sv, int32_t root_ref,
                                              const float *root_box, const TraceOut &h, float ox,
                                              float oy, float oz, float dx, float dy, float dz,
@@ -357,7 +479,7 @@ __device__ __forceinline__ float direct_term(const SceneView &sv, int32_t root_r
         const float rs = (nx * dx + ny * dy) + nz * dz;
         uint8_t o = 2;
         if ((ls * rs) > 0.0f) {
-            const TraceOut s = traverse<true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz,
+            const TraceOut s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz,
                                               -lx, -ly, -lz, dist, h.tri, stk);
             boxes += s.boxes;
             ntris += s.tris;
@@ -400,7 +522,7 @@ struct Tile {
 };
 
 // Primary + shadow rays + direct term for one image, one tile per block.
-template <bool KEEP, bool COUNT, int BLOCK, int KLDS>
+template <bool KEEP, bool COUNT, int BLOCK, int KLDS, int FORM>
 __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderParams rp,
                                                          float4 *__restrict__ rgba,
                                                          rt_hit24 *__restrict__ hits,
@@ -427,8 +549,8 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
         float dx, dy, dz;
         primary_ray(rp.cam, px, py, (float)rp.width, (float)rp.height, 0.0f, 0.0f, dx, dy, dz);
         const float ox = rp.cam.pos[0], oy = rp.cam.pos[1], oz = rp.cam.pos[2];
-        const TraceOut h = traverse<false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz, dx,
-                                           dy, dz, 10000000000000000.0f, -1, stk);
+        const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz,
+                                                      dx, dy, dz, 10000000000000000.0f, -1, stk);
         boxes += h.boxes;
         ntris += h.tris;
         if (KEEP) store_hit(&hits[pix], h);
@@ -438,7 +560,7 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
             if (KEEP)
                 for (uint32_t l = 0; l < rp.nlights; ++l) occ[(size_t)pix * rp.nlights + l] = 2;
         } else {
-            color = direct_term(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz, rp.lights,
+            color = direct_term<FORM>(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz, rp.lights,
                                 rp.nlights, rp.ambient, rp.albedo, stk, boxes, ntris, nshadow,
                                 KEEP ? &occ[(size_t)pix * rp.nlights] : nullptr);
         }
@@ -533,7 +655,7 @@ struct RenderJob {
     float lights[24];
 };
 
-template <int BLOCK, int KLDS, bool PERSISTENT>
+template <int BLOCK, int KLDS, bool PERSISTENT, int FORM>
 __global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restrict__ jobs,
                                                        uint32_t njobs, uint32_t total_tiles,
                                                        uint32_t tile_base, uint32_t *tile_counter,
@@ -584,13 +706,14 @@ __global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restri
                 primary_ray(J.cam, px, py, fw, fh, jx, jy, dx, dy, dz);
                 float ox = J.cam.pos[0], oy = J.cam.pos[1], oz = J.cam.pos[2];
                 if (J.mode == 0) {
-                    const TraceOut h = traverse<false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy,
-                                                       oz, dx, dy, dz, 10000000000000000.0f, -1, stk);
+                    const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox,
+                                                                  oy, oz, dx, dy, dz, 10000000000000000.0f, -1,
+                                                                  stk);
                     nprimary += 1;
                     boxes += h.boxes;
                     ntris += h.tris;
                     sum += h.tri < 0 ? J.sky
-                                     : direct_term(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz,
+                                     : direct_term<FORM>(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz,
                                                    J.lights, J.nlights, J.ambient, J.albedo, stk,
                                                    boxes, ntris, nshadow, nullptr);
                     continue;
@@ -598,8 +721,9 @@ __global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restri
                 float thr = 1.0f, L = 0.0f;
                 int32_t self = -1;
                 for (uint32_t b = 0; b <= J.max_bounces; ++b) {
-                    const TraceOut h = traverse<false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy,
-                                                       oz, dx, dy, dz, 10000000000000000.0f, self, stk);
+                    const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox,
+                                                                  oy, oz, dx, dy, dz, 10000000000000000.0f, self,
+                                                                  stk);
                     if (b == 0) nprimary += 1;
                     else nbounce += 1;
                     boxes += h.boxes;
@@ -661,16 +785,22 @@ static SceneView view_of(const Batch &b, uint32_t scene)
     return SceneView{b.trav, b.tris, b.hdr + scene};
 }
 
-#define RT_TUNED(MACRO)                                  \
+#define RT_SHAPES(MACRO, F)                              \
     switch (tune_index(tune)) {                          \
-        case 0: MACRO(64, 8); break;                     \
-        case 1: MACRO(64, 16); break;                    \
-        case 2: MACRO(128, 8); break;                    \
-        case 3: MACRO(128, 12); break;                   \
-        case 4: MACRO(128, 16); break;                   \
-        case 5: MACRO(128, 26); break;                   \
-        case 6: MACRO(256, 8); break;                    \
-        default: MACRO(256, 12); break;                  \
+        case 0: MACRO(64, 8, F); break;                  \
+        case 1: MACRO(64, 16, F); break;                 \
+        case 2: MACRO(128, 8, F); break;                 \
+        case 3: MACRO(128, 12, F); break;                \
+        case 4: MACRO(128, 16, F); break;                \
+        case 5: MACRO(128, 26, F); break;                \
+        case 6: MACRO(256, 8, F); break;                 \
+        default: MACRO(256, 12, F); break;               \
+    }
+#define RT_TUNED(MACRO)                                  \
+    if (tune.traversal == 1) {                           \
+        RT_SHAPES(MACRO, 1)                              \
+    } else {                                             \
+        RT_SHAPES(MACRO, 0)                              \
     }
 
 int tune_index(const rt_tuning &t)
@@ -690,13 +820,13 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
     if (n == 0) return hipSuccess;
     const SceneView sv = view_of(b, scene);
     const bool any = flags & RT_FLAG_ANYHIT, cnt = flags & RT_FLAG_COUNT;
-#define RT_L(B, K)                                                                                 \
+#define RT_L(B, K, F)                                                                               \
     {                                                                                              \
         const dim3 grid((n + B - 1) / B), block(B);                                                \
-        if (any && cnt) hipLaunchKernelGGL((k_trace<true, true, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
-        else if (any) hipLaunchKernelGGL((k_trace<true, false, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
-        else if (cnt) hipLaunchKernelGGL((k_trace<false, true, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
-        else hipLaunchKernelGGL((k_trace<false, false, B, K>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        if (any && cnt) hipLaunchKernelGGL((k_trace<true, true, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        else if (any) hipLaunchKernelGGL((k_trace<true, false, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        else if (cnt) hipLaunchKernelGGL((k_trace<false, true, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        else hipLaunchKernelGGL((k_trace<false, false, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
     }
     RT_TUNED(RT_L)
 #undef RT_L
@@ -718,15 +848,15 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     for (uint32_t i = 0; i < 24; ++i) rp.lights[i] = (i < 3 * d.nlights) ? d.lights[i] : 0.0f;
     const SceneView sv = view_of(b, scene);
     const bool keep = flags & RT_FLAG_KEEP_HITS, cnt = flags & RT_FLAG_COUNT;
-#define RT_L(B, K)                                                                                 \
+#define RT_L(B, K, F)                                                                               \
     {                                                                                              \
         constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                            \
         const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);             \
         const dim3 grid(tiles), block(B);                                                          \
-        if (keep && cnt) hipLaunchKernelGGL((k_render_direct<true, true, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
-        else if (keep) hipLaunchKernelGGL((k_render_direct<true, false, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
-        else if (cnt) hipLaunchKernelGGL((k_render_direct<false, true, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
-        else hipLaunchKernelGGL((k_render_direct<false, false, B, K>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        if (keep && cnt) hipLaunchKernelGGL((k_render_direct<true, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        else if (keep) hipLaunchKernelGGL((k_render_direct<true, false, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        else if (cnt) hipLaunchKernelGGL((k_render_direct<false, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        else hipLaunchKernelGGL((k_render_direct<false, false, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
     }
     RT_TUNED(RT_L)
 #undef RT_L
@@ -775,16 +905,16 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
 {
     if (total_tiles == 0 || njobs == 0 || (!persistent && tile_count == 0)) return hipSuccess;
     const RenderJob *jobs = static_cast<const RenderJob *>(d_jobs);
-#define RT_L(B, K)                                                                                  \
+#define RT_L(B, K, F)                                                                                \
     {                                                                                               \
         if (persistent) {                                                                           \
             hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, tile_counter);            \
             uint32_t blocks = (uint32_t)cu_count * (uint32_t)(tune.blocks_per_cu > 0 ? tune.blocks_per_cu : 4); \
             if (blocks > total_tiles) blocks = total_tiles;                                         \
-            hipLaunchKernelGGL((k_render_jobs<B, K, true>), dim3(blocks), dim3(B), 0, st, jobs, njobs, \
+            hipLaunchKernelGGL((k_render_jobs<B, K, true, F>), dim3(blocks), dim3(B), 0, st, jobs, njobs, \
                                total_tiles, 0u, tile_counter, frame_counter, stats);                \
         } else {                                                                                    \
-            hipLaunchKernelGGL((k_render_jobs<B, K, false>), dim3(tile_count), dim3(B), 0, st, jobs, \
+            hipLaunchKernelGGL((k_render_jobs<B, K, false, F>), dim3(tile_count), dim3(B), 0, st, jobs, \
                                njobs, first_tile + tile_count, first_tile, tile_counter,            \
                                frame_counter, stats);                                               \
         }                                                                                           \

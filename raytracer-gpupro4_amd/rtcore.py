@@ -80,10 +80,11 @@ class RenderDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("blocks_per_cu", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("traversal", C.c_int32)]
 
     def as_tuple(self):
-        return (self.block, self.lds_stack, self.blocks_per_cu)
+        return (self.block, self.lds_stack, self.blocks_per_cu, self.traversal)
 
 
 class JobDesc(C.Structure):
@@ -336,8 +337,8 @@ class Context:
         self.check(self.L.rt_tuning_get(self.h, C.byref(t)), "rt_tuning_get")
         return t.as_tuple()
 
-    def set_tuning(self, block, lds_stack, blocks_per_cu=4):
-        t = Tuning(block, lds_stack, blocks_per_cu)
+    def set_tuning(self, block, lds_stack, blocks_per_cu=4, traversal=0):
+        t = Tuning(block, lds_stack, blocks_per_cu, traversal)
         self.check(self.L.rt_tuning_set(self.h, C.byref(t)), "rt_tuning_set")
 
     def alloc(self, nbytes: int) -> int:

@@ -99,8 +99,8 @@ def test_jobs_mix_and_launch_mode_invariance(ctx, rtcore):
     alone = [run([j], False)[k] for k, j in enumerate(jobs)]
     saved = ctx.tuning()
     try:
-        for shape in [(64, 8), (128, 8), (128, 12), (128, 26), (256, 12)]:
-            ctx.set_tuning(shape[0], shape[1], 2)
+        for shape in [(64, 8, 0), (128, 8, 1), (128, 12, 0), (128, 26, 1), (128, 26, 0), (256, 12, 1)]:
+            ctx.set_tuning(shape[0], shape[1], 2, shape[2])
             for persistent in (False, True):
                 together = run(jobs, persistent)
                 for k in range(3):
@@ -123,12 +123,13 @@ def test_trace_parity_under_every_launch_shape(ctx, oracle, rtcore):
     r = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays, nthreads=8)
     saved = ctx.tuning()
     try:
-        for block, lds in [(64, 8), (64, 16), (128, 8), (128, 12), (128, 16), (128, 26), (256, 8), (256, 12)]:
-            ctx.set_tuning(block, lds, 4)
-            assert np.array_equal(ctx.trace(s, rays), r), (block, lds)
-            st = ctx.render_direct(s, cam, 160, 160, [(0.5, 2.5, -1.0)], flags=rtcore.FLAG_KEEP_HITS)
-            _, hits, _ = ctx.frame_read(160, 160, 1, hits=True)
-            assert np.array_equal(hits, r), (block, lds)
+        for form in (0, 1):
+            for block, lds in [(64, 8), (64, 16), (128, 8), (128, 12), (128, 16), (128, 26), (256, 8), (256, 12)]:
+                ctx.set_tuning(block, lds, 4, form)
+                assert np.array_equal(ctx.trace(s, rays), r), (block, lds, form)
+                st = ctx.render_direct(s, cam, 160, 160, [(0.5, 2.5, -1.0)], flags=rtcore.FLAG_KEEP_HITS)
+                _, hits, _ = ctx.frame_read(160, 160, 1, hits=True)
+                assert np.array_equal(hits, r), (block, lds, form)
     finally:
         ctx.set_tuning(*saved)
     ctx.scene_destroy(s)
@@ -183,9 +184,9 @@ def test_autotuner_picks_and_caches(ctx, rtcore, tmp_path):
     jobs = [ctx.job(s, cam, 128, 128, acc, rtcore.MODE_GI, 1, 2, 0.7, 1.0)]
     cache = str(tmp_path / "tune.txt")
     best = ctx.autotune(jobs, cache)
-    assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] in (2, 4, 8)
+    assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] in (2, 4, 8) and best[3] in (0, 1)
     assert ctx.tuning() == best
-    assert open(cache).read().strip().endswith(f"{best[0]} {best[1]} {best[2]}")
+    assert open(cache).read().strip().endswith(" ".join(str(v) for v in best))
     ctx.set_tuning(128, 26, 4)
     assert ctx.autotune(jobs, cache) == best  # cache hit
     # the search rendered into scratch buffers, not into the caller's
@@ -223,9 +224,9 @@ def test_autotune_direct_picks_and_caches(ctx, rtcore, mix, tmp_path):
     saved = ctx.tuning()
     try:
         best = ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache)
-        assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] == saved[2]
+        assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] == saved[2] and best[3] in (0, 1)
         assert ctx.tuning() == best
-        assert open(cache).read().split()[-3:] == [str(v) for v in best]
+        assert open(cache).read().split()[-4:] == [str(v) for v in best]
         ctx.set_tuning(64, 8, saved[2])
         assert ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache) == best  # cache hit
     finally:

@@ -149,3 +149,26 @@ def test_facade_demo_binary(rtcore):
     assert rec["samples"] == 3 and rec["keyboard_id"] == 9 and rec["name"] == "LBVH"
     assert rec["probe_tri"] >= 0 and 2.0 < rec["probe_t"] < 3.0
     assert rec["rays"] >= 3 * 512 * 512
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_render_direct_forms_equal(ctx, rtcore, soup262k, form):
+    """Both traversal forms give the same frame, hits, occlusion and counters."""
+    s = soup262k[0]
+    W, H = 333, 177
+    cam = rtcore.camera_look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0, 1, 0), 60.0, W / H)
+    lights = [(0.5, 2.0, -1.0), (1.5, 0.5, 0.5), (-0.5, 0.5, 0.2)]
+    saved = ctx.tuning()
+    out = []
+    try:
+        for f in (0, form):
+            ctx.set_tuning(128, 12, 4, f)
+            st = ctx.render_direct(s, cam, W, H, lights, flags=rtcore.FLAG_KEEP_HITS | rtcore.FLAG_COUNT)
+            out.append((st, ctx.frame_read(W, H, len(lights), hits=True)))
+    finally:
+        ctx.set_tuning(*saved)
+    (st0, a), (st1, b) = out
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    for k in ("primary_rays", "shadow_rays", "box_tests", "tri_tests"):
+        assert st0[k] == st1[k], k
