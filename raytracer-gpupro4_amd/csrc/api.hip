@@ -685,8 +685,8 @@ static int check_tuning(rt_ctx *ctx, const rt_tuning &t)
 }
 
 // The instantiated launch shapes, in the order of rt::tune_index.
-static const rt_tuning kShapes[] = {{64, 8, 4, 0},   {64, 16, 4, 0},  {128, 8, 4, 0},  {128, 12, 4, 0},
-                                    {128, 16, 4, 0}, {128, 26, 4, 0}, {256, 8, 4, 0},  {256, 12, 4, 0}};
+static const rt_tuning kShapes[] = {{64, 16, 4, 0},  {64, 20, 4, 0},  {128, 12, 4, 0}, {128, 16, 4, 0},
+                                    {128, 20, 4, 0}, {128, 22, 4, 0}, {128, 26, 4, 0}, {256, 12, 4, 0}};
 
 int rt_tuning_get(rt_ctx *ctx, rt_tuning *t)
 {
@@ -710,11 +710,12 @@ int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t)
 
 // Builds the device job table for jobs; returns the total tile count.
 static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const rt_tuning &t,
-                      void *dst_dev, std::vector<uint32_t> *tile_offs, uint32_t *total)
+                      void *dst_dev, std::vector<uint32_t> *tile_offs, uint32_t *total, uint32_t *max_depth)
 {
     const size_t jb = rt::render_job_bytes();
     std::vector<unsigned char> host(jb * njobs);
     uint32_t off = 0;
+    *max_depth = 0;
     if (tile_offs) tile_offs->clear();
     for (uint32_t k = 0; k < njobs; ++k) {
         const rt_job_desc &d = jobs[k];
@@ -730,6 +731,7 @@ static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, cons
             return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights or mode", k);
         uint32_t tiles = 0;
         rt::fill_render_job(host.data() + jb * k, *b, i, d, t, off, &tiles);
+        if (b->desc[i].depth > *max_depth) *max_depth = b->desc[i].depth;
         if (tile_offs) tile_offs->push_back(off);
         off += tiles;
     }
@@ -762,8 +764,8 @@ static int run_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32
     int rc = ensure_jobs(ctx, njobs);
     if (rc) return rc;
     std::vector<uint32_t> offs;
-    uint32_t total = 0;
-    rc = stage_jobs(ctx, jobs, njobs, t, ctx->jobs_dev, &offs, &total);
+    uint32_t total = 0, depth = 0;
+    rc = stage_jobs(ctx, jobs, njobs, t, ctx->jobs_dev, &offs, &total, &depth);
     if (rc) return rc;
     hipStream_t st = ctx->stream;
     const bool persistent = (flags & RT_FLAG_PERSISTENT) != 0;
@@ -774,13 +776,13 @@ static int run_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32
     if (!timing && ctx->profiling) ev = prof_events(ctx, 1, 2);
     if (ev) hipEventRecord(ev[0], st);
     if (persistent) {
-        RT_HIP(ctx, rt::launch_render_jobs(ctx->jobs_dev, njobs, total, 0, total, true,
-                                           ctx->tile_counter, nullptr, ctx->stats, t, ctx->cu_count, st));
+        RT_HIP(ctx, rt::launch_render_jobs(ctx->jobs_dev, njobs, total, 0, total, true, ctx->tile_counter,
+                                           nullptr, ctx->stats, t, depth, ctx->cu_count, st));
     } else {
         for (uint32_t k = 0; k < njobs; ++k)
             RT_HIP(ctx, rt::launch_render_jobs(ctx->jobs_dev, njobs, total, offs[k],
                                                offs[k + 1] - offs[k], false, ctx->tile_counter,
-                                               nullptr, ctx->stats, t, ctx->cu_count, st));
+                                               nullptr, ctx->stats, t, depth, ctx->cu_count, st));
     }
     if (ev) hipEventRecord(ev[1], st);
     RT_HIP(ctx, hipEventRecord(ctx->jobs_ev, st));
@@ -988,8 +990,8 @@ int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_des
     RT_HIP(ctx, hipMalloc(&g.jobs, jb * (njobs ? njobs : 1)));
     RT_HIP(ctx, hipMalloc(&g.counter, 256));
     RT_HIP(ctx, hipMemset(g.counter, 0, 256));
-    uint32_t total = 0;
-    int rc = stage_jobs(ctx, jobs, njobs, ctx->tune, g.jobs, nullptr, &total);
+    uint32_t total = 0, depth = 0;
+    int rc = stage_jobs(ctx, jobs, njobs, ctx->tune, g.jobs, nullptr, &total, &depth);
     if (rc) {
         hipFree(g.jobs);
         hipFree(g.counter);
@@ -1003,7 +1005,7 @@ int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_des
     // g.counter[0] = replay index (read by the kernel), g.counter[16] = tile queue
     if (e == hipSuccess)
         e = rt::launch_render_jobs(g.jobs, njobs, total, 0, total, true, g.counter + 16, g.counter,
-                                   ctx->stats, ctx->tune, ctx->cu_count, st);
+                                   ctx->stats, ctx->tune, depth, ctx->cu_count, st);
     if (e == hipSuccess) e = rt::launch_bump_counter(g.counter, st);
     hipError_t e2 = hipStreamEndCapture(st, &g.graph);
     if (e == hipSuccess) e = e2;

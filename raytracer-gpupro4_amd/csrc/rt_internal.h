@@ -212,7 +212,8 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
 hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,
                               uint32_t first_tile, uint32_t tile_count, bool persistent,
                               uint32_t *tile_counter, const uint32_t *frame_counter,
-                              unsigned long long *stats, const rt_tuning &tune, int cu_count,
+                              unsigned long long *stats, const rt_tuning &tune, uint32_t max_depth,
+                              int cu_count,
                               hipStream_t st);
 hipError_t launch_bump_counter(uint32_t *c, hipStream_t st);
 int tune_index(const rt_tuning &t);

@@ -33,21 +33,29 @@ struct TraceOut {
     int32_t pos;  // sorted slot of the hit triangle, -1 when none
 };
 
+// Stacks of at most 16 LDS entries spill deeper entries to private memory.
+// Larger ones (20, 22, 26) hold the whole stack in LDS with no spill path: a
+// traversal pushes at most one entry per level below the root, so a heap of
+// depth d needs d - 1 entries, and the host only launches a shape whose stack
+// covers the deepest heap of the launch (26 covers every allowed depth).
+constexpr bool stack_spills(int klds) { return klds <= 16; }
+
 template <int BLOCK, int KLDS>
 struct RayStack {
+    static constexpr bool kSpill = stack_spills(KLDS) && KLDS < kStackMax;
     uint2 *lds;  // this lane's LDS column: entry j at lds[j * BLOCK]
-    uint2 spill[(kStackMax - KLDS) > 0 ? (kStackMax - KLDS) : 1];
+    uint2 spill[kSpill ? (kStackMax - KLDS) : 1];
     int sp;
     __device__ __forceinline__ void push(uint2 e)
     {
-        if (KLDS >= kStackMax || sp < KLDS) lds[sp * BLOCK] = e;
+        if (!kSpill || sp < KLDS) lds[sp * BLOCK] = e;
         else spill[sp - KLDS] = e;
         ++sp;
     }
     __device__ __forceinline__ uint2 pop()
     {
         --sp;
-        return (KLDS >= kStackMax || sp < KLDS) ? lds[sp * BLOCK] : spill[sp - KLDS];
+        return (!kSpill || sp < KLDS) ? lds[sp * BLOCK] : spill[sp - KLDS];
     }
 };
 
@@ -786,14 +794,14 @@ static SceneView view_of(const Batch &b, uint32_t scene)
 }
 
 #define RT_SHAPES(MACRO, F)                              \
-    switch (tune_index(tune)) {                          \
-        case 0: MACRO(64, 8, F); break;                  \
-        case 1: MACRO(64, 16, F); break;                 \
-        case 2: MACRO(128, 8, F); break;                 \
-        case 3: MACRO(128, 12, F); break;                \
-        case 4: MACRO(128, 16, F); break;                \
-        case 5: MACRO(128, 26, F); break;                \
-        case 6: MACRO(256, 8, F); break;                 \
+    switch (launch_shape(tune, max_depth)) {             \
+        case 0: MACRO(64, 16, F); break;                 \
+        case 1: MACRO(64, 20, F); break;                 \
+        case 2: MACRO(128, 12, F); break;                \
+        case 3: MACRO(128, 16, F); break;                \
+        case 4: MACRO(128, 20, F); break;                \
+        case 5: MACRO(128, 22, F); break;                \
+        case 6: MACRO(128, 26, F); break;                \
         default: MACRO(256, 12, F); break;               \
     }
 #define RT_TUNED(MACRO)                                  \
@@ -803,14 +811,32 @@ static SceneView view_of(const Batch &b, uint32_t scene)
         RT_SHAPES(MACRO, 0)                              \
     }
 
+// Instantiated (block, LDS stack entries): a stack of depth - 1 entries never
+// spills, so 20 / 22 / 26 cover heaps of depth 21 / 23 / 27 entirely in LDS,
+// while smaller stacks trade spills to private memory for occupancy.
 int tune_index(const rt_tuning &t)
 {
-    if (t.block == 64) return t.lds_stack <= 8 ? 0 : 1;
-    if (t.block == 256) return t.lds_stack <= 8 ? 6 : 7;
-    if (t.lds_stack <= 8) return 2;
-    if (t.lds_stack <= 12) return 3;
-    if (t.lds_stack <= 16) return 4;
-    return 5;
+    if (t.block == 64) return t.lds_stack <= 16 ? 0 : 1;
+    if (t.block == 256) return 7;
+    if (t.lds_stack <= 12) return 2;
+    if (t.lds_stack <= 16) return 3;
+    if (t.lds_stack <= 20) return 4;
+    if (t.lds_stack <= 22) return 5;
+    return 6;
+}
+
+// The shape actually launched for heaps up to max_depth: a no-spill stack
+// too small for the deepest heap is replaced by the smallest one that fits
+// (block 128), or by the spilling stack of the same block size (block 64).
+static int launch_shape(const rt_tuning &t, uint32_t max_depth)
+{
+    static const int kEntries[8] = {16, 20, 12, 16, 20, 22, 26, 12};
+    const int need = max_depth > 1 ? (int)max_depth - 1 : 0;
+    int i = tune_index(t);
+    if (stack_spills(kEntries[i]) || kEntries[i] >= need) return i;
+    if (i == 1) return 0;
+    if (need <= 22) return 5;
+    return 6;
 }
 
 hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, uint32_t n,
@@ -819,6 +845,7 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
 {
     if (n == 0) return hipSuccess;
     const SceneView sv = view_of(b, scene);
+    const uint32_t max_depth = b.desc[scene].depth;
     const bool any = flags & RT_FLAG_ANYHIT, cnt = flags & RT_FLAG_COUNT;
 #define RT_L(B, K, F)                                                                               \
     {                                                                                              \
@@ -837,6 +864,7 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st)
 {
+    const uint32_t max_depth = b.desc[scene].depth;
     RenderParams rp;
     rp.cam = d.cam;
     rp.width = d.width;
@@ -900,8 +928,8 @@ size_t render_job_bytes() { return sizeof(RenderJob); }
 hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,
                               uint32_t first_tile, uint32_t tile_count, bool persistent,
                               uint32_t *tile_counter, const uint32_t *frame_counter,
-                              unsigned long long *stats, const rt_tuning &tune, int cu_count,
-                              hipStream_t st)
+                              unsigned long long *stats, const rt_tuning &tune, uint32_t max_depth,
+                              int cu_count, hipStream_t st)
 {
     if (total_tiles == 0 || njobs == 0 || (!persistent && tile_count == 0)) return hipSuccess;
     const RenderJob *jobs = static_cast<const RenderJob *>(d_jobs);

@@ -99,7 +99,7 @@ def test_jobs_mix_and_launch_mode_invariance(ctx, rtcore):
     alone = [run([j], False)[k] for k, j in enumerate(jobs)]
     saved = ctx.tuning()
     try:
-        for shape in [(64, 8, 0), (128, 8, 1), (128, 12, 0), (128, 26, 1), (128, 26, 0), (256, 12, 1)]:
+        for shape in [(64, 16, 0), (64, 20, 1), (128, 12, 0), (128, 20, 1), (128, 26, 0), (256, 12, 1)]:
             ctx.set_tuning(shape[0], shape[1], 2, shape[2])
             for persistent in (False, True):
                 together = run(jobs, persistent)
@@ -124,7 +124,7 @@ def test_trace_parity_under_every_launch_shape(ctx, oracle, rtcore):
     saved = ctx.tuning()
     try:
         for form in (0, 1):
-            for block, lds in [(64, 8), (64, 16), (128, 8), (128, 12), (128, 16), (128, 26), (256, 8), (256, 12)]:
+            for block, lds in [(64, 16), (64, 20), (128, 12), (128, 16), (128, 20), (128, 22), (128, 26), (256, 12)]:
                 ctx.set_tuning(block, lds, 4, form)
                 assert np.array_equal(ctx.trace(s, rays), r), (block, lds, form)
                 st = ctx.render_direct(s, cam, 160, 160, [(0.5, 2.5, -1.0)], flags=rtcore.FLAG_KEEP_HITS)

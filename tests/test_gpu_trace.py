@@ -172,3 +172,57 @@ def test_render_direct_forms_equal(ctx, rtcore, soup262k, form):
         assert np.array_equal(x, y)
     for k in ("primary_rays", "shadow_rays", "box_tests", "tri_tests"):
         assert st0[k] == st1[k], k
+
+
+def chain_scene():
+    """Triangles whose Morton codes are 2^29, 2^28, ..., 1, 0: each split
+    peels one off, so the tree is a chain as deep as the heap allows."""
+    tris = []
+    for b in range(29, -1, -1):
+        axis, j = 2 - (b % 3), b // 3
+        c = np.zeros(3)
+        c[axis] = (2 ** j + 0.5) / 1024.0
+        tris.append(c)
+    tris.append(np.full(3, 0.2 / 1024.0))
+    tris.append(np.full(3, 1.0 - 0.5 / 1024.0))  # far corner: fixes the scene box
+    tris = np.array(tris)
+    e = 0.2 / 1024.0
+    xyz = np.concatenate([tris + [0, 0, 0], tris + [e, 0, 0], tris + [0, e, 0]], axis=1).reshape(-1, 3)
+    xyz = np.vstack([xyz, [[0, 0, 0], [1e-4, 0, 0], [0, 1e-4, 0]]]).astype(np.float32)  # near corner
+    idx = np.arange(len(xyz), dtype=np.uint32).reshape(-1, 3)
+    return xyz, idx
+
+
+def test_deep_heap_under_every_shape(ctx, oracle, rtcore):
+    """A chain 25 levels deep (heap depth 26): launch shapes whose LDS stack
+    is shorter than depth - 1 either spill (16 or fewer entries) or are
+    replaced by one that fits (20 and 22 by 26); every shape and form gives
+    the oracle's hits on rays that walk down the chain."""
+    xyz, idx = chain_scene()
+    s = ctx.add_scene(xyz, idx, 26)
+    ctx.build(s)
+    ref = oracle.build_scene(xyz, idx, 26)
+    assert int(np.floor(np.log2(oracle.reachable(ref["nodes"]))).max()) == 25
+    rng = np.random.default_rng(5)
+    n = 4096
+    rays = np.zeros(n, oracle.RAY_DTYPE)
+    o = np.array([-0.01, -0.01, -0.01], np.float32)
+    tgt = rng.uniform(0.0, 0.02, (n, 3)).astype(np.float32)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["o"] = o
+    rays["d"] = d
+    rays["tmax"] = 1e16
+    rays["tri"] = -1
+    r = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays)
+    assert (r["visited"] & 0xFFFF).max() >= 40  # the rays walk far down the chain
+    saved = ctx.tuning()
+    try:
+        for form in (0, 1):
+            for block, lds in [(64, 16), (64, 20), (128, 12), (128, 16), (128, 20), (128, 22), (128, 26),
+                               (256, 12)]:
+                ctx.set_tuning(block, lds, 4, form)
+                assert np.array_equal(ctx.trace(s, rays), r), (block, lds, form)
+    finally:
+        ctx.set_tuning(*saved)
+    ctx.scene_destroy(s)
