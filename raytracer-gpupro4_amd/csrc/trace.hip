@@ -7,7 +7,9 @@
 // leaf triangles tested first to last, a hit taken only with t < best t.
 // Here a node holds both child boxes, so both are tested when the node is
 // entered; the second child is kept on a per-lane stack with its T0 and
-// culled on pop against the then-current best t.  That is the same decision,
+// culled on pop against the then-current best t (an entry whose T0 already
+// exceeds the best t is not pushed at all: best t only shrinks, so the pop
+// would discard it without counting anything).  That is the same decision,
 // at the same point of the sequence, as the trail algorithm makes when it
 // reaches the sibling, so the hits, the tie-breaking and the visit counters
 // (+1 per box, +65536 per triangle) equal the reference's.
@@ -124,7 +126,7 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
             const bool f_ok = !(f0 > r.t || f1 < 0.0f);
             const bool s_live = !(s1 < 0.0f);
             if (f_ok) {
-                if (s_live) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+                if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
                 cur = fref;
                 continue;
             }
@@ -260,7 +262,7 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
         const bool f_ok = !(f0 > r.t || f1 < 0.0f);
         const bool s_live = !(s1 < 0.0f);
         if (f_ok) {
-            if (s_live) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+            if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
             T.cur = fref;
             return false;
         }
