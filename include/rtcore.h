@@ -357,6 +357,20 @@ int rt_gen_noisy_sphere(uint32_t nlat, uint32_t nlon, float amplitude,
 int rt_gen_object_mix(uint32_t target_tris, uint64_t seed, float *xyz,
                       uint32_t *nv, uint32_t *idx, uint32_t *nt);
 
+/* ---- scene files (Model::LoadFile + Model::Scale, Common/Scenes/Model.cpp:210-381,647-709;
+ *      ModelOBJ, Common/Scenes/Loaders/model_obj.cpp) */
+/* Loads a ".obj" (faces fan-triangulated in file order, one vertex per
+ * distinct position index + texture coordinate + normal, in order of first
+ * use) or the reference's ".txt" model, then applies the reference's
+ * normalisation (OBJ: centre and divide by the largest extent; both: fit into
+ * a sphere of radius 0.49 about the origin) unless flags has RT_LOAD_RAW.
+ * Call with xyz = idx = NULL to get the sizes in *nv / *nt; then with arrays
+ * of at least those sizes.  Errors: rt_load_last_error() (this thread). */
+enum { RT_LOAD_RAW = 1u << 0 };
+int rt_load_model(const char *path, uint32_t flags, float *xyz, uint32_t *nv,
+                  uint32_t *idx, uint32_t *nt);
+const char *rt_load_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,6 +197,8 @@ def lib() -> C.CDLL:
         "rt_server_poll": (I32, [P, P, U32, C.POINTER(U32)]),
         "rt_server_status": (I32, [P, C.POINTER(ServerStats)]),
         "rt_server_read_stream": (I32, [P, U32, U32, U32, U32, U32, P, C.POINTER(U32)]),
+        "rt_load_model": (I32, [C.c_char_p, U32, P, C.POINTER(U32), P, C.POINTER(U32)]),
+        "rt_load_last_error": (C.c_char_p, []),
         "rt_camera_look_at": (None, [C.POINTER(Camera), P, P, P, C.c_float, C.c_float]),
         "rt_gen_torus": (I32, [C.c_float, C.c_float, U32, U32, C.c_float, U64, P, C.POINTER(U32), P,
                                C.POINTER(U32)]),
@@ -234,6 +236,26 @@ def _gen(fn, *args):
     rc = getattr(L, fn)(*args, _ptr(xyz), C.byref(nv), _ptr(idx), C.byref(nt))
     if rc != RT_OK:
         raise RtError(rc, fn)
+    return xyz, idx
+
+
+LOAD_RAW = 1 << 0
+
+
+def load_model(path, raw=False):
+    """Vertices and triangles of a .obj or the reference's .txt model file,
+    normalised as the reference normalises its models unless raw."""
+    L = lib()
+    nv, nt = C.c_uint32(0), C.c_uint32(0)
+    flags = LOAD_RAW if raw else 0
+    rc = L.rt_load_model(os.fsencode(path), flags, None, C.byref(nv), None, C.byref(nt))
+    if rc != RT_OK:
+        raise RtError(rc, L.rt_load_last_error().decode())
+    xyz = np.empty((nv.value, 3), np.float32)
+    idx = np.empty((nt.value, 3), np.uint32)
+    rc = L.rt_load_model(os.fsencode(path), flags, _ptr(xyz), C.byref(nv), _ptr(idx), C.byref(nt))
+    if rc != RT_OK:
+        raise RtError(rc, L.rt_load_last_error().decode())
     return xyz, idx
 
 

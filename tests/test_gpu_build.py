@@ -219,3 +219,17 @@ def test_batch_of_scenes_matches_per_scene_oracle(ctx, oracle, rtcore):
     assert np.array_equal(ids, ref["ids"])
     for h in handles:
         ctx.scene_destroy(h)
+
+
+def test_build_loaded_obj_model(ctx, oracle, rtcore, tmp_path):
+    """A model read through rt_load_model (the reference's OBJ path and
+    normalisation) builds bit for bit like the oracle on the same arrays."""
+    xyz, idx = rtcore.gen_noisy_sphere(129, 128, 0.05, 7)
+    lines = [f"v {x:.9g} {y:.9g} {z:.9g}" for x, y, z in xyz * 3.0 + [5.0, -2.0, 1.0]]
+    lines += [f"f {a + 1} {b + 1} {c + 1}" for a, b, c in idx]
+    path = tmp_path / "sphere.obj"
+    path.write_text("\n".join(lines) + "\n")
+    mx, mi = rtcore.load_model(str(path))
+    assert len(mi) == len(idx)
+    assert abs(np.sqrt((mx.astype(np.float64) ** 2).sum(1)).max() - 0.49) < 1e-6
+    check_build(ctx, oracle, mx, mi, 18, check_partition_mode=True)
