@@ -122,12 +122,21 @@ def cpu_baseline(rtcore, xyz, idx, cam, budget_s=12.0):
         t0 = time.perf_counter()
         ref = oracle.build_scene(rot, idx, DEPTH, presorted=False)
         build_s += time.perf_counter() - t0
+        if frames == 0:
+            ref0, rot0 = ref, rot
         t0 = time.perf_counter()
         hits = oracle.trace(ref["nodes"], ref["ids"], rot, idx, rays0, nthreads=cores)
         _, occ = oracle.render_direct(ref["nodes"], ref["ids"], rot, idx, rays0, hits, LIGHTS, nthreads=cores)
         trace_s += time.perf_counter() - t0
         nrays += len(rays0) + int((occ != 2).sum())
         frames += 1
+    # the same traversal on one thread, over every 16th pixel of frame 0
+    sub = rays0[::16].copy()
+    t0 = time.perf_counter()
+    hits1 = oracle.trace(ref0["nodes"], ref0["ids"], rot0, idx, sub, nthreads=1)
+    _, occ1 = oracle.render_direct(ref0["nodes"], ref0["ids"], rot0, idx, sub, hits1, LIGHTS, nthreads=1)
+    one_s = time.perf_counter() - t0
+    one_rays = len(sub) + int((occ1 != 2).sum())
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -140,6 +149,8 @@ def cpu_baseline(rtcore, xyz, idx, cam, budget_s=12.0):
             "sample": f"{frames} full frames of the same workload ({W}x{H} primary + shadow rays, "
                       f"closest hit, shadow rays from the hit leaf): {nrays} rays in {trace_s:.1f} s "
                       f"on {cores} threads; oracle rebuild of {NTRI} tris per frame on 1 thread",
+            "value_1thread": round(one_rays / one_s / 1e6, 3),
+            "sample_1thread": f"every 16th pixel of frame 0: {one_rays} rays in {one_s:.1f} s",
             "build_ms_1thread": round(build_s * 1e3 / frames, 2), "cpu_model": model}
 
 
@@ -267,7 +278,7 @@ def workload_sphere(args, rtcore):
 
 # ------------------------------------------------------------------ batched
 def gi_rmse_vs_fp64(ctx, rtcore, xyz, idx, depth, cam_fn, seed=7):
-    """One scene at 48x48: 1024 device samples against 2048 independent FP64
+    """One scene at 48x48: 1024 device samples against 4096 independent FP64
     oracle samples per pixel; RMSE of the per-pixel mean radiance."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -284,9 +295,9 @@ def gi_rmse_vs_fp64(ctx, rtcore, xyz, idx, depth, cam_fn, seed=7):
     ctx.free(acc)
     ctx.scene_destroy(s)
     ref = oracle.build_scene(xyz, idx, depth)
-    r, _ = oracle.render_gi(ref["nodes"], ref["ids"], xyz, idx, cam.as_array(), w, h, 2048, 4, 0.7, 1.0,
+    r, _ = oracle.render_gi(ref["nodes"], ref["ids"], xyz, idx, cam.as_array(), w, h, 4096, 4, 0.7, 1.0,
                             seed, first_sample=1 << 20, nthreads=min(CPU_SHARE, len(os.sched_getaffinity(0))))
-    return float(np.sqrt(np.mean((g - r / 2048) ** 2)))
+    return float(np.sqrt(np.mean((g - r / 4096) ** 2)))
 
 
 def workload_batched(args, rtcore):
@@ -386,7 +397,7 @@ def workload_batched(args, rtcore):
                   identical_images=bool(same), tuning=list(tuned), autotune_s=round(tune_s, 2),
                   rays_per_step=rays_b / args.steps,
                   gi_rmse_vs_fp64={"value": rmse, "scene_tris": int(sizes[k_mid]), "res": "48x48",
-                                   "gpu_spp": 1024, "fp64_spp": 2048}, cpu_baseline=None)
+                                   "gpu_spp": 1024, "fp64_spp": 4096}, cpu_baseline=None)
 
 
 # ------------------------------------------------------------------ serve
