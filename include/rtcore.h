@@ -220,6 +220,76 @@ int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc,
  * per pixel per light, 0 lit, 1 occluded, 2 no shadow ray (KEEP_HITS). */
 int rt_frame_read(rt_ctx *ctx, float *rgba, rt_hit24 *hits, uint8_t *occluded);
 
+/* ---- Whitted frames (the reference's ray-tracing colour stage) */
+/* Material: Structures.hlsl:69-78 (64 bytes), the layout of MaterialData
+ * (Common/Scenes/MaterialData.h:24-28) that fills the material buffer.  As the
+ * colour stage uses it (CSComputeColorRT.hlsl:179-236): diffuse.rgb is the
+ * surface colour and diffuse.w the weight of the lit diffuse term (1 - w
+ * weights the specular term); ambient.rgb is added to the diffuse term and
+ * ambient.w is the index of refraction; specular.rgb colours the specular
+ * term and the reflected continuation, specular.w is the gloss (times the
+ * RT_WHITTED_GLOSS switch); alpha is the probability that the continuation
+ * ray reflects rather than refracts.  shininess is carried but unused (the
+ * specular exponent is the constant 49). */
+typedef struct rt_material {
+    float diffuse[4];
+    float ambient[4];
+    float specular[4];
+    float shininess;
+    float alpha;
+    uint32_t id;
+    uint32_t pad;
+} rt_material;
+
+/* Shading attributes of an uploaded scene: vertex normals (Model vertices,
+ * Model.cpp:320-333), one material id per triangle (PrimitiveCS,
+ * RayTracerCS.cpp:322-327) and the material table (RayTracerCS.cpp:348-363).
+ * normals: 3 floats per vertex in rt_scene_upload's vertex order, nv equal
+ * to the uploaded count; NULL keeps flat shading only.  mat_ids: nt entries
+ * (nt equal to the uploaded count), each < nmat; NULL means material 0
+ * everywhere.  materials: nmat entries (1..65536); NULL with nmat 0 installs
+ * the reference's default MaterialData (MaterialData.cpp:17-24).  Upload the
+ * geometry first; a later rt_scene_upload with other counts voids these. */
+int rt_scene_shading(rt_ctx *ctx, int scene, const float *normals, uint32_t nv,
+                     const uint32_t *mat_ids, uint32_t nt,
+                     const rt_material *materials, uint32_t nmat);
+
+/* Whitted frame: RayTracerCS::Render (RayTracerCS.cpp:286-295) in the
+ * ray-tracing mode -- primary rays (CSPrimaryRays.hlsl, no jitter), then
+ * bounces + 1 passes of closest hit (CSIntersections.hlsl:18-39) and colour
+ * (CSComputeColorRT.hlsl:18-262), one lane carrying its pixel through all
+ * passes.  A hit adds its material colour lit by one point light (diffuse
+ * max(0, -n.L), specular max(0, -L.R)^49, a shadow ray when
+ * RT_WHITTED_SHADOWS) times the ray's reflective factor, then continues:
+ * reflected with probability alpha (factor *= specular.rgb), else refracted
+ * (factor *= diffuse.rgb; total internal reflection reflects).  A miss adds
+ * factor * env and ends the path; with env_flag 0 a miss instead paints the
+ * pixel (1, 1, 1, 1), as the reference's environment-mapping switch does.
+ * Normals are flat, or interpolated vertex normals with RT_WHITTED_PHONG
+ * (needs rt_scene_shading normals).  The reflect/refract draw is a PCG32
+ * stream keyed by (seed, frame, pixel).  Texture, normal and gloss maps are
+ * not sampled: this is the reference with texturing and normal mapping off.
+ * Output: rt_frame_read's rgba is the reference's result texture; with
+ * RT_FLAG_KEEP_HITS its hits / occluded are the first pass's (one light). */
+enum {
+    RT_WHITTED_SHADOWS = 1u << 0, /* g_bIsShadowOn */
+    RT_WHITTED_PHONG = 1u << 1,   /* g_bIsPhongShadingOn */
+    RT_WHITTED_GLOSS = 1u << 2    /* g_bIsGlossMappingOn (specular.w) */
+};
+typedef struct rt_whitted_desc {
+    rt_camera cam;
+    uint32_t width, height;
+    float light[3];    /* point light position (g_vfLightPosition) */
+    uint32_t bounces;  /* 0..16; passes = bounces + 1 (g_iNumBounces) */
+    uint32_t switches; /* RT_WHITTED_* */
+    int32_t env_flag;  /* -1 (reference default) or 0 (g_iEnvMappingFlag) */
+    float env[3];      /* colour of a miss (a uniform environment map) */
+    uint32_t frame;
+    uint64_t seed;
+} rt_whitted_desc;
+int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc,
+                      uint32_t flags, rt_frame_stats *stats);
+
 /* ---- profiling window (Timer role, DeviceCore/Timer.hpp:27-66; Performance,
  *      Common/Performance.cpp:52-101).  Between begin and end every build,
  *      trace and render call records HIP events around its kernels on the
@@ -370,6 +440,26 @@ enum { RT_LOAD_RAW = 1u << 0 };
 int rt_load_model(const char *path, uint32_t flags, float *xyz, uint32_t *nv,
                   uint32_t *idx, uint32_t *nt);
 const char *rt_load_last_error(void);
+
+/* The same model with its shading attributes, as Model::LoadFile hands them
+ * to the renderer.  Normals: the file's (".txt": normalised; ".obj": as
+ * written, zero for corners without one) or, for an OBJ with no "vn" at all,
+ * ModelOBJ::generateNormals of the raw positions (model_obj.cpp:541-616).
+ * Materials: an OBJ's MTL file (mtllib, read as ModelOBJ::importMaterials,
+ * model_obj.cpp:1119-1297) gives the table, and with more than one material
+ * each triangle the one active at its face (usemtl; unknown names -> 0);
+ * without materials, and for ".txt", one default MaterialData.
+ * In: capacities in nv, nt, nmat; out: counts.  xyz = idx = NULL asks for
+ * the counts only; normals, mat_ids and materials are optional outputs. */
+typedef struct rt_model {
+    float *xyz;             /* 3 per vertex */
+    float *normals;         /* 3 per vertex */
+    uint32_t *idx;          /* 3 per triangle */
+    uint32_t *mat_ids;      /* 1 per triangle */
+    rt_material *materials; /* nmat */
+    uint32_t nv, nt, nmat;
+} rt_model;
+int rt_load_model_ex(const char *path, uint32_t flags, rt_model *model);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,8 @@
  *   - orc_raygen_pinhole                 : pixel-centre rule of CSPrimaryRays.hlsl:37-40
  *   - orc_render_direct                  : shadow-ray construction + direct term of
  *                                          CSComputeColorRT.hlsl:54-177
+ *   - orc_render_whitted                 : the ray-tracing frame, RayTracerCS.cpp:286-295 +
+ *                                          CSComputeColorRT.hlsl:18-262 (whitted_oracle.c)
  *
  * Floating point: every float expression is evaluated in IEEE single
  * precision, in the order written, with no contraction into FMA (the build
@@ -118,6 +120,38 @@ uint64_t orc_render_gi(const orc_node *nodes, const int32_t *ids, const float *x
                        uint32_t spp, uint32_t max_bounces, double albedo, double sky,
                        uint64_t seed, uint32_t frame, uint32_t first_sample, double *out,
                        int nthreads);
+
+/* Material (Structures.hlsl:69-78), 64 bytes. */
+typedef struct {
+    float diffuse[4];
+    float ambient[4];
+    float specular[4];
+    float shininess, alpha;
+    uint32_t id, pad;
+} orc_material;
+
+/* Whitted frame parameters (whitted_oracle.c).  switches: 1 shadows,
+ * 2 Phong normals, 4 gloss; env_flag -1 or 0; ref_order 1 starts each ray at
+ * the previous hit's leaf as the reference does, 0 at the root. */
+typedef struct {
+    float light[3];
+    uint32_t bounces, switches;
+    int32_t env_flag;
+    float env[3];
+    uint32_t frame;
+    uint64_t seed;
+    int32_t ref_order;
+} orc_whitted_params;
+
+/* Whitted frame of w x h pixels into rgba (4 floats per pixel); normals,
+ * mat_ids and mats may be NULL (flat only / material 0 / the default
+ * MaterialData).  hits0 / occ0 (optional) receive the first pass's hit and
+ * shadow result (0 lit, 1 occluded, 2 none).  Returns the rays traced. */
+uint64_t orc_render_whitted(const orc_node *nodes, const int32_t *ids, const float *xyz,
+                            const uint32_t *idx, const float *normals, const uint32_t *mat_ids,
+                            const orc_material *mats, const float cam[14], uint32_t w, uint32_t h,
+                            const orc_whitted_params *p, float *rgba, orc_hit *hits0, uint8_t *occ0,
+                            int nthreads);
 
 #ifdef __cplusplus
 }

@@ -20,7 +20,15 @@ RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("refl", "<f4", 3), ("tm
                       ("tmax", "<f4"), ("tri", "<i4")])
 HIT_DTYPE = np.dtype([("tri", "<i4"), ("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("visited", "<i4"),
                       ("root", "<i4")])
+MATERIAL_DTYPE = np.dtype([("diffuse", "<f4", 4), ("ambient", "<f4", 4), ("specular", "<f4", 4),
+                           ("shininess", "<f4"), ("alpha", "<f4"), ("id", "<u4"), ("pad", "<u4")])
 INVALID = -2147483647
+
+
+class WhittedParams(C.Structure):
+    _fields_ = [("light", C.c_float * 3), ("bounces", C.c_uint32), ("switches", C.c_uint32),
+                ("env_flag", C.c_int32), ("env", C.c_float * 3), ("frame", C.c_uint32),
+                ("seed", C.c_uint64), ("ref_order", C.c_int32)]
 
 
 def build_library(force: bool = False) -> None:
@@ -50,6 +58,9 @@ def lib():
         L.orc_render_gi.argtypes = [P, P, P, P, P, U32, U32, U32, U32, C.c_double, C.c_double,
                                     C.c_uint64, U32, U32, P, I32]
         L.orc_render_gi.restype = C.c_uint64
+        L.orc_render_whitted.argtypes = [P, P, P, P, P, P, P, P, U32, U32, C.POINTER(WhittedParams), P, P,
+                                         P, I32]
+        L.orc_render_whitted.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -160,6 +171,32 @@ def render_gi(nodes, ids, xyz, idx, cam14, w, h, spp, max_bounces, albedo, sky, 
                                _p(np.ascontiguousarray(cam14, np.float32)), w, h, spp, max_bounces,
                                albedo, sky, seed, frame, first_sample, _p(out), nthreads)
     return out.reshape(h, w), int(rays)
+
+
+def render_whitted(nodes, ids, xyz, idx, cam14, w, h, light, bounces=2, switches=1, env_flag=-1,
+                   env=(0.0, 0.0, 0.0), seed=1, frame=0, normals=None, mat_ids=None, materials=None,
+                   ref_order=False, nthreads=1):
+    """Whitted frame (whitted_oracle.c); returns (rgba [h*w, 4], first-pass hits, first-pass
+    shadow results, rays traced)."""
+    p = WhittedParams()
+    p.light[:] = [float(x) for x in light]
+    p.bounces, p.switches, p.env_flag = bounces, switches, env_flag
+    p.env[:] = [float(x) for x in env]
+    p.frame, p.seed, p.ref_order = frame, seed, 1 if ref_order else 0
+    n = w * h
+    rgba = np.empty((n, 4), np.float32)
+    hits = np.empty(n, HIT_DTYPE)
+    occ = np.empty(n, np.uint8)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float32)
+    mid = None if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32)
+    mats = None if materials is None else np.ascontiguousarray(materials, MATERIAL_DTYPE)
+    rays = lib().orc_render_whitted(_p(np.ascontiguousarray(nodes, NODE_DTYPE)),
+                                    _p(np.ascontiguousarray(ids, np.int32)),
+                                    _p(np.ascontiguousarray(xyz, np.float32)),
+                                    _p(np.ascontiguousarray(idx, np.uint32)), _p(nrm), _p(mid), _p(mats),
+                                    _p(np.ascontiguousarray(cam14, np.float32)), w, h, C.byref(p),
+                                    _p(rgba), _p(hits), _p(occ), nthreads)
+    return rgba, hits, occ, int(rays)
 
 
 def reachable(nodes: np.ndarray) -> np.ndarray:

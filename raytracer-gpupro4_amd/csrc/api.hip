@@ -65,6 +65,7 @@ void batch_free(Batch *b)
 {
     if (!b) return;
     for (void *p : b->allocs) hipFree(p);
+    for (Batch::Shading &s : b->shading) hipFree(s.mats);
     delete b;
 }
 
@@ -467,6 +468,63 @@ int rt_scene_upload_device(rt_ctx *ctx, int scene, const float *d_xyz, uint32_t 
     return finish_upload(ctx, b, i, nv, nt);
 }
 
+int rt_scene_shading(rt_ctx *ctx, int scene, const float *normals, uint32_t nv, const uint32_t *mat_ids,
+                     uint32_t nt, const rt_material *materials, uint32_t nmat)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_scene_shading: unknown scene %d", scene);
+    const SceneDesc &d = b->desc[i];
+    if (nv != d.nv || nt != d.nt)
+        return fail(ctx, RT_ERR_ARG, "rt_scene_shading: scene %d has %u vertices / %u triangles, got %u / %u", scene,
+                    d.nv, d.nt, nv, nt);
+    if (materials ? (nmat == 0 || nmat > 65536) : nmat != 0)
+        return fail(ctx, RT_ERR_ARG, "rt_scene_shading: 1..65536 materials with a table, or none (got %u)", nmat);
+    const uint32_t ntab = materials ? nmat : 1u;
+    if (mat_ids)
+        for (uint32_t t = 0; t < nt; ++t)
+            if (mat_ids[t] >= ntab)
+                return fail(ctx, RT_ERR_ARG, "rt_scene_shading: triangle %u has material %u of %u", t, mat_ids[t],
+                            ntab);
+    hipSetDevice(ctx->device);
+    if ((normals && !b->nrm) || (mat_ids && !b->mat)) {
+        uint64_t cap_v = 0, cap_t = 0;
+        for (uint32_t k = 0; k < b->nscenes; ++k) {
+            cap_v += b->max_verts[k];
+            cap_t += b->max_tris[k];
+        }
+        if (normals && !b->nrm) RT_HIP(ctx, batch_alloc(*b, &b->nrm, 3 * (size_t)(cap_v ? cap_v : 1)));
+        if (mat_ids && !b->mat) RT_HIP(ctx, batch_alloc(*b, &b->mat, (size_t)(cap_t ? cap_t : 1)));
+    }
+    if (b->shading.size() < b->nscenes) b->shading.resize(b->nscenes);
+    Batch::Shading &S = b->shading[i];
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the old table may still be read
+    if (S.mats) {
+        hipFree(S.mats);
+        b->bytes -= sizeof(rt_material) * S.nmat;
+        S.mats = nullptr;
+        S.nmat = 0;
+    }
+    if (materials) {
+        RT_HIP(ctx, hipMalloc(&S.mats, sizeof(rt_material) * nmat));
+        S.nmat = nmat;
+        b->bytes += sizeof(rt_material) * nmat;
+        RT_HIP(ctx, hipMemcpyAsync(S.mats, materials, sizeof(rt_material) * nmat, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    }
+    if (normals && nv)
+        RT_HIP(ctx, hipMemcpyAsync(b->nrm + 3 * (size_t)d.vert_off, normals, 12ull * nv, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    if (mat_ids && nt)
+        RT_HIP(ctx, hipMemcpyAsync(b->mat + d.in_prim_off, mat_ids, 4ull * nt, hipMemcpyHostToDevice, ctx->stream));
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    S.normals = normals != nullptr;
+    S.ids = mat_ids != nullptr;
+    S.nv = nv;
+    S.nt = nt;
+    return RT_OK;
+}
+
 int rt_scene_transform(rt_ctx *ctx, int scene, const float m[12])
 {
     Batch *b;
@@ -667,6 +725,55 @@ int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, uint32_
         stats->tri_tests = c[1];
     }
     ctx->frame.nlights = desc->nlights > 0 ? desc->nlights : 1;
+    return RT_OK;
+}
+
+int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint32_t flags, rt_frame_stats *stats)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i) || !desc) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: bad argument");
+    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_render_whitted: scene %d not built", scene);
+    if (desc->width == 0 || desc->height == 0 || desc->width > 16384 || desc->height > 16384)
+        return fail(ctx, RT_ERR_ARG, "rt_render_whitted: image %ux%u", desc->width, desc->height);
+    if (desc->bounces > 16) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: %u bounces (0..16)", desc->bounces);
+    if (desc->env_flag != -1 && desc->env_flag != 0)
+        return fail(ctx, RT_ERR_ARG, "rt_render_whitted: env_flag %d (-1 or 0)", desc->env_flag);
+    if (desc->switches & ~(uint32_t)(RT_WHITTED_SHADOWS | RT_WHITTED_PHONG | RT_WHITTED_GLOSS))
+        return fail(ctx, RT_ERR_ARG, "rt_render_whitted: unknown switches 0x%x", desc->switches);
+    const SceneDesc &d = b->desc[i];
+    const bool have = i < b->shading.size();
+    if (have && (b->shading[i].nv != d.nv || b->shading[i].nt != d.nt) &&
+        (b->shading[i].normals || b->shading[i].ids || b->shading[i].mats))
+        return fail(ctx, RT_ERR_STATE, "rt_render_whitted: scene %d was re-uploaded after rt_scene_shading", scene);
+    if ((desc->switches & RT_WHITTED_PHONG) && !(have && b->shading[i].normals))
+        return fail(ctx, RT_ERR_STATE, "rt_render_whitted: Phong shading needs rt_scene_shading normals");
+    hipSetDevice(ctx->device);
+    int rc = ensure_frame(ctx, desc->width, desc->height, 1);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    const bool timing = (flags & RT_FLAG_TIMING) != 0 || stats != nullptr;
+    if (stats || !ctx->profiling)
+        RT_HIP(ctx, hipMemsetAsync(ctx->stats, 0, 8 * sizeof(unsigned long long), st));
+    hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
+    if (!stats && ctx->profiling) ev = prof_events(ctx, 1, 2);
+    if (ev) hipEventRecord(ev[0], st);
+    RT_HIP(ctx, rt::launch_render_whitted(*b, i, *desc, ctx->frame, flags, ctx->stats, ctx->tune, st));
+    if (ev) hipEventRecord(ev[1], st);
+    if (ctx->profiling) ctx->prof_primary += (uint64_t)desc->width * desc->height;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        unsigned long long c[8] = {};
+        RT_HIP(ctx, hipMemcpyAsync(c, ctx->stats, sizeof(c), hipMemcpyDeviceToHost, st));
+        RT_HIP(ctx, hipStreamSynchronize(st));
+        hipEventElapsedTime(&stats->trace_ms, ctx->ev[6], ctx->ev[7]);
+        stats->primary_rays = (uint64_t)desc->width * desc->height;
+        stats->shadow_rays = c[2];
+        stats->bounce_rays = c[3];
+        stats->box_tests = c[0];
+        stats->tri_tests = c[1];
+    }
+    ctx->frame.nlights = 1;
     return RT_OK;
 }
 

@@ -1,26 +1,36 @@
 // Scene files: the reference's two model formats, loaded to the vertex and
-// index arrays rt_scene_upload takes, with the reference's normalisation.
+// index arrays rt_scene_upload takes, with the reference's normalisation, and
+// to the normals, material ids and materials rt_scene_shading takes.
 //
 // Model::LoadFile (Common/Scenes/Model.cpp:210-381) reads
 //   - ".txt": vertex count, triangle count, per vertex "x y z nx ny nz u v",
-//     per triangle "a b c" (:214-265);
+//     per triangle "a b c" (:214-265); normals are normalised (Normalize,
+//     Common/Geometry/Geometry.h:268-275) and every triangle gets the default
+//     MaterialData (MaterialData.cpp:17-24);
 //   - ".obj": through ModelOBJ (Common/Scenes/Loaders/model_obj.cpp), which
 //     fan-triangulates faces of the forms v, v/vt, v//vn, v/vt/vn in file
 //     order (:922-1116), gives each distinct (position index, texture
 //     coordinate, normal) its own vertex in order of first use (addVertex,
-//     :451-493), then normalize() centres the box and divides by its largest
-//     extent (:228-256, bounds :77-128);
-// and Model::Scale (Model.cpp:647-709) then fits every model into a sphere
-// of radius 0.49: centre the box and scale its largest extent to 1.8, then
-// scale by 0.49 / max |p|.  Float arithmetic follows those functions'
-// operations and order.
+//     :451-493), builds normals from the raw positions when the file has none
+//     (generateNormals, :541-616), reads the MTL file named by mtllib
+//     (importMaterials, :1119-1297), then normalize() centres the box and
+//     divides by its largest extent (:228-256, bounds :77-128).  Model then
+//     takes one default MaterialData when the OBJ brought no materials, gives
+//     every triangle material 0 when it brought one, and otherwise the
+//     material active (usemtl) at the triangle's face (:284-378).
+// Model::Scale (Model.cpp:647-709) then fits every model into a sphere of
+// radius 0.49: centre the box and scale its largest extent to 1.8, then scale
+// by 0.49 / max |p|.  Float arithmetic follows those functions' operations
+// and order.
 //
-// Two reference behaviours are not reproduced: a negative OBJ index -k here
-// is the k-th most recent vertex (the OBJ convention; the reference maps it
-// one further back), and the bounds start from the lowest float (the
-// reference starts its maxima at the smallest positive float, which breaks
-// models lying entirely below zero on an axis).  Materials, normals and
-// texture coordinates only steer vertex numbering; the core uses positions.
+// Reference behaviours not reproduced: a negative OBJ index -k here is the
+// k-th most recent vertex (the OBJ convention; the reference maps it one
+// further back); the bounds start from the lowest float (the reference
+// starts its maxima at the smallest positive float, which breaks models lying
+// entirely below zero on an axis); of several mtllib lines the last one
+// names the materials (the reference merges their name tables); a material
+// property before any newmtl is an error (the reference writes through a
+// null pointer).  Texture file names are not kept: no map is sampled.
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
@@ -39,8 +49,24 @@ namespace {
 
 struct Loaded {
     std::vector<float> xyz;
+    std::vector<float> nrm;  // 3 per vertex
     std::vector<uint32_t> idx;
+    std::vector<uint32_t> mat;  // 1 per triangle
+    std::vector<rt_material> mats;
 };
+
+// MaterialData() (Common/Scenes/MaterialData.cpp:17-24).
+rt_material default_material_data()
+{
+    rt_material m;
+    memset(&m, 0, sizeof(m));
+    m.diffuse[0] = m.diffuse[1] = m.diffuse[2] = 0.5f;
+    m.diffuse[3] = 0.8f;
+    m.ambient[3] = 1.0f;
+    m.specular[0] = m.specular[1] = m.specular[2] = 0.8f;
+    m.specular[3] = 1.0f;
+    return m;
+}
 
 bool ends_with_ci(const std::string &s, const char *suffix)
 {
@@ -157,10 +183,131 @@ bool parse_corner(const char *tok, int nv, int nvt, int nvn, int &v, int &vt, in
     return true;
 }
 
-int load_obj(FILE *f, Loaded &out, std::string &err)
+// The name argument of an mtllib / newmtl / usemtl line as the reference's
+// sscanf(buffer, "%s %s", buffer, buffer) leaves it: the second of the first
+// two words when there are two, else the first.
+std::string name_arg(char *rest)
+{
+    char *save = nullptr;
+    char *w0 = strtok_r(rest, " \t\r\n", &save);
+    if (!w0) return std::string();
+    char *w1 = strtok_r(nullptr, " \t\r\n", &save);
+    return std::string(w1 ? w1 : w0);
+}
+
+struct MtlTable {
+    std::vector<rt_material> mats;
+    std::map<std::string, int> cache;  // name -> index (a repeated name maps to its last definition)
+};
+
+// ModelOBJ::importMaterials (model_obj.cpp:1119-1297): "newmtl" starts a
+// material with ModelOBJ's defaults (ambient 0.2, diffuse 0.8, specular 0,
+// each with w = 1, shininess 0, alpha 1); Ka / Kd / Ks read up to four
+// numbers and set w = 1 when fewer than four convert; Ns is divided by 1000;
+// Tr sets alpha = 1 - value, d sets alpha; illum 1 zeroes specular.rgb and
+// sets its w to 1.  A file that cannot be opened brings no materials.
+int load_mtl(const std::string &path, MtlTable &t, std::string &err)
+{
+    t.mats.clear();
+    t.cache.clear();
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return RT_OK;
+    char *line = nullptr;
+    size_t cap = 0;
+    long lineno = 0;
+    int illum = 0;  // keeps its last value when an illum line does not convert, as in the reference
+    rt_material *m = nullptr;
+    int rc = RT_OK;
+    while (getline(&line, &cap, f) >= 0) {
+        ++lineno;
+        char *save = nullptr;
+        char *tok = strtok_r(line, " \t\r\n", &save);
+        if (!tok) continue;
+        char *rest = save ? save : tok + strlen(tok);
+        const char c0 = tok[0], c1 = tok[1];
+        if (c0 == 'n') {  // newmtl
+            rt_material d;
+            memset(&d, 0, sizeof(d));
+            d.ambient[0] = d.ambient[1] = d.ambient[2] = 0.2f;
+            d.ambient[3] = 1.0f;
+            d.diffuse[0] = d.diffuse[1] = d.diffuse[2] = 0.8f;
+            d.diffuse[3] = 1.0f;
+            d.specular[3] = 1.0f;
+            d.shininess = 0.0f;
+            d.alpha = 1.0f;
+            d.id = (uint32_t)t.mats.size();
+            t.cache[name_arg(rest)] = (int)t.mats.size();
+            t.mats.push_back(d);
+            m = &t.mats.back();
+            continue;
+        }
+        const bool prop = (c0 == 'N' && c1 == 's') || (c0 == 'K' && (c1 == 'a' || c1 == 'd' || c1 == 's')) ||
+                          (c0 == 'T' && c1 == 'r') || c0 == 'd' || c0 == 'i';
+        if (!prop) continue;  // comments, maps, Ni, Ke, Tf and the rest
+        if (!m) {
+            err = path + ": line " + std::to_string(lineno) + ": '" + tok + "' before any newmtl";
+            rc = RT_ERR_ARG;
+            break;
+        }
+        if (c0 == 'N') {
+            sscanf(rest, "%f", &m->shininess);
+            m->shininess /= 1000.0f;
+        } else if (c0 == 'K') {
+            float *dst = c1 == 'a' ? m->ambient : (c1 == 'd' ? m->diffuse : m->specular);
+            if (sscanf(rest, "%f %f %f %f", &dst[0], &dst[1], &dst[2], &dst[3]) < 4) dst[3] = 1.0f;
+        } else if (c0 == 'T') {
+            sscanf(rest, "%f", &m->alpha);
+            m->alpha = 1.0f - m->alpha;
+        } else if (c0 == 'd') {
+            sscanf(rest, "%f", &m->alpha);
+        } else {  // illum
+            sscanf(rest, "%d", &illum);
+            if (illum == 1) {
+                m->specular[0] = m->specular[1] = m->specular[2] = 0.0f;
+                m->specular[3] = 1.0f;
+            }
+        }
+    }
+    free(line);
+    fclose(f);
+    return rc;
+}
+
+// ModelOBJ::generateNormals: per vertex the sum of its faces' unnormalised
+// cross(B - A, C - A), times 1 / length.
+void generate_normals(const std::vector<float> &xyz, const std::vector<uint32_t> &idx, std::vector<float> &nrm)
+{
+    nrm.assign(xyz.size(), 0.0f);
+    for (size_t t = 0; t + 2 < idx.size(); t += 3) {
+        const float *A = &xyz[3 * (size_t)idx[t]], *B = &xyz[3 * (size_t)idx[t + 1]], *C = &xyz[3 * (size_t)idx[t + 2]];
+        const float e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+        const float e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+        const float n[3] = {(e1[1] * e2[2]) - (e1[2] * e2[1]), (e1[2] * e2[0]) - (e1[0] * e2[2]),
+                            (e1[0] * e2[1]) - (e1[1] * e2[0])};
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) nrm[3 * (size_t)idx[t + k] + c] += n[c];
+    }
+    for (size_t v = 0; v + 2 < nrm.size(); v += 3) {
+        float *n = &nrm[v];
+        const float len = 1.0f / sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+        n[0] *= len;
+        n[1] *= len;
+        n[2] *= len;
+    }
+}
+
+std::string dir_of(const std::string &path)
+{
+    size_t k = path.find_last_of('\\');
+    if (k == std::string::npos) k = path.find_last_of('/');
+    return k == std::string::npos ? std::string() : path.substr(0, k + 1);
+}
+
+int load_obj(const std::string &path, FILE *f, bool want_materials, Loaded &out, std::string &err)
 {
     std::vector<float> pos, tex, nrm;
     std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, uint32_t> cache;
+    MtlTable mtl;
     char *line = nullptr;
     size_t cap = 0;
     long lineno = 0;
@@ -168,6 +315,26 @@ int load_obj(FILE *f, Loaded &out, std::string &err)
         char **p;
         ~Free() { free(*p); }
     } guard{&line};
+    // ModelOBJ reads the materials in its first pass, so usemtl resolves
+    // against them wherever the mtllib line stands.
+    if (want_materials) {
+        std::string lib;
+        bool have = false;
+        while (getline(&line, &cap, f) >= 0) {
+            char *save = nullptr;
+            char *tok = strtok_r(line, " \t\r\n", &save);
+            if (tok && strcmp(tok, "mtllib") == 0) {
+                lib = name_arg(save ? save : tok + strlen(tok));
+                have = true;
+            }
+        }
+        rewind(f);
+        if (have) {
+            const int rc = load_mtl(dir_of(path) + lib, mtl, err);
+            if (rc != RT_OK) return rc;
+        }
+    }
+    int active = 0;
     while (getline(&line, &cap, f) >= 0) {
         ++lineno;
         char *save = nullptr;
@@ -185,6 +352,11 @@ int load_obj(FILE *f, Loaded &out, std::string &err)
                     return RT_ERR_ARG;
                 }
                 dst.push_back(x);
+            }
+        } else if (strcmp(tok, "usemtl") == 0) {
+            if (want_materials) {
+                const auto it = mtl.cache.find(name_arg(save ? save : tok + strlen(tok)));
+                active = it == mtl.cache.end() ? 0 : it->second;
             }
         } else if (strcmp(tok, "f") == 0) {
             std::vector<uint32_t> corners;
@@ -209,6 +381,7 @@ int load_obj(FILE *f, Loaded &out, std::string &err)
                 if (it == cache.end()) {
                     id = (uint32_t)(out.xyz.size() / 3);
                     out.xyz.insert(out.xyz.end(), &pos[3 * v], &pos[3 * v] + 3);
+                    out.nrm.insert(out.nrm.end(), nn, nn + 3);
                     cache.emplace(key, id);
                 } else {
                     id = it->second;
@@ -223,9 +396,18 @@ int load_obj(FILE *f, Loaded &out, std::string &err)
                 out.idx.push_back(corners[0]);
                 out.idx.push_back(corners[k - 1]);
                 out.idx.push_back(corners[k]);
+                out.mat.push_back((uint32_t)active);
             }
         }
-        // mtllib, usemtl, o, g, s and the rest carry no geometry
+        // o, g, s and the rest carry nothing the renderer uses
+    }
+    if (nrm.empty()) generate_normals(out.xyz, out.idx, out.nrm);
+    if (mtl.mats.empty()) {
+        out.mats.assign(1, default_material_data());
+        std::fill(out.mat.begin(), out.mat.end(), 0u);
+    } else {
+        out.mats = mtl.mats;
+        if (out.mats.size() == 1) std::fill(out.mat.begin(), out.mat.end(), 0u);
     }
     return RT_OK;
 }
@@ -238,6 +420,7 @@ int load_txt(FILE *f, Loaded &out, std::string &err)
         return RT_ERR_ARG;
     }
     out.xyz.resize(3 * (size_t)nv);
+    out.nrm.resize(3 * (size_t)nv);
     for (long i = 0; i < nv; ++i) {
         float r[8];
         if (fscanf(f, "%f %f %f %f %f %f %f %f", &r[0], &r[1], &r[2], &r[3], &r[4], &r[5], &r[6], &r[7]) != 8) {
@@ -245,6 +428,10 @@ int load_txt(FILE *f, Loaded &out, std::string &err)
             return RT_ERR_ARG;
         }
         memcpy(&out.xyz[3 * (size_t)i], r, 12);
+        const float len = sqrtf((r[3] * r[3] + r[4] * r[4]) + r[5] * r[5]);
+        out.nrm[3 * (size_t)i] = r[3] / len;
+        out.nrm[3 * (size_t)i + 1] = r[4] / len;
+        out.nrm[3 * (size_t)i + 2] = r[5] / len;
     }
     out.idx.resize(3 * (size_t)nt);
     for (long i = 0; i < nt; ++i) {
@@ -257,19 +444,20 @@ int load_txt(FILE *f, Loaded &out, std::string &err)
         out.idx[3 * i + 1] = (uint32_t)b;
         out.idx[3 * i + 2] = (uint32_t)c;
     }
+    out.mat.assign((size_t)nt, 0u);
+    out.mats.assign(1, default_material_data());
     return RT_OK;
 }
 
 thread_local std::string g_load_err;
 
-}  // namespace
-
-extern "C" {
-
-int rt_load_model(const char *path, uint32_t flags, float *xyz, uint32_t *nv, uint32_t *idx, uint32_t *nt)
+int load_any(const char *path, uint32_t flags, bool want_materials, Loaded &m)
 {
-    if (!path || !nv || !nt) return RT_ERR_ARG;
     g_load_err.clear();
+    if (!path) {
+        g_load_err = "rt_load_model: null path";
+        return RT_ERR_ARG;
+    }
     const std::string p(path);
     const bool obj = ends_with_ci(p, ".obj"), txt = ends_with_ci(p, ".txt");
     if (!obj && !txt) {
@@ -281,9 +469,8 @@ int rt_load_model(const char *path, uint32_t flags, float *xyz, uint32_t *nv, ui
         g_load_err = "rt_load_model: cannot open " + p;
         return RT_ERR_ARG;
     }
-    Loaded m;
     std::string err;
-    const int rc = obj ? load_obj(f, m, err) : load_txt(f, m, err);
+    const int rc = obj ? load_obj(p, f, want_materials, m, err) : load_txt(f, m, err);
     fclose(f);
     if (rc != RT_OK) {
         g_load_err = "rt_load_model: " + p + ": " + err;
@@ -297,6 +484,19 @@ int rt_load_model(const char *path, uint32_t flags, float *xyz, uint32_t *nv, ui
         if (obj) obj_normalize(m.xyz);
         model_scale(m.xyz);
     }
+    return RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_load_model(const char *path, uint32_t flags, float *xyz, uint32_t *nv, uint32_t *idx, uint32_t *nt)
+{
+    if (!path || !nv || !nt) return RT_ERR_ARG;
+    Loaded m;
+    const int rc = load_any(path, flags, false, m);
+    if (rc != RT_OK) return rc;
     const uint32_t cv = (uint32_t)(m.xyz.size() / 3), ct = (uint32_t)(m.idx.size() / 3);
     if (xyz || idx) {
         if (!xyz || !idx || *nv < cv || *nt < ct) {
@@ -308,6 +508,32 @@ int rt_load_model(const char *path, uint32_t flags, float *xyz, uint32_t *nv, ui
     }
     *nv = cv;
     *nt = ct;
+    return RT_OK;
+}
+
+int rt_load_model_ex(const char *path, uint32_t flags, rt_model *model)
+{
+    if (!path || !model) return RT_ERR_ARG;
+    Loaded m;
+    const int rc = load_any(path, flags, true, m);
+    if (rc != RT_OK) return rc;
+    const uint32_t cv = (uint32_t)(m.xyz.size() / 3), ct = (uint32_t)(m.idx.size() / 3);
+    const uint32_t cm = (uint32_t)m.mats.size();
+    if (model->xyz || model->idx) {
+        if (!model->xyz || !model->idx || model->nv < cv || model->nt < ct ||
+            (model->materials && model->nmat < cm)) {
+            g_load_err = "rt_load_model_ex: output arrays too small";
+            return RT_ERR_CAPACITY;
+        }
+        memcpy(model->xyz, m.xyz.data(), m.xyz.size() * sizeof(float));
+        memcpy(model->idx, m.idx.data(), m.idx.size() * sizeof(uint32_t));
+        if (model->normals) memcpy(model->normals, m.nrm.data(), m.nrm.size() * sizeof(float));
+        if (model->mat_ids) memcpy(model->mat_ids, m.mat.data(), m.mat.size() * sizeof(uint32_t));
+        if (model->materials) memcpy(model->materials, m.mats.data(), m.mats.size() * sizeof(rt_material));
+    }
+    model->nv = cv;
+    model->nt = ct;
+    model->nmat = cm;
     return RT_OK;
 }
 

@@ -138,6 +138,17 @@ struct Batch {
     uint64_t bytes = 0;
     std::vector<void *> allocs;
     int refs = 0;                  // scene handles pointing here
+    // Shading attributes (rt_scene_shading), allocated at their first upload:
+    // normals in the vertex layout of xyz, material ids in the triangle
+    // layout of idx, and one material table per scene.
+    float *nrm = nullptr;
+    uint32_t *mat = nullptr;
+    struct Shading {
+        rt_material *mats = nullptr;  // device; null: the default MaterialData
+        uint32_t nmat = 0, nv = 0, nt = 0;
+        bool normals = false, ids = false;
+    };
+    std::vector<Shading> shading;  // per scene
 };
 
 struct SceneRef {
@@ -206,6 +217,9 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
 hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st);
+hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitted_desc &d,
+                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
+                                 const rt_tuning &tune, hipStream_t st);
 size_t render_job_bytes();
 void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_desc &d,
                      const rt_tuning &t, uint32_t tile_off, uint32_t *tiles);

@@ -651,6 +651,232 @@ __device__ __forceinline__ void cosine_dir(float nx, float ny, float nz, float r
     dz = (lx * tz + ly * bz) + lz * nz;
 }
 
+// ---------------------------------------------------------------------------
+// Whitted frames: the reference's ray-tracing mode (RayTracerCS.cpp:286-295)
+// runs CSIntersections and CSComputeColor (CSComputeColorRT.hlsl:18-262) once
+// per pass over a ray buffer in memory.  Here one lane carries its pixel
+// through every pass, so rays, hits and the accumulator stay in registers.
+// ---------------------------------------------------------------------------
+struct ShadeView {
+    const float *nrm;         // the scene's vertex normals (null: flat shading only)
+    const uint32_t *idx;      // the scene's vertex indices, 3 per triangle
+    const uint32_t *mat;      // the scene's material id per triangle (null: material 0)
+    const rt_material *mats;  // the scene's materials (null: defmat)
+};
+
+struct WhittedParams {
+    rt_camera cam;
+    uint32_t width, height, bounces, switches;
+    float light[3];
+    int32_t env_flag;
+    float env[3];
+    uint32_t frame;
+    uint64_t seed;
+    rt_material defmat;
+};
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return (ax * bx + ay * by) + az * bz;
+}
+
+// x^49 as x^32 * x^16 * x by repeated squaring (the shader's pow(x, 49)).
+__device__ __forceinline__ float pow49(float x)
+{
+    const float x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8, x32 = x16 * x16;
+    return (x32 * x16) * x;
+}
+
+template <int BLOCK, int KLDS, int FORM>
+__global__ __launch_bounds__(BLOCK) void k_render_whitted(SceneView sv, ShadeView sh, WhittedParams wp,
+                                                          float4 *__restrict__ rgba,
+                                                          rt_hit24 *__restrict__ hits,
+                                                          uint8_t *__restrict__ occ, int keep, int count,
+                                                          unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
+    const uint32_t tiles_x = (wp.width + TW - 1) / TW;
+    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    uint32_t ox_, oy_;
+    Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
+    const uint32_t px = tx * TW + ox_;
+    const uint32_t py = ty * TH + oy_;
+    const bool valid = px < wp.width && py < wp.height;
+    int32_t root_ref;
+    float root_box[6];
+    load_root(sv.hdr, root_ref, root_box);
+    uint32_t boxes = 0, ntris = 0, nshadow = 0, nbounce = 0;
+    if (valid) {
+        const uint32_t pix = py * wp.width + px;
+        float dx, dy, dz;
+        primary_ray(wp.cam, px, py, (float)wp.width, (float)wp.height, 0.0f, 0.0f, dx, dy, dz);
+        float ox = wp.cam.pos[0], oy = wp.cam.pos[1], oz = wp.cam.pos[2];
+        float fx = 1.0f, fy = 1.0f, fz = 1.0f;  // reflective factor (CSPrimaryRays.hlsl:52)
+        int32_t self = -1;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // g_uAccumulation
+        float res[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // g_uResultTexture
+        Pcg32 rng = pcg_seed(sample_seed(wp.seed, wp.frame, 0u), pix);
+        const bool shadows = (wp.switches & RT_WHITTED_SHADOWS) != 0;
+        const bool phong = (wp.switches & RT_WHITTED_PHONG) != 0;
+        const float gloss = (wp.switches & RT_WHITTED_GLOSS) ? 1.0f : 0.0f;
+        for (uint32_t pass = 0; pass <= wp.bounces; ++pass) {
+            const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz,
+                                                          dx, dy, dz, 10000000000000000.0f, self, stk);
+            boxes += h.boxes;
+            ntris += h.tris;
+            nbounce += pass > 0 ? 1u : 0u;
+            if (keep && pass == 0) {
+                store_hit(&hits[pix], h);
+                occ[pix] = 2;
+            }
+            // A miss under environment-mapping flag 0 paints the pixel white
+            // and leaves the ray as it is, so every later pass repeats it
+            // (:30-35).
+            if (h.tri < wp.env_flag) {
+                res[0] = res[1] = res[2] = res[3] = 1.0f;
+                break;
+            }
+            if (h.tri < 0) {  // environment colour, ray ended (:238-248)
+                acc[0] = acc[0] + fx * wp.env[0];
+                acc[1] = acc[1] + fy * wp.env[1];
+                acc[2] = acc[2] + fz * wp.env[2];
+                acc[3] = acc[3] + 1.0f;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) res[c] = acc[c];
+                break;
+            }
+            const uint32_t mid = sh.mat ? sh.mat[h.tri] : 0u;
+            const rt_material M = sh.mats ? sh.mats[mid] : wp.defmat;
+            // shading normal: interpolated vertex normals or the face normal (:56-77)
+            float nx, ny, nz;
+            if (phong) {
+                const float *na = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri];
+                const float *nb = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri + 1];
+                const float *nc = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri + 2];
+                const float w = 1.0f - (h.u + h.v);  // Vec3BaryCentric (Intersection.hlsl:98-108)
+                nx = (na[0] * w + nb[0] * h.u) + nc[0] * h.v;
+                ny = (na[1] * w + nb[1] * h.u) + nc[1] * h.v;
+                nz = (na[2] * w + nb[2] * h.u) + nc[2] * h.v;
+                const float nl = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+                nx = nx / nl;
+                ny = ny / nl;
+                nz = nz / nl;
+            } else {
+                flat_normal(sv.tris[h.pos], nx, ny, nz);
+            }
+            // facing the ray (:124-131)
+            float mx = nx, my = ny, mz = nz;
+            const bool inverse = dot3(mx, my, mz, dx, dy, dz) > 0.0f;
+            if (inverse) {
+                mx = -mx;
+                my = -my;
+                mz = -mz;
+            }
+            const float hx = ox + h.t * dx, hy = oy + h.t * dy, hz = oz + h.t * dz;
+            float lx = hx - wp.light[0], ly = hy - wp.light[1], lz = hz - wp.light[2];
+            const float dist = sqrtf(dot3(lx, ly, lz, lx, ly, lz));
+            lx = lx / dist;
+            ly = ly / dist;
+            lz = lz / dist;
+            // normalize(reflect(d, n')), reflect(i, n) = i - 2 dot(n, i) n (:141-142)
+            const float k2 = 2.0f * dot3(mx, my, mz, dx, dy, dz);
+            float rx = dx - k2 * mx, ry = dy - k2 * my, rz = dz - k2 * mz;
+            const float rl = sqrtf(dot3(rx, ry, rz, rx, ry, rz));
+            rx = rx / rl;
+            ry = ry / rl;
+            rz = rz / rl;
+            // light and viewer on the same side of the surface: shadow ray to
+            // the light, then diffuse and specular terms (:145-176)
+            bool lit = false;
+            float D = 0.0f, S = 0.0f;
+            if ((dot3(nx, ny, nz, lx, ly, lz) * dot3(nx, ny, nz, dx, dy, dz)) > 0.0f) {
+                bool unoccluded = true;
+                if (shadows) {
+                    const TraceOut s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy,
+                                                                 hz, -lx, -ly, -lz, dist, h.tri, stk);
+                    boxes += s.boxes;
+                    ntris += s.tris;
+                    nshadow += 1;
+                    unoccluded = s.tri < 0;
+                    if (keep && pass == 0) occ[pix] = unoccluded ? 0 : 1;
+                }
+                if (unoccluded) {
+                    lit = true;
+                    const float sp = -dot3(lx, ly, lz, rx, ry, rz);
+                    S = pow49(sp > 0.0f ? sp : 0.0f);
+                    const float dd = -dot3(mx, my, mz, lx, ly, lz);
+                    D = dd > 0.0f ? dd : 0.0f;
+                }
+            }
+            // colour (:181-198): (ambient + D * diffuse.w) * diffuse
+            // + S * (1 - diffuse.w) * specular, times (factor, 1)
+            const float tw = M.diffuse[3];
+            const float f4[4] = {fx, fy, fz, 1.0f};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float Dc = lit ? D : (c == 3 ? 1.0f : 0.0f);
+                const float Sc = lit ? S : (c == 3 ? 1.0f : 0.0f);
+                const float spc = c == 3 ? M.specular[3] * gloss : M.specular[c];
+                const float dcol = (M.ambient[c] + Dc * tw) * M.diffuse[c];
+                const float scol = (Sc * (1.0f - tw)) * spc;
+                acc[c] = acc[c] + (dcol + scol) * f4[c];
+                res[c] = acc[c];
+            }
+            // continuation ray (:200-235): reflect with probability alpha,
+            // else refract(d, n', eta) with eta = ambient.w entering, its
+            // inverse leaving; total internal reflection reflects
+            const float u = pcg_unit(rng);
+            if (M.alpha >= u) {
+                fx = fx * M.specular[0];
+                fy = fy * M.specular[1];
+                fz = fz * M.specular[2];
+                dx = rx;
+                dy = ry;
+                dz = rz;
+            } else {
+                float eta = M.ambient[3];
+                if (!inverse) eta = 1.0f / eta;
+                // refract(i, n, eta): cosi = dot(-i, n), cost2 = 1 - eta^2 (1 - cosi^2),
+                // t = (eta i + (eta cosi - sqrt|cost2|) n) * (cost2 > 0)
+                const float cosi = -dot3(dx, dy, dz, mx, my, mz);
+                const float cost2 = 1.0f - (eta * eta) * (1.0f - cosi * cosi);
+                const float g = eta * cosi - sqrtf(fabsf(cost2));
+                const float keepr = cost2 > 0.0f ? 1.0f : 0.0f;
+                const float tx_ = (eta * dx + g * mx) * keepr;
+                const float ty_ = (eta * dy + g * my) * keepr;
+                const float tz_ = (eta * dz + g * mz) * keepr;
+                const float tl = sqrtf(dot3(tx_, ty_, tz_, tx_, ty_, tz_));
+                if (tl < 0.000001f) {
+                    dx = rx;
+                    dy = ry;
+                    dz = rz;
+                } else {
+                    dx = tx_ / tl;
+                    dy = ty_ / tl;
+                    dz = tz_ / tl;
+                    fx = fx * M.diffuse[0];
+                    fy = fy * M.diffuse[1];
+                    fz = fz * M.diffuse[2];
+                }
+            }
+            ox = hx;
+            oy = hy;
+            oz = hz;
+            self = h.tri;
+        }
+        rgba[pix] = make_float4(res[0], res[1], res[2], res[3]);
+    }
+    if (count) {
+        count_add(stats, 0, boxes);
+        count_add(stats, 1, ntris);
+    }
+    count_add(stats, 2, nshadow);
+    count_add(stats, 3, nbounce);
+}
+
 struct RenderJob {
     const TravNode *nodes;
     const TravTri *tris;
@@ -887,6 +1113,53 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
         else if (keep) hipLaunchKernelGGL((k_render_direct<true, false, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
         else if (cnt) hipLaunchKernelGGL((k_render_direct<false, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
         else hipLaunchKernelGGL((k_render_direct<false, false, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+    }
+    RT_TUNED(RT_L)
+#undef RT_L
+    return hipGetLastError();
+}
+
+hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitted_desc &d,
+                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
+                                 const rt_tuning &tune, hipStream_t st)
+{
+    const uint32_t max_depth = b.desc[scene].depth;
+    const SceneDesc &sd = b.desc[scene];
+    WhittedParams wp;
+    memset(&wp, 0, sizeof(wp));
+    wp.cam = d.cam;
+    wp.width = d.width;
+    wp.height = d.height;
+    wp.bounces = d.bounces;
+    wp.switches = d.switches;
+    for (int k = 0; k < 3; ++k) {
+        wp.light[k] = d.light[k];
+        wp.env[k] = d.env[k];
+    }
+    wp.env_flag = d.env_flag;
+    wp.frame = d.frame;
+    wp.seed = d.seed;
+    // MaterialData() (Common/Scenes/MaterialData.cpp:17-24)
+    wp.defmat.diffuse[0] = wp.defmat.diffuse[1] = wp.defmat.diffuse[2] = 0.5f;
+    wp.defmat.diffuse[3] = 0.8f;
+    wp.defmat.ambient[3] = 1.0f;
+    wp.defmat.specular[0] = wp.defmat.specular[1] = wp.defmat.specular[2] = 0.8f;
+    wp.defmat.specular[3] = 1.0f;
+    ShadeView sh{nullptr, b.idx + 3 * (size_t)sd.in_prim_off, nullptr, nullptr};
+    if (scene < b.shading.size()) {
+        const Batch::Shading &S = b.shading[scene];
+        if (S.normals) sh.nrm = b.nrm + 3 * (size_t)sd.vert_off;
+        if (S.ids) sh.mat = b.mat + sd.in_prim_off;
+        sh.mats = S.mats;
+    }
+    const SceneView sv = view_of(b, scene);
+    const int keep = (flags & RT_FLAG_KEEP_HITS) ? 1 : 0, cnt = (flags & RT_FLAG_COUNT) ? 1 : 0;
+#define RT_L(B, K, F)                                                                               \
+    {                                                                                              \
+        constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                            \
+        const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);             \
+        hipLaunchKernelGGL((k_render_whitted<B, K, F>), dim3(tiles), dim3(B), 0, st, sv, sh, wp,   \
+                           fb.rgba, fb.hits, fb.occ, keep, cnt, stats);                            \
     }
     RT_TUNED(RT_L)
 #undef RT_L

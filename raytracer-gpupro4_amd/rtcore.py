@@ -32,6 +32,9 @@ FLAG_TIMING = 1 << 3
 FLAG_PERSISTENT = 1 << 4
 MODE_DIRECT = 0
 MODE_GI = 1
+WHITTED_SHADOWS = 1 << 0
+WHITTED_PHONG = 1 << 1
+WHITTED_GLOSS = 1 << 2
 
 # Byte layouts of RayTracerCS/Shaders/Structures.hlsl (Node 32 B, Ray 48 B,
 # Intersection 24 B) as numpy dtypes.
@@ -40,7 +43,11 @@ RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("refl", "<f4", 3), ("tm
                       ("tmax", "<f4"), ("tri", "<i4")])
 HIT_DTYPE = np.dtype([("tri", "<i4"), ("u", "<f4"), ("v", "<f4"), ("t", "<f4"), ("visited", "<i4"),
                       ("root", "<i4")])
+# Material (Structures.hlsl:69-78, 64 B).
+MATERIAL_DTYPE = np.dtype([("diffuse", "<f4", 4), ("ambient", "<f4", 4), ("specular", "<f4", 4),
+                           ("shininess", "<f4"), ("alpha", "<f4"), ("id", "<u4"), ("pad", "<u4")])
 assert NODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 48 and HIT_DTYPE.itemsize == 24
+assert MATERIAL_DTYPE.itemsize == 64
 
 
 class BuildTimes(C.Structure):
@@ -77,6 +84,17 @@ class RenderDesc(C.Structure):
     _fields_ = [("cam", Camera), ("width", C.c_uint32), ("height", C.c_uint32),
                 ("lights", C.POINTER(C.c_float)), ("nlights", C.c_uint32), ("ambient", C.c_float),
                 ("albedo", C.c_float), ("env", C.c_float)]
+
+
+class WhittedDesc(C.Structure):
+    _fields_ = [("cam", Camera), ("width", C.c_uint32), ("height", C.c_uint32), ("light", C.c_float * 3),
+                ("bounces", C.c_uint32), ("switches", C.c_uint32), ("env_flag", C.c_int32),
+                ("env", C.c_float * 3), ("frame", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class Model(C.Structure):
+    _fields_ = [("xyz", C.c_void_p), ("normals", C.c_void_p), ("idx", C.c_void_p), ("mat_ids", C.c_void_p),
+                ("materials", C.c_void_p), ("nv", C.c_uint32), ("nt", C.c_uint32), ("nmat", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -174,6 +192,9 @@ def lib() -> C.CDLL:
         "rt_trace_device": (I32, [P, I32, P, U32, P, U32, C.POINTER(FrameStats)]),
         "rt_render_direct": (I32, [P, I32, C.POINTER(RenderDesc), U32, C.POINTER(FrameStats)]),
         "rt_frame_read": (I32, [P, P, P, P]),
+        "rt_scene_shading": (I32, [P, I32, P, U32, P, U32, P, U32]),
+        "rt_render_whitted": (I32, [P, I32, C.POINTER(WhittedDesc), U32, C.POINTER(FrameStats)]),
+        "rt_load_model_ex": (I32, [C.c_char_p, U32, C.POINTER(Model)]),
         "rt_profile_begin": (I32, [P]),
         "rt_profile_end": (I32, [P, C.POINTER(Profile)]),
         "rt_tuning_get": (I32, [P, C.POINTER(Tuning)]),
@@ -257,6 +278,26 @@ def load_model(path, raw=False):
     if rc != RT_OK:
         raise RtError(rc, L.rt_load_last_error().decode())
     return xyz, idx
+
+
+def load_model_shading(path, raw=False) -> dict:
+    """load_model plus shading attributes: dict with xyz, idx, normals (per
+    vertex), mat_ids (per triangle) and materials (MATERIAL_DTYPE)."""
+    L = lib()
+    flags = LOAD_RAW if raw else 0
+    m = Model()
+    rc = L.rt_load_model_ex(os.fsencode(path), flags, C.byref(m))
+    if rc != RT_OK:
+        raise RtError(rc, L.rt_load_last_error().decode())
+    out = dict(xyz=np.empty((m.nv, 3), np.float32), normals=np.empty((m.nv, 3), np.float32),
+               idx=np.empty((m.nt, 3), np.uint32), mat_ids=np.empty(m.nt, np.uint32),
+               materials=np.empty(m.nmat, MATERIAL_DTYPE))
+    m.xyz, m.normals, m.idx = out["xyz"].ctypes.data, out["normals"].ctypes.data, out["idx"].ctypes.data
+    m.mat_ids, m.materials = out["mat_ids"].ctypes.data, out["materials"].ctypes.data
+    rc = L.rt_load_model_ex(os.fsencode(path), flags, C.byref(m))
+    if rc != RT_OK:
+        raise RtError(rc, L.rt_load_last_error().decode())
+    return out
 
 
 def gen_torus(R=1.0, r=0.3, nu=128, nv=64, jitter=1e-3, seed=1):
@@ -529,6 +570,29 @@ class Context:
         st = FrameStats()
         self.check(self.L.rt_render_direct(self.h, scene, C.byref(d), flags,
                                            C.byref(st) if want_stats else None), "rt_render_direct")
+        return st.as_dict() if want_stats else None
+
+    def scene_shading(self, scene: int, normals=None, mat_ids=None, materials=None):
+        nt, nv, _ = self.scene_info(scene)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32)
+        mid = None if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32)
+        mats = None if materials is None else np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        self.check(self.L.rt_scene_shading(self.h, scene, _ptr(nrm), nv, _ptr(mid), nt, _ptr(mats),
+                                           0 if mats is None else len(mats)), "rt_scene_shading")
+
+    def render_whitted(self, scene: int, cam: Camera, width: int, height: int, light, bounces=2,
+                       switches=WHITTED_SHADOWS, env_flag=-1, env=(0.0, 0.0, 0.0), seed=1, frame=0,
+                       flags: int = 0, want_stats: bool = True):
+        d = WhittedDesc()
+        d.cam = cam
+        d.width, d.height = width, height
+        d.light[:] = [float(x) for x in light]
+        d.bounces, d.switches, d.env_flag = bounces, switches, env_flag
+        d.env[:] = [float(x) for x in env]
+        d.frame, d.seed = frame, seed
+        st = FrameStats()
+        self.check(self.L.rt_render_whitted(self.h, scene, C.byref(d), flags,
+                                            C.byref(st) if want_stats else None), "rt_render_whitted")
         return st.as_dict() if want_stats else None
 
     def frame_read(self, width: int, height: int, nlights: int, hits: bool = False):
