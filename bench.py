@@ -18,6 +18,10 @@ Other workloads (--workload), each printing one JSON line of the same shape:
            rebuild all 64 + render all 64, batched (one build, one persistent
            launch; also as a replayed graph) against per-scene builds and
            launches; images compared bit for bit.
+  whitted  the frame's 262,144-triangle soup (static) with a seeded mix of matte,
+           mirror, glass and half-mirror materials, 1920x1080 Whitted frames
+           of 3 bounces (primary, closest-hit continuation and shadow rays, the
+           reference's ray-tracing colour stage); value = Mrays/s.
   serve    Poisson 200 requests/s for 60 s over a pool of 500 scenes
            (1k..500k triangles), 512^2 or 1920x1080, direct or 2-bounce GI at
            1 spp with progressive accumulation, 200 GB device budget; p50/p99
@@ -274,6 +278,99 @@ def workload_sphere(args, rtcore):
                             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                             "kernel": "k_render_direct", "bytes_per_launch": alg / 8},
                   cpu_baseline=None)
+
+
+# ------------------------------------------------------------------ whitted
+WHITTED_BOUNCES = 3
+
+
+def whitted_materials(rtcore, nt, seed=17):
+    """Matte, mirror, glass (index 1.5), half mirror and the default
+    MaterialData, assigned to triangles uniformly at random."""
+    m = np.zeros(5, rtcore.MATERIAL_DTYPE)
+    m[0] = ((0.8, 0.6, 0.4, 1.0), (0.1, 0.1, 0.1, 1.0), (0.2, 0.2, 0.2, 1.0), 0.0, 1.0, 0, 0)
+    m[1] = ((0.3, 0.3, 0.3, 0.2), (0.0, 0.0, 0.0, 1.0), (0.9, 0.9, 0.95, 1.0), 0.5, 1.0, 1, 0)
+    m[2] = ((0.95, 0.97, 0.9, 0.6), (0.02, 0.02, 0.02, 1.5), (0.8, 0.8, 0.8, 0.7), 0.9, 0.0, 2, 0)
+    m[3] = ((0.2, 0.7, 0.3, 0.5), (0.05, 0.1, 0.05, 1.2), (0.6, 0.6, 0.6, 1.0), 0.3, 0.5, 3, 0)
+    m[4] = ((0.5, 0.5, 0.5, 0.8), (0.0, 0.0, 0.0, 1.0), (0.8, 0.8, 0.8, 1.0), 0.0, 0.0, 4, 0)
+    ids = np.random.default_rng(seed).integers(0, 5, nt).astype(np.uint32)
+    return m, ids
+
+
+def workload_whitted(args, rtcore):
+    xyz, idx = rtcore.gen_triangle_soup(NTRI, SIGMA, SEED)
+    mats, mat_ids = whitted_materials(rtcore, len(idx))
+    cam = rtcore.camera_look_at(CAM_POS, CAM_TGT, (0, 1, 0), FOV, W / H)
+    kw = dict(bounces=WHITTED_BOUNCES, switches=rtcore.WHITTED_SHADOWS | rtcore.WHITTED_GLOSS,
+              env=(0.4, 0.5, 0.6), seed=5)
+    ctx = rtcore.Context(0)
+    s = ctx.add_scene(xyz, idx, DEPTH)
+    ctx.build(s)
+    ctx.scene_shading(s, None, mat_ids, mats)
+    tuned = ctx.autotune_direct(s, cam, W, H, LIGHTS)
+    for f in range(args.warmup):
+        ctx.render_whitted(s, cam, W, H, LIGHTS[0], frame=f, want_stats=False, **kw)
+    device_sync(ctx)
+    ctx.profile_begin()
+    t0 = time.perf_counter()
+    for f in range(args.steps):
+        ctx.render_whitted(s, cam, W, H, LIGHTS[0], frame=f, want_stats=False, **kw)
+    device_sync(ctx)
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_end()
+    boxes = tris = counted = 0
+    for f in range(args.steps):
+        st = ctx.render_whitted(s, cam, W, H, LIGHTS[0], frame=f, flags=rtcore.FLAG_COUNT, **kw)
+        boxes += st["box_tests"]
+        tris += st["tri_tests"]
+        counted += st["primary_rays"] + st["shadow_rays"] + st["bounce_rays"]
+    # the other device form over the same frames (same images and rays)
+    alt = dict(kw, switches=kw["switches"] ^ rtcore.WHITTED_ONE_PASS)
+    ctx.profile_begin()
+    for f in range(args.steps):
+        ctx.render_whitted(s, cam, W, H, LIGHTS[0], frame=f, want_stats=False, **alt)
+    alt_prof = ctx.profile_end()
+    rays = prof["primary_rays"] + prof["shadow_rays"] + prof["bounce_rays"]
+    assert rays == counted, (rays, counted)
+    mrays = rays / (prof["trace_ms"] * 1e-3) / 1e6
+    per_launch_ms = prof["trace_ms"] / prof["launches"]
+    alg = (RAY_BYTES * rays + BOX_BYTES * boxes + TRI_BYTES * tris) / prof["launches"]
+    achieved = alg / (per_launch_ms * 1e-3) / 1e9
+    ctx.close()
+    form = "one-pass" if kw["switches"] & rtcore.WHITTED_ONE_PASS else "wavefront"
+    other = "wavefront" if form == "one-pass" else "one-pass"
+    out = result(METRIC, mrays, "Mrays/s", args.steps, args.warmup, wall,
+                 {"workload": f"{NTRI}-triangle soup (static), 5 materials (matte, mirror, glass 1.5, "
+                              f"half mirror, default), {W}x{H} Whitted frames, {WHITTED_BOUNCES} bounces, "
+                              f"shadow rays to 1 point light",
+                  "tris": NTRI, "depth": DEPTH, "width": W, "height": H, "bounces": WHITTED_BOUNCES},
+                 form=form, trace_ms_per_frame=round(per_launch_ms, 4),
+                 other_form={"form": other, "trace_ms_per_frame": round(alt_prof["trace_ms"] / alt_prof["launches"], 4),
+                             "mrays_s": round(rays / (alt_prof["trace_ms"] * 1e-3) / 1e6, 2)},
+                 rays_per_frame=rays / args.steps,
+                 shadow_rays_per_frame=prof["shadow_rays"] / args.steps,
+                 bounce_rays_per_frame=prof["bounce_rays"] / args.steps,
+                 tests_per_ray={"box": boxes / rays, "tri": tris / rays}, tuning=list(tuned),
+                 roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": "k_render_whitted", "bytes_per_launch": alg},
+                 cpu_baseline=None)
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        cores = min(CPU_SHARE, len(os.sched_getaffinity(0)))
+        ref = oracle.build_scene(xyz, idx, DEPTH)
+        sw, sh = W // 4, H // 4  # the same view at a quarter of the resolution per axis
+        scam = rtcore.camera_look_at(CAM_POS, CAM_TGT, (0, 1, 0), FOV, sw / sh)
+        t0 = time.perf_counter()
+        _, _, _, n = oracle.render_whitted(ref["nodes"], ref["ids"], xyz, idx, scam.as_array(), sw, sh, LIGHTS[0],
+                                           mat_ids=mat_ids, materials=mats, nthreads=cores, **kw)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+                               "sample": f"one {sw}x{sh} Whitted frame of the same scene, view and materials: "
+                                         f"{n} rays in {dt:.1f} s on {cores} threads"}
+    return out
 
 
 # ------------------------------------------------------------------ batched
@@ -540,7 +637,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["frame", "sphere", "batched", "serve"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "sphere", "whitted", "batched", "serve"], default="frame")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--plain-seconds", type=float, default=10.0,
                     help="serve: seconds of the same arrivals replayed one request per launch")
@@ -555,7 +652,7 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)
         sys.exit(2)
-    defaults = {"frame": (100, 5), "sphere": (10, 1), "batched": (3, 1), "serve": (0, 0)}
+    defaults = {"frame": (100, 5), "sphere": (10, 1), "whitted": (20, 2), "batched": (3, 1), "serve": (0, 0)}
     ds, dw = defaults[args.workload]
     args.steps = ds if args.steps is None else args.steps
     args.warmup = dw if args.warmup is None else args.warmup
@@ -567,8 +664,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     rtcore.lib()
-    fn = {"frame": workload_frame, "sphere": workload_sphere, "batched": workload_batched,
-          "serve": workload_serve}[args.workload]
+    fn = {"frame": workload_frame, "sphere": workload_sphere, "whitted": workload_whitted,
+          "batched": workload_batched, "serve": workload_serve}[args.workload]
     print(json.dumps(fn(args, rtcore)), flush=True)
 
 

@@ -274,7 +274,10 @@ int rt_scene_shading(rt_ctx *ctx, int scene, const float *normals, uint32_t nv,
 enum {
     RT_WHITTED_SHADOWS = 1u << 0, /* g_bIsShadowOn */
     RT_WHITTED_PHONG = 1u << 1,   /* g_bIsPhongShadingOn */
-    RT_WHITTED_GLOSS = 1u << 2    /* g_bIsGlossMappingOn (specular.w) */
+    RT_WHITTED_GLOSS = 1u << 2,   /* g_bIsGlossMappingOn (specular.w) */
+    RT_WHITTED_ONE_PASS = 1u << 3 /* one launch, each lane runs its pixel's passes; by default
+                                     pass 0 runs per pixel and each bounce pass over a compacted
+                                     queue of the live rays (same results) */
 };
 typedef struct rt_whitted_desc {
     rt_camera cam;

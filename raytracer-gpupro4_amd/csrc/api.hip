@@ -295,11 +295,11 @@ int rt_create(rt_ctx **out, int device)
     for (auto &e : ctx->ev) hipEventCreate(&e);
     hipEventCreateWithFlags(&ctx->jobs_ev, hipEventDisableTiming);
     if (hipMalloc(&ctx->stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&ctx->tile_counter, 256) != hipSuccess) {
+        hipMalloc(&ctx->tile_counter, 256) != hipSuccess || hipMalloc(&ctx->wcount, 128) != hipSuccess) {
         rt_destroy(ctx);
         return RT_ERR_NOMEM;
     }
-    ctx->bytes += 8 * sizeof(unsigned long long) + 256;
+    ctx->bytes += 8 * sizeof(unsigned long long) + 256 + 128;
     *out = ctx;
     return RT_OK;
 }
@@ -325,6 +325,8 @@ void rt_destroy(rt_ctx *ctx)
     hipFree(ctx->stats);
     hipFree(ctx->staging);
     hipFree(ctx->tile_counter);
+    hipFree(ctx->wcount);
+    hipFree(ctx->wqueue);
     hipFree(ctx->jobs_dev);
     if (ctx->jobs_ev) hipEventDestroy(ctx->jobs_ev);
     for (auto &e : ctx->ev)
@@ -739,7 +741,7 @@ int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint3
     if (desc->bounces > 16) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: %u bounces (0..16)", desc->bounces);
     if (desc->env_flag != -1 && desc->env_flag != 0)
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: env_flag %d (-1 or 0)", desc->env_flag);
-    if (desc->switches & ~(uint32_t)(RT_WHITTED_SHADOWS | RT_WHITTED_PHONG | RT_WHITTED_GLOSS))
+    if (desc->switches & ~(uint32_t)(RT_WHITTED_SHADOWS | RT_WHITTED_PHONG | RT_WHITTED_GLOSS | RT_WHITTED_ONE_PASS))
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: unknown switches 0x%x", desc->switches);
     const SceneDesc &d = b->desc[i];
     const bool have = i < b->shading.size();
@@ -752,13 +754,26 @@ int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint3
     int rc = ensure_frame(ctx, desc->width, desc->height, 1);
     if (rc) return rc;
     hipStream_t st = ctx->stream;
+    // wavefront form: two ray queues of one 64-byte record per pixel
+    const uint64_t qbytes = rt::whitted_queue_bytes((uint64_t)desc->width * desc->height);
+    if (!(desc->switches & RT_WHITTED_ONE_PASS) && desc->bounces > 0 && ctx->wqueue_bytes < qbytes) {
+        hipStreamSynchronize(st);
+        hipFree(ctx->wqueue);
+        ctx->bytes -= ctx->wqueue_bytes;
+        ctx->wqueue = nullptr;
+        ctx->wqueue_bytes = 0;
+        RT_HIP(ctx, hipMalloc(&ctx->wqueue, qbytes));
+        ctx->wqueue_bytes = qbytes;
+        ctx->bytes += qbytes;
+    }
     const bool timing = (flags & RT_FLAG_TIMING) != 0 || stats != nullptr;
     if (stats || !ctx->profiling)
         RT_HIP(ctx, hipMemsetAsync(ctx->stats, 0, 8 * sizeof(unsigned long long), st));
     hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
     if (!stats && ctx->profiling) ev = prof_events(ctx, 1, 2);
     if (ev) hipEventRecord(ev[0], st);
-    RT_HIP(ctx, rt::launch_render_whitted(*b, i, *desc, ctx->frame, flags, ctx->stats, ctx->tune, st));
+    RT_HIP(ctx, rt::launch_render_whitted(*b, i, *desc, ctx->frame, flags, ctx->stats, ctx->tune, ctx->wqueue,
+                                          ctx->wcount, ctx->cu_count, st));
     if (ev) hipEventRecord(ev[1], st);
     if (ctx->profiling) ctx->prof_primary += (uint64_t)desc->width * desc->height;
     if (stats) {

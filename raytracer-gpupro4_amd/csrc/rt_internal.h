@@ -180,6 +180,9 @@ struct rt_ctx {
     int cu_count = 256;
     rt_tuning tune{128, 26, 8, 0};
     uint32_t *tile_counter = nullptr;     // persistent job launches
+    void *wqueue = nullptr;               // Whitted wavefront ray queues
+    uint64_t wqueue_bytes = 0;
+    uint32_t *wcount = nullptr;           // Whitted queue lengths, one per pass (17)
     void *jobs_dev = nullptr;             // render-job table
     uint64_t jobs_cap = 0;
     hipEvent_t jobs_ev = nullptr;         // last launch that read jobs_dev
@@ -217,9 +220,13 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
 hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st);
+// queue: whitted_queue_bytes(width * height) of device memory (two ray
+// queues); qcount: bounces + 1 words.  Unused by the one-pass form.
 hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitted_desc &d,
                                  FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
-                                 const rt_tuning &tune, hipStream_t st);
+                                 const rt_tuning &tune, void *queue, uint32_t *qcount, int cu_count,
+                                 hipStream_t st);
+size_t whitted_queue_bytes(uint64_t pixels);
 size_t render_job_bytes();
 void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_desc &d,
                      const rt_tuning &t, uint32_t tile_off, uint32_t *tiles);

@@ -654,8 +654,13 @@ __device__ __forceinline__ void cosine_dir(float nx, float ny, float nz, float r
 // ---------------------------------------------------------------------------
 // Whitted frames: the reference's ray-tracing mode (RayTracerCS.cpp:286-295)
 // runs CSIntersections and CSComputeColor (CSComputeColorRT.hlsl:18-262) once
-// per pass over a ray buffer in memory.  Here one lane carries its pixel
-// through every pass, so rays, hits and the accumulator stay in registers.
+// per pass over a ray buffer of every pixel.  Two forms here, same results:
+//   - one pass (k_render_whitted): one lane carries its pixel through every
+//     pass, rays and the accumulator in registers;
+//   - wavefront (k_whitted_first + k_whitted_next per bounce): pass 0 per
+//     pixel, then each bounce pass over a compacted queue of the rays still
+//     alive, so waves stay full when most paths have ended.  The accumulator
+//     lives in the frame buffer; each queued ray is one 64-byte record.
 // ---------------------------------------------------------------------------
 struct ShadeView {
     const float *nrm;         // the scene's vertex normals (null: flat shading only)
@@ -675,6 +680,67 @@ struct WhittedParams {
     rt_material defmat;
 };
 
+// A live path between passes: origin, direction, reflective factor, the id
+// of the triangle it leaves (self), its pixel and its random stream.
+struct WRay {
+    float ox, oy, oz, dx, dy, dz, fx, fy, fz;
+    int32_t self;
+    uint32_t pix;
+    Pcg32 rng;
+};
+
+// Queue record of a WRay (64 bytes; the stream increment is 2 pix + 1).
+struct alignas(16) RayRec {
+    float4 a;  // ox oy oz dx
+    float4 b;  // dy dz fx fy
+    float4 c;  // fz, self, pix, unused
+    uint4 d;   // rng state lo, hi, unused
+};
+
+__device__ __forceinline__ void put_ray(RayRec *q, uint32_t i, const WRay &r)
+{
+    RayRec e;
+    e.a = make_float4(r.ox, r.oy, r.oz, r.dx);
+    e.b = make_float4(r.dy, r.dz, r.fx, r.fy);
+    e.c = make_float4(r.fz, __int_as_float(r.self), __uint_as_float(r.pix), 0.0f);
+    e.d = make_uint4((uint32_t)r.rng.state, (uint32_t)(r.rng.state >> 32), 0u, 0u);
+    q[i] = e;
+}
+
+__device__ __forceinline__ WRay get_ray(const RayRec *q, uint32_t i)
+{
+    const RayRec e = q[i];
+    WRay r;
+    r.ox = e.a.x;
+    r.oy = e.a.y;
+    r.oz = e.a.z;
+    r.dx = e.a.w;
+    r.dy = e.b.x;
+    r.dz = e.b.y;
+    r.fx = e.b.z;
+    r.fy = e.b.w;
+    r.fz = e.c.x;
+    r.self = __float_as_int(e.c.y);
+    r.pix = __float_as_uint(e.c.z);
+    r.rng.state = (uint64_t)e.d.x | ((uint64_t)e.d.y << 32);
+    r.rng.inc = ((uint64_t)r.pix << 1u) | 1u;
+    return r;
+}
+
+// Slot for this lane in an append-only queue: one atomic per wave.  Every
+// lane of the wave must call it; lanes with want == false get no slot.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
+{
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return 0u;
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
 {
     return (ax * bx + ay * by) + az * bz;
@@ -687,6 +753,177 @@ __device__ __forceinline__ float pow49(float x)
     return (x32 * x16) * x;
 }
 
+struct WCount {
+    uint32_t boxes = 0, ntris = 0, nshadow = 0;
+};
+
+// One intersection + colour pass of ray r.  Adds the pass's colour into acc
+// (or sets white), and returns true when the path continues, with r turned
+// into the continuation ray.  first: store the hit and shadow result.
+template <int FORM, class Stack>
+__device__ __forceinline__ bool whitted_pass(const SceneView &sv, const ShadeView &sh, const WhittedParams &wp,
+                                             int32_t root_ref, const float *root_box, WRay &r, float *acc,
+                                             bool &white, rt_hit24 *hit_out, uint8_t *occ_out, Stack &stk,
+                                             WCount &cnt)
+{
+    const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, r.ox, r.oy, r.oz, r.dx,
+                                                  r.dy, r.dz, 10000000000000000.0f, r.self, stk);
+    cnt.boxes += h.boxes;
+    cnt.ntris += h.tris;
+    if (hit_out) {
+        store_hit(hit_out, h);
+        *occ_out = 2;
+    }
+    // A miss under environment-mapping flag 0 paints the pixel white and
+    // leaves the ray as it is, so every later pass repeats it (:30-35).
+    if (h.tri < wp.env_flag) {
+        white = true;
+        return false;
+    }
+    if (h.tri < 0) {  // environment colour, ray ended (:238-248)
+        acc[0] = acc[0] + r.fx * wp.env[0];
+        acc[1] = acc[1] + r.fy * wp.env[1];
+        acc[2] = acc[2] + r.fz * wp.env[2];
+        acc[3] = acc[3] + 1.0f;
+        return false;
+    }
+    const uint32_t mid = sh.mat ? sh.mat[h.tri] : 0u;
+    const rt_material M = sh.mats ? sh.mats[mid] : wp.defmat;
+    // shading normal: interpolated vertex normals or the face normal (:56-77)
+    float nx, ny, nz;
+    if (wp.switches & RT_WHITTED_PHONG) {
+        const float *na = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri];
+        const float *nb = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri + 1];
+        const float *nc = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri + 2];
+        const float w = 1.0f - (h.u + h.v);  // Vec3BaryCentric (Intersection.hlsl:98-108)
+        nx = (na[0] * w + nb[0] * h.u) + nc[0] * h.v;
+        ny = (na[1] * w + nb[1] * h.u) + nc[1] * h.v;
+        nz = (na[2] * w + nb[2] * h.u) + nc[2] * h.v;
+        const float nl = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+        nx = nx / nl;
+        ny = ny / nl;
+        nz = nz / nl;
+    } else {
+        flat_normal(sv.tris[h.pos], nx, ny, nz);
+    }
+    // facing the ray (:124-131)
+    float mx = nx, my = ny, mz = nz;
+    const bool inverse = dot3(mx, my, mz, r.dx, r.dy, r.dz) > 0.0f;
+    if (inverse) {
+        mx = -mx;
+        my = -my;
+        mz = -mz;
+    }
+    const float hx = r.ox + h.t * r.dx, hy = r.oy + h.t * r.dy, hz = r.oz + h.t * r.dz;
+    float lx = hx - wp.light[0], ly = hy - wp.light[1], lz = hz - wp.light[2];
+    const float dist = sqrtf(dot3(lx, ly, lz, lx, ly, lz));
+    lx = lx / dist;
+    ly = ly / dist;
+    lz = lz / dist;
+    // normalize(reflect(d, n')), reflect(i, n) = i - 2 dot(n, i) n (:141-142)
+    const float k2 = 2.0f * dot3(mx, my, mz, r.dx, r.dy, r.dz);
+    float rx = r.dx - k2 * mx, ry = r.dy - k2 * my, rz = r.dz - k2 * mz;
+    const float rl = sqrtf(dot3(rx, ry, rz, rx, ry, rz));
+    rx = rx / rl;
+    ry = ry / rl;
+    rz = rz / rl;
+    // light and viewer on the same side of the surface: shadow ray to the
+    // light, then diffuse and specular terms (:145-176)
+    bool lit = false;
+    float D = 0.0f, S = 0.0f;
+    if ((dot3(nx, ny, nz, lx, ly, lz) * dot3(nx, ny, nz, r.dx, r.dy, r.dz)) > 0.0f) {
+        bool unoccluded = true;
+        if (wp.switches & RT_WHITTED_SHADOWS) {
+            const TraceOut s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz, -lx,
+                                                         -ly, -lz, dist, h.tri, stk);
+            cnt.boxes += s.boxes;
+            cnt.ntris += s.tris;
+            cnt.nshadow += 1;
+            unoccluded = s.tri < 0;
+            if (occ_out) *occ_out = unoccluded ? 0 : 1;
+        }
+        if (unoccluded) {
+            lit = true;
+            const float sp = -dot3(lx, ly, lz, rx, ry, rz);
+            S = pow49(sp > 0.0f ? sp : 0.0f);
+            const float dd = -dot3(mx, my, mz, lx, ly, lz);
+            D = dd > 0.0f ? dd : 0.0f;
+        }
+    }
+    // colour (:181-198): (ambient + D * diffuse.w) * diffuse
+    // + S * (1 - diffuse.w) * specular, times (factor, 1)
+    const float tw = M.diffuse[3];
+    const float gloss = (wp.switches & RT_WHITTED_GLOSS) ? 1.0f : 0.0f;
+    const float f4[4] = {r.fx, r.fy, r.fz, 1.0f};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float Dc = lit ? D : (c == 3 ? 1.0f : 0.0f);
+        const float Sc = lit ? S : (c == 3 ? 1.0f : 0.0f);
+        const float spc = c == 3 ? M.specular[3] * gloss : M.specular[c];
+        const float dcol = (M.ambient[c] + Dc * tw) * M.diffuse[c];
+        const float scol = (Sc * (1.0f - tw)) * spc;
+        acc[c] = acc[c] + (dcol + scol) * f4[c];
+    }
+    // continuation ray (:200-235): reflect with probability alpha, else
+    // refract(d, n', eta) with eta = ambient.w entering, its inverse leaving;
+    // total internal reflection reflects
+    const float u = pcg_unit(r.rng);
+    if (M.alpha >= u) {
+        r.fx = r.fx * M.specular[0];
+        r.fy = r.fy * M.specular[1];
+        r.fz = r.fz * M.specular[2];
+        r.dx = rx;
+        r.dy = ry;
+        r.dz = rz;
+    } else {
+        float eta = M.ambient[3];
+        if (!inverse) eta = 1.0f / eta;
+        // refract(i, n, eta): cosi = dot(-i, n), cost2 = 1 - eta^2 (1 - cosi^2),
+        // t = (eta i + (eta cosi - sqrt|cost2|) n) * (cost2 > 0)
+        const float cosi = -dot3(r.dx, r.dy, r.dz, mx, my, mz);
+        const float cost2 = 1.0f - (eta * eta) * (1.0f - cosi * cosi);
+        const float g = eta * cosi - sqrtf(fabsf(cost2));
+        const float keepr = cost2 > 0.0f ? 1.0f : 0.0f;
+        const float tx_ = (eta * r.dx + g * mx) * keepr;
+        const float ty_ = (eta * r.dy + g * my) * keepr;
+        const float tz_ = (eta * r.dz + g * mz) * keepr;
+        const float tl = sqrtf(dot3(tx_, ty_, tz_, tx_, ty_, tz_));
+        if (tl < 0.000001f) {
+            r.dx = rx;
+            r.dy = ry;
+            r.dz = rz;
+        } else {
+            r.dx = tx_ / tl;
+            r.dy = ty_ / tl;
+            r.dz = tz_ / tl;
+            r.fx = r.fx * M.diffuse[0];
+            r.fy = r.fy * M.diffuse[1];
+            r.fz = r.fz * M.diffuse[2];
+        }
+    }
+    r.ox = hx;
+    r.oy = hy;
+    r.oz = hz;
+    r.self = h.tri;
+    return true;
+}
+
+// The pixel's primary ray (CSPrimaryRays.hlsl:35-53, no jitter) and stream.
+__device__ __forceinline__ WRay whitted_primary(const WhittedParams &wp, uint32_t px, uint32_t py)
+{
+    WRay r;
+    r.pix = py * wp.width + px;
+    primary_ray(wp.cam, px, py, (float)wp.width, (float)wp.height, 0.0f, 0.0f, r.dx, r.dy, r.dz);
+    r.ox = wp.cam.pos[0];
+    r.oy = wp.cam.pos[1];
+    r.oz = wp.cam.pos[2];
+    r.fx = r.fy = r.fz = 1.0f;  // reflective factor (:52)
+    r.self = -1;
+    r.rng = pcg_seed(sample_seed(wp.seed, wp.frame, 0u), r.pix);
+    return r;
+}
+
+// One-pass form: every pass of a pixel in one lane.
 template <int BLOCK, int KLDS, int FORM>
 __global__ __launch_bounds__(BLOCK) void k_render_whitted(SceneView sv, ShadeView sh, WhittedParams wp,
                                                           float4 *__restrict__ rgba,
@@ -704,177 +941,120 @@ __global__ __launch_bounds__(BLOCK) void k_render_whitted(SceneView sv, ShadeVie
     Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
     const uint32_t px = tx * TW + ox_;
     const uint32_t py = ty * TH + oy_;
-    const bool valid = px < wp.width && py < wp.height;
     int32_t root_ref;
     float root_box[6];
     load_root(sv.hdr, root_ref, root_box);
-    uint32_t boxes = 0, ntris = 0, nshadow = 0, nbounce = 0;
-    if (valid) {
-        const uint32_t pix = py * wp.width + px;
-        float dx, dy, dz;
-        primary_ray(wp.cam, px, py, (float)wp.width, (float)wp.height, 0.0f, 0.0f, dx, dy, dz);
-        float ox = wp.cam.pos[0], oy = wp.cam.pos[1], oz = wp.cam.pos[2];
-        float fx = 1.0f, fy = 1.0f, fz = 1.0f;  // reflective factor (CSPrimaryRays.hlsl:52)
-        int32_t self = -1;
+    WCount cnt;
+    uint32_t nbounce = 0;
+    if (px < wp.width && py < wp.height) {
+        WRay r = whitted_primary(wp, px, py);
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // g_uAccumulation
-        float res[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // g_uResultTexture
-        Pcg32 rng = pcg_seed(sample_seed(wp.seed, wp.frame, 0u), pix);
-        const bool shadows = (wp.switches & RT_WHITTED_SHADOWS) != 0;
-        const bool phong = (wp.switches & RT_WHITTED_PHONG) != 0;
-        const float gloss = (wp.switches & RT_WHITTED_GLOSS) ? 1.0f : 0.0f;
+        bool white = false;
         for (uint32_t pass = 0; pass <= wp.bounces; ++pass) {
-            const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz,
-                                                          dx, dy, dz, 10000000000000000.0f, self, stk);
-            boxes += h.boxes;
-            ntris += h.tris;
             nbounce += pass > 0 ? 1u : 0u;
-            if (keep && pass == 0) {
-                store_hit(&hits[pix], h);
-                occ[pix] = 2;
-            }
-            // A miss under environment-mapping flag 0 paints the pixel white
-            // and leaves the ray as it is, so every later pass repeats it
-            // (:30-35).
-            if (h.tri < wp.env_flag) {
-                res[0] = res[1] = res[2] = res[3] = 1.0f;
+            const bool first = keep && pass == 0;
+            if (!whitted_pass<FORM>(sv, sh, wp, root_ref, root_box, r, acc, white, first ? &hits[r.pix] : nullptr,
+                                    first ? &occ[r.pix] : nullptr, stk, cnt))
                 break;
-            }
-            if (h.tri < 0) {  // environment colour, ray ended (:238-248)
-                acc[0] = acc[0] + fx * wp.env[0];
-                acc[1] = acc[1] + fy * wp.env[1];
-                acc[2] = acc[2] + fz * wp.env[2];
-                acc[3] = acc[3] + 1.0f;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) res[c] = acc[c];
-                break;
-            }
-            const uint32_t mid = sh.mat ? sh.mat[h.tri] : 0u;
-            const rt_material M = sh.mats ? sh.mats[mid] : wp.defmat;
-            // shading normal: interpolated vertex normals or the face normal (:56-77)
-            float nx, ny, nz;
-            if (phong) {
-                const float *na = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri];
-                const float *nb = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri + 1];
-                const float *nc = sh.nrm + 3 * (size_t)sh.idx[3 * (size_t)h.tri + 2];
-                const float w = 1.0f - (h.u + h.v);  // Vec3BaryCentric (Intersection.hlsl:98-108)
-                nx = (na[0] * w + nb[0] * h.u) + nc[0] * h.v;
-                ny = (na[1] * w + nb[1] * h.u) + nc[1] * h.v;
-                nz = (na[2] * w + nb[2] * h.u) + nc[2] * h.v;
-                const float nl = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
-                nx = nx / nl;
-                ny = ny / nl;
-                nz = nz / nl;
-            } else {
-                flat_normal(sv.tris[h.pos], nx, ny, nz);
-            }
-            // facing the ray (:124-131)
-            float mx = nx, my = ny, mz = nz;
-            const bool inverse = dot3(mx, my, mz, dx, dy, dz) > 0.0f;
-            if (inverse) {
-                mx = -mx;
-                my = -my;
-                mz = -mz;
-            }
-            const float hx = ox + h.t * dx, hy = oy + h.t * dy, hz = oz + h.t * dz;
-            float lx = hx - wp.light[0], ly = hy - wp.light[1], lz = hz - wp.light[2];
-            const float dist = sqrtf(dot3(lx, ly, lz, lx, ly, lz));
-            lx = lx / dist;
-            ly = ly / dist;
-            lz = lz / dist;
-            // normalize(reflect(d, n')), reflect(i, n) = i - 2 dot(n, i) n (:141-142)
-            const float k2 = 2.0f * dot3(mx, my, mz, dx, dy, dz);
-            float rx = dx - k2 * mx, ry = dy - k2 * my, rz = dz - k2 * mz;
-            const float rl = sqrtf(dot3(rx, ry, rz, rx, ry, rz));
-            rx = rx / rl;
-            ry = ry / rl;
-            rz = rz / rl;
-            // light and viewer on the same side of the surface: shadow ray to
-            // the light, then diffuse and specular terms (:145-176)
-            bool lit = false;
-            float D = 0.0f, S = 0.0f;
-            if ((dot3(nx, ny, nz, lx, ly, lz) * dot3(nx, ny, nz, dx, dy, dz)) > 0.0f) {
-                bool unoccluded = true;
-                if (shadows) {
-                    const TraceOut s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy,
-                                                                 hz, -lx, -ly, -lz, dist, h.tri, stk);
-                    boxes += s.boxes;
-                    ntris += s.tris;
-                    nshadow += 1;
-                    unoccluded = s.tri < 0;
-                    if (keep && pass == 0) occ[pix] = unoccluded ? 0 : 1;
-                }
-                if (unoccluded) {
-                    lit = true;
-                    const float sp = -dot3(lx, ly, lz, rx, ry, rz);
-                    S = pow49(sp > 0.0f ? sp : 0.0f);
-                    const float dd = -dot3(mx, my, mz, lx, ly, lz);
-                    D = dd > 0.0f ? dd : 0.0f;
-                }
-            }
-            // colour (:181-198): (ambient + D * diffuse.w) * diffuse
-            // + S * (1 - diffuse.w) * specular, times (factor, 1)
-            const float tw = M.diffuse[3];
-            const float f4[4] = {fx, fy, fz, 1.0f};
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const float Dc = lit ? D : (c == 3 ? 1.0f : 0.0f);
-                const float Sc = lit ? S : (c == 3 ? 1.0f : 0.0f);
-                const float spc = c == 3 ? M.specular[3] * gloss : M.specular[c];
-                const float dcol = (M.ambient[c] + Dc * tw) * M.diffuse[c];
-                const float scol = (Sc * (1.0f - tw)) * spc;
-                acc[c] = acc[c] + (dcol + scol) * f4[c];
-                res[c] = acc[c];
-            }
-            // continuation ray (:200-235): reflect with probability alpha,
-            // else refract(d, n', eta) with eta = ambient.w entering, its
-            // inverse leaving; total internal reflection reflects
-            const float u = pcg_unit(rng);
-            if (M.alpha >= u) {
-                fx = fx * M.specular[0];
-                fy = fy * M.specular[1];
-                fz = fz * M.specular[2];
-                dx = rx;
-                dy = ry;
-                dz = rz;
-            } else {
-                float eta = M.ambient[3];
-                if (!inverse) eta = 1.0f / eta;
-                // refract(i, n, eta): cosi = dot(-i, n), cost2 = 1 - eta^2 (1 - cosi^2),
-                // t = (eta i + (eta cosi - sqrt|cost2|) n) * (cost2 > 0)
-                const float cosi = -dot3(dx, dy, dz, mx, my, mz);
-                const float cost2 = 1.0f - (eta * eta) * (1.0f - cosi * cosi);
-                const float g = eta * cosi - sqrtf(fabsf(cost2));
-                const float keepr = cost2 > 0.0f ? 1.0f : 0.0f;
-                const float tx_ = (eta * dx + g * mx) * keepr;
-                const float ty_ = (eta * dy + g * my) * keepr;
-                const float tz_ = (eta * dz + g * mz) * keepr;
-                const float tl = sqrtf(dot3(tx_, ty_, tz_, tx_, ty_, tz_));
-                if (tl < 0.000001f) {
-                    dx = rx;
-                    dy = ry;
-                    dz = rz;
-                } else {
-                    dx = tx_ / tl;
-                    dy = ty_ / tl;
-                    dz = tz_ / tl;
-                    fx = fx * M.diffuse[0];
-                    fy = fy * M.diffuse[1];
-                    fz = fz * M.diffuse[2];
-                }
-            }
-            ox = hx;
-            oy = hy;
-            oz = hz;
-            self = h.tri;
         }
-        rgba[pix] = make_float4(res[0], res[1], res[2], res[3]);
+        rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : make_float4(acc[0], acc[1], acc[2], acc[3]);
     }
     if (count) {
-        count_add(stats, 0, boxes);
-        count_add(stats, 1, ntris);
+        count_add(stats, 0, cnt.boxes);
+        count_add(stats, 1, cnt.ntris);
     }
-    count_add(stats, 2, nshadow);
+    count_add(stats, 2, cnt.nshadow);
     count_add(stats, 3, nbounce);
+}
+
+// Wavefront form, pass 0: one lane per pixel; live paths go to queue q.
+template <int BLOCK, int KLDS, int FORM>
+__global__ __launch_bounds__(BLOCK) void k_whitted_first(SceneView sv, ShadeView sh, WhittedParams wp,
+                                                         float4 *__restrict__ rgba, rt_hit24 *__restrict__ hits,
+                                                         uint8_t *__restrict__ occ, int keep, int count,
+                                                         RayRec *__restrict__ q, uint32_t *qcount,
+                                                         unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
+    const uint32_t tiles_x = (wp.width + TW - 1) / TW;
+    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    uint32_t ox_, oy_;
+    Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
+    const uint32_t px = tx * TW + ox_;
+    const uint32_t py = ty * TH + oy_;
+    int32_t root_ref;
+    float root_box[6];
+    load_root(sv.hdr, root_ref, root_box);
+    WCount cnt;
+    WRay r;
+    bool live = false;
+    if (px < wp.width && py < wp.height) {
+        r = whitted_primary(wp, px, py);
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        bool white = false;
+        live = whitted_pass<FORM>(sv, sh, wp, root_ref, root_box, r, acc, white, keep ? &hits[r.pix] : nullptr,
+                                  keep ? &occ[r.pix] : nullptr, stk, cnt);
+        live = live && wp.bounces > 0;
+        rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : make_float4(acc[0], acc[1], acc[2], acc[3]);
+    }
+    const uint32_t slot = wave_append(qcount, live);
+    if (live) put_ray(q, slot, r);
+    if (count) {
+        count_add(stats, 0, cnt.boxes);
+        count_add(stats, 1, cnt.ntris);
+    }
+    count_add(stats, 2, cnt.nshadow);
+}
+
+// Wavefront form, a bounce pass: the *in_count rays of queue in (at most one
+// per pixel), their colour added into rgba, live ones to queue out.
+template <int BLOCK, int KLDS, int FORM>
+__global__ __launch_bounds__(BLOCK) void k_whitted_next(SceneView sv, ShadeView sh, WhittedParams wp,
+                                                        float4 *__restrict__ rgba, const RayRec *__restrict__ qin,
+                                                        const uint32_t *in_count, RayRec *__restrict__ qout,
+                                                        uint32_t *out_count, int last, int count,
+                                                        unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    int32_t root_ref;
+    float root_box[6];
+    load_root(sv.hdr, root_ref, root_box);
+    WCount cnt;
+    uint32_t nbounce = 0;
+    const uint32_t n = *in_count;
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        WRay r;
+        bool live = false;
+        if (i < n) {
+            r = get_ray(qin, i);
+            const float4 a = rgba[r.pix];
+            float acc[4] = {a.x, a.y, a.z, a.w};
+            bool white = false;
+            nbounce += 1;
+            live = whitted_pass<FORM>(sv, sh, wp, root_ref, root_box, r, acc, white, nullptr, nullptr, stk, cnt);
+            live = live && !last;
+            rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        const uint32_t slot = wave_append(out_count, live);
+        if (live) put_ray(qout, slot, r);
+    }
+    if (count) {
+        count_add(stats, 0, cnt.boxes);
+        count_add(stats, 1, cnt.ntris);
+    }
+    count_add(stats, 2, cnt.nshadow);
+    count_add(stats, 3, nbounce);
+}
+
+__global__ void k_zero_words(uint32_t *p, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 struct RenderJob {
@@ -1121,7 +1301,8 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
 
 hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitted_desc &d,
                                  FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
-                                 const rt_tuning &tune, hipStream_t st)
+                                 const rt_tuning &tune, void *queue, uint32_t *qcount, int cu_count,
+                                 hipStream_t st)
 {
     const uint32_t max_depth = b.desc[scene].depth;
     const SceneDesc &sd = b.desc[scene];
@@ -1154,17 +1335,35 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     }
     const SceneView sv = view_of(b, scene);
     const int keep = (flags & RT_FLAG_KEEP_HITS) ? 1 : 0, cnt = (flags & RT_FLAG_COUNT) ? 1 : 0;
-#define RT_L(B, K, F)                                                                               \
-    {                                                                                              \
-        constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                            \
-        const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);             \
-        hipLaunchKernelGGL((k_render_whitted<B, K, F>), dim3(tiles), dim3(B), 0, st, sv, sh, wp,   \
-                           fb.rgba, fb.hits, fb.occ, keep, cnt, stats);                            \
+    const bool one_pass = (d.switches & RT_WHITTED_ONE_PASS) != 0 || d.bounces == 0;
+    const uint64_t px = (uint64_t)d.width * d.height;
+    RayRec *q[2] = {static_cast<RayRec *>(queue), static_cast<RayRec *>(queue) + px};
+    if (!one_pass) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, qcount, d.bounces + 1);
+#define RT_L(B, K, F)                                                                                      \
+    {                                                                                                     \
+        constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                                   \
+        const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);                    \
+        if (one_pass) {                                                                                   \
+            hipLaunchKernelGGL((k_render_whitted<B, K, F>), dim3(tiles), dim3(B), 0, st, sv, sh, wp,      \
+                               fb.rgba, fb.hits, fb.occ, keep, cnt, stats);                               \
+        } else {                                                                                          \
+            hipLaunchKernelGGL((k_whitted_first<B, K, F>), dim3(tiles), dim3(B), 0, st, sv, sh, wp,       \
+                               fb.rgba, fb.hits, fb.occ, keep, cnt, q[0], qcount, stats);                 \
+            uint32_t blocks = (uint32_t)((px + B - 1) / B);                                               \
+            const uint32_t cap = (uint32_t)cu_count * 8u;                                                 \
+            if (blocks > cap) blocks = cap;                                                               \
+            for (uint32_t p = 1; p <= d.bounces; ++p)                                                     \
+                hipLaunchKernelGGL((k_whitted_next<B, K, F>), dim3(blocks), dim3(B), 0, st, sv, sh, wp,   \
+                                   fb.rgba, q[(p - 1) & 1], qcount + (p - 1), q[p & 1], qcount + p,       \
+                                   p == d.bounces ? 1 : 0, cnt, stats);                                   \
+        }                                                                                                 \
     }
     RT_TUNED(RT_L)
 #undef RT_L
     return hipGetLastError();
 }
+
+size_t whitted_queue_bytes(uint64_t pixels) { return 2 * sizeof(RayRec) * (size_t)pixels; }
 
 uint32_t job_tile_pixels(const rt_tuning &t) { return (uint32_t)t.block; }
 

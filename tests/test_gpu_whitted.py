@@ -11,7 +11,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 F = np.float32
-SHADOWS, PHONG, GLOSS = 1, 2, 4
+SHADOWS, PHONG, GLOSS, ONE_PASS = 1, 2, 4, 8
 
 
 def psnr(a, b):
@@ -95,14 +95,40 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("form", ["wavefront", "one-pass"])
 @pytest.mark.parametrize("tag,kw", CASES, ids=[c[0] for c in CASES])
-def test_whitted_room_matches_oracle(ctx, oracle, rtcore, room, tag, kw):
+def test_whitted_room_matches_oracle(ctx, oracle, rtcore, room, tag, kw, form):
+    """Both forms of the device frame (a compacted queue per bounce pass, or
+    one lane per pixel through every pass) against the oracle."""
     s, xyz, idx, normals, mat_ids, mats, ref = room
+    kw = dict(kw)
+    if form == "one-pass":
+        kw["switches"] = kw["switches"] | ONE_PASS
     out = run_pair(ctx, oracle, rtcore, s, xyz, idx, ref, 160, 120, (0.4, -1.7, 0.9), (0.8, -0.6, 1.5),
                    normals=normals, mat_ids=mat_ids, mats=mats, **kw)
-    check(tag, *out)
+    check(f"{tag} [{form}]", *out)
     st, nrays = out[0], out[-1]
     assert st["primary_rays"] + st["shadow_rays"] + st["bounce_rays"] == nrays
+
+
+def test_whitted_forms_identical_at_1080p(ctx, rtcore, room):
+    """The two device forms give identical frames, counters included, on a
+    full-size image; then a smaller image reuses the larger queues."""
+    s = room[0]
+    for w, h in ((1920, 1080), (333, 197)):
+        cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
+        kw = dict(bounces=4, env=(0.2, 0.3, 0.4), seed=3)
+        a = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | PHONG | GLOSS,
+                               flags=rtcore.FLAG_COUNT, **kw)
+        fa = ctx.frame_read(w, h, 1)[0]
+        b = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | PHONG | GLOSS | ONE_PASS,
+                               flags=rtcore.FLAG_COUNT, **kw)
+        fb = ctx.frame_read(w, h, 1)[0]
+        print(f"{w}x{h}: wavefront {a['trace_ms']:.3f} ms, one-pass {b['trace_ms']:.3f} ms, "
+              f"rays {a['primary_rays'] + a['shadow_rays'] + a['bounce_rays']}", flush=True)
+        assert np.array_equal(fa, fb)
+        for k in ("primary_rays", "shadow_rays", "bounce_rays", "box_tests", "tri_tests"):
+            assert a[k] == b[k], k
 
 
 def test_whitted_default_material_and_flat(ctx, oracle, rtcore):
