@@ -293,6 +293,58 @@ typedef struct rt_whitted_desc {
 int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc,
                       uint32_t flags, rt_frame_stats *stats);
 
+/* ---- second structure: the SAH BVH of the reference (BVH::Build,
+ *      AccelerationStructures/BVH/bvh.cpp:64-349; traversal BVH_IntersectP,
+ *      Shaders/AccelerationStructures/BVH.hlsl:124-217; keyboard id 0) */
+enum { RT_BVH_SAH = 0, RT_BVH_MIDDLE = 1, RT_BVH_EQUAL = 2 }; /* conf.ini bvh_split_algorithm */
+enum { RT_STRUCTURE_BVH = 0, RT_STRUCTURE_LBVH = 9 };         /* g_iAccelerationStructure */
+typedef struct rt_bvh_info {
+    uint32_t nodes;          /* LinearBVHNode count */
+    uint32_t leaves;
+    uint32_t max_depth;      /* levels of the deepest leaf + 1 (at most 27) */
+    uint32_t forced_medians; /* splits replaced by the median to keep the depth */
+    float sah_cost;          /* sum over nodes of area / root area x (0.125 inner, prims leaf) */
+    double build_ms;         /* host build time */
+} rt_bvh_info;
+/* Builds the BVH of the scene's current vertices on the host (max_prims per
+ * leaf, conf.ini bvh_max_prims_node; 1..255), stores its device layout and
+ * selects it for every later rt_trace / rt_render_* of the scene (one
+ * closest-hit order, the reference's BVH_IntersectP: near child first by the
+ * split axis and the ray's sign).  Synchronises.  The BVH is static, as in
+ * the reference: a new upload or rt_scene_transform drops it and selects
+ * the LBVH. */
+int rt_bvh_build(rt_ctx *ctx, int scene, uint32_t max_prims, uint32_t split, rt_bvh_info *info);
+/* Which structure the scene's traversals use (the other must exist). */
+int rt_scene_select(rt_ctx *ctx, int scene, int structure);
+/* The built BVH in the reference's layout: nodes (info.nodes LinearBVHNode =
+ * rt_node32: leaf prim_count > 0 primitives from prim_pos; inner prim_count =
+ * -axis, first child next, second child at prim_pos) and the ordered
+ * primitive ids (nt entries); either may be NULL. */
+int rt_bvh_read(rt_ctx *ctx, int scene, rt_node32 *nodes, int32_t *ordered_ids);
+/* Host only, no context: the same build from arrays.  nodes needs 2 nt - 1
+ * entries, ordered_ids nt; info (optional) as above. */
+int rt_bvh_build_host(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt,
+                      uint32_t max_prims, uint32_t split, rt_node32 *nodes,
+                      int32_t *ordered_ids, rt_bvh_info *info);
+
+/* ---- presentation (the swap chain's R8G8B8A8_UNORM back buffer,
+ *      Renderer/DX11/RendererDX11.inl:60-100; TakeScreenshot, :134-150) */
+/* 8-bit RGBA of a float RGBA image, each channel stored as D3D stores a
+ * float into a UNORM8 texture: NaN -> 0, clamp to [0, 1], x 255, rounded to
+ * nearest.  d_rgba: device image of width * height float4, or NULL for the
+ * context's frame buffer (the last rt_render_direct / rt_render_whitted, of
+ * this size).  RT_PRESENT_AVERAGE first divides r, g, b by w (the sample
+ * count of an rt_render_jobs accumulation buffer) and sets alpha to 1.
+ * rgba8: host, 4 bytes per pixel, rows top to bottom.  Synchronises. */
+enum { RT_PRESENT_AVERAGE = 1u << 0 };
+int rt_present(rt_ctx *ctx, const float *d_rgba, uint32_t width, uint32_t height,
+               uint32_t flags, uint8_t *rgba8);
+/* Host only: writes an 8-bit RGBA image as ".png" (RGBA; zlib stream of
+ * stored blocks) or ".ppm" (binary P6, RGB), chosen by the file name.
+ * Errors: rt_image_last_error() (this thread). */
+int rt_image_write(const char *path, const uint8_t *rgba8, uint32_t width, uint32_t height);
+const char *rt_image_last_error(void);
+
 /* ---- profiling window (Timer role, DeviceCore/Timer.hpp:27-66; Performance,
  *      Common/Performance.cpp:52-101).  Between begin and end every build,
  *      trace and render call records HIP events around its kernels on the

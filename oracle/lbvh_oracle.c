@@ -387,6 +387,143 @@ void orc_trace(const orc_node *nodes, const int32_t *ids, const float *xyz,
 }
 
 /* ------------------------------------------------------------------------ */
+/* BVH_IntersectP (BVH.hlsl:124-217): stack traversal of the flattened SAH   */
+/* BVH, box test on every visited node, near child first by dirIsNeg[axis].  */
+/* ------------------------------------------------------------------------ */
+static void trace_bvh_one(const orc_node *nodes, const int32_t *ordered, const float *xyz,
+                          const uint32_t *idx, const orc_ray *ray, int anyhit, int ref_leaf_loop,
+                          orc_hit *out)
+{
+    orc_hit best;
+    best.tri = -1;
+    best.u = -1.0f;
+    best.v = -1.0f;
+    best.t = ray->tmax;
+    best.root = 0;
+    float inv[3];
+    int neg[3];
+    for (int k = 0; k < 3; ++k) {
+        inv[k] = 1.0f / ray->d[k];
+        neg[k] = inv[k] < 0.0f ? 1 : 0;
+    }
+    int32_t stack[64];
+    int sp = 0;
+    int32_t node = 0;
+    int result = 0;
+    int stop = 0;
+    while (!stop) {
+        float t0, t1;
+        intersect_box(&nodes[node], ray->o, inv, &t0, &t1);
+        result++;
+        if ((t0 > best.t) || (t1 < 0.0f)) {
+            if (sp == 0) break;
+            node = stack[--sp];
+        } else if (nodes[node].prim_count > 0) {
+            const int32_t cnt = nodes[node].prim_count;
+            const int32_t tests = ref_leaf_loop ? cnt + 1 : cnt;
+            for (int32_t i = 0; i < tests; ++i) {
+                result += 65536;
+                const int32_t tri = ordered[nodes[node].prim_pos + (ref_leaf_loop ? 0 : i)];
+                const float *A = xyz + 3 * (size_t)idx[3 * (size_t)tri + 0];
+                const float *B = xyz + 3 * (size_t)idx[3 * (size_t)tri + 1];
+                const float *C = xyz + 3 * (size_t)idx[3 * (size_t)tri + 2];
+                float u, v, t;
+                moller_trumbore(ray->o, ray->d, A, B, C, &u, &v, &t);
+                if ((ray->tri != tri) && (u >= 0.0f) && (v >= 0.0f) && ((u + v) <= 1.0f) &&
+                    (t >= 0.0f) && (t < best.t)) {
+                    best.tri = tri;
+                    best.u = u;
+                    best.v = v;
+                    best.t = t;
+                    best.root = node;
+                    if (anyhit) {
+                        stop = 1;
+                        break;
+                    }
+                }
+            }
+            if (stop || sp == 0) break;
+            node = stack[--sp];
+        } else {
+            const int aux = neg[-nodes[node].prim_count];
+            const int32_t first = node + 1, second = nodes[node].prim_pos;
+            stack[sp++] = aux ? first : second;
+            node = aux ? second : first;
+        }
+    }
+    best.visited = result;
+    *out = best;
+}
+
+typedef struct {
+    const orc_node *nodes;
+    const int32_t *ordered;
+    const float *xyz;
+    const uint32_t *idx;
+    const orc_ray *rays;
+    orc_hit *hits;
+    uint32_t begin, end;
+    int anyhit, ref_leaf_loop;
+} bvh_job;
+
+static void *bvh_worker(void *arg)
+{
+    bvh_job *j = (bvh_job *)arg;
+    for (uint32_t i = j->begin; i < j->end; ++i)
+        trace_bvh_one(j->nodes, j->ordered, j->xyz, j->idx, &j->rays[i], j->anyhit, j->ref_leaf_loop,
+                      &j->hits[i]);
+    return NULL;
+}
+
+void orc_trace_bvh(const orc_node *nodes, const int32_t *ordered, const float *xyz, const uint32_t *idx,
+                   const orc_ray *rays, uint32_t n, int anyhit, int ref_leaf_loop, orc_hit *hits,
+                   int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if ((uint32_t)nthreads > n) nthreads = n ? (int)n : 1;
+    pthread_t th[256];
+    bvh_job jobs[256];
+    for (int t = 0; t < nthreads; ++t) {
+        bvh_job j = {nodes, ordered, xyz, idx, rays, hits, (uint32_t)((uint64_t)n * t / nthreads),
+                     (uint32_t)((uint64_t)n * (t + 1) / nthreads), anyhit, ref_leaf_loop};
+        jobs[t] = j;
+    }
+    if (nthreads == 1) {
+        bvh_worker(&jobs[0]);
+        return;
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, bvh_worker, &jobs[t]);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+}
+
+/* Closest hit over every triangle in index order (no structure): the
+ * brute-force check of any traversal.  hit.root is 0. */
+void orc_trace_brute(const float *xyz, const uint32_t *idx, uint32_t nt, const orc_ray *rays, uint32_t n,
+                     orc_hit *hits)
+{
+    for (uint32_t r = 0; r < n; ++r) {
+        const orc_ray *ray = &rays[r];
+        orc_hit best = {-1, -1.0f, -1.0f, ray->tmax, 0, 0};
+        for (uint32_t tri = 0; tri < nt; ++tri) {
+            const float *A = xyz + 3 * (size_t)idx[3 * (size_t)tri + 0];
+            const float *B = xyz + 3 * (size_t)idx[3 * (size_t)tri + 1];
+            const float *C = xyz + 3 * (size_t)idx[3 * (size_t)tri + 2];
+            float u, v, t;
+            moller_trumbore(ray->o, ray->d, A, B, C, &u, &v, &t);
+            if ((ray->tri != (int32_t)tri) && (u >= 0.0f) && (v >= 0.0f) && ((u + v) <= 1.0f) && (t >= 0.0f) &&
+                (t < best.t)) {
+                best.tri = (int32_t)tri;
+                best.u = u;
+                best.v = v;
+                best.t = t;
+            }
+        }
+        hits[r] = best;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Pinhole ray generation: the pixel-centre rule of CSPrimaryRays.hlsl:37-40 */
 /* with an explicit camera basis; maxT and the self id as at :49-53.         */
 /* ------------------------------------------------------------------------ */

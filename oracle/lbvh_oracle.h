@@ -14,6 +14,7 @@
  *   - orc_sort_pairs                     : the (code, id) order of cmpMorton, LBVH.cpp:21-38
  *   - orc_trace                          : LBVH.hlsl:363-440 (stackless trail traversal)
  *                                          + Shaders/Core/Intersection.hlsl:18-78
+ *   - orc_trace_bvh                      : BVH.hlsl:124-217 (stack traversal of the SAH BVH)
  *   - orc_raygen_pinhole                 : pixel-centre rule of CSPrimaryRays.hlsl:37-40
  *   - orc_render_direct                  : shadow-ray construction + direct term of
  *                                          CSComputeColorRT.hlsl:54-177
@@ -92,6 +93,19 @@ void orc_trace(const orc_node *nodes, const int32_t *ids, const float *xyz,
                const uint32_t *idx, const orc_ray *rays, const int32_t *start,
                uint32_t n, int anyhit, orc_hit *hits, int nthreads);
 
+/* BVH_IntersectP (BVH.hlsl:124-217) over a flattened SAH BVH in the
+ * reference's LinearBVHNode layout (orc_node: prim_count > 0 is a leaf of
+ * prim_count primitives from prim_pos of ordered; otherwise -axis, first
+ * child next, second child at prim_pos).  ref_leaf_loop = 1 runs the
+ * reference's leaf loop, prim_count + 1 tests all of the leaf's first
+ * primitive (BVH.hlsl:165-169); 0 tests each primitive once.  hit.root is
+ * the leaf's node index, 0 without a hit. */
+void orc_trace_bvh(const orc_node *nodes, const int32_t *ordered, const float *xyz, const uint32_t *idx,
+                   const orc_ray *rays, uint32_t n, int anyhit, int ref_leaf_loop, orc_hit *hits,
+                   int nthreads);
+/* Closest hit by testing every triangle in index order (no structure). */
+void orc_trace_brute(const float *xyz, const uint32_t *idx, uint32_t nt, const orc_ray *rays, uint32_t n,
+                     orc_hit *hits);
 /* Pinhole camera.  cam = {pos[3], fwd[3], right[3], up[3], tan_x, tan_y}
  * (unit basis vectors and the half-angle tangents, already rounded to float).
  * Pixel (px, py): sx = ((2px+1) - W)/W * tan_x, sy = -((2py+1) - H)/H * tan_y,

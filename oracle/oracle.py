@@ -61,6 +61,8 @@ def lib():
         L.orc_render_whitted.argtypes = [P, P, P, P, P, P, P, P, U32, U32, C.POINTER(WhittedParams), P, P,
                                          P, I32]
         L.orc_render_whitted.restype = C.c_uint64
+        L.orc_trace_bvh.argtypes = [P, P, P, P, P, U32, I32, I32, P, I32]
+        L.orc_trace_brute.argtypes = [P, P, U32, P, U32, P]
         _lib = L
     return _lib
 
@@ -133,6 +135,24 @@ def trace(nodes, ids, xyz, idx, rays, start=None, anyhit=False, nthreads=1):
     lib().orc_trace(_p(np.ascontiguousarray(nodes, NODE_DTYPE)), _p(np.ascontiguousarray(ids, np.int32)),
                     _p(np.ascontiguousarray(xyz, np.float32)), _p(np.ascontiguousarray(idx, np.uint32)),
                     _p(rays), _p(st), len(rays), 1 if anyhit else 0, _p(hits), nthreads)
+    return hits
+
+
+def trace_bvh(nodes, ordered, xyz, idx, rays, anyhit=False, ref_leaf_loop=True, nthreads=1):
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.empty(len(rays), HIT_DTYPE)
+    lib().orc_trace_bvh(_p(np.ascontiguousarray(nodes, NODE_DTYPE)), _p(np.ascontiguousarray(ordered, np.int32)),
+                        _p(np.ascontiguousarray(xyz, np.float32)), _p(np.ascontiguousarray(idx, np.uint32)),
+                        _p(rays), len(rays), 1 if anyhit else 0, 1 if ref_leaf_loop else 0, _p(hits), nthreads)
+    return hits
+
+
+def trace_brute(xyz, idx, rays):
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.empty(len(rays), HIT_DTYPE)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    lib().orc_trace_brute(_p(np.ascontiguousarray(xyz, np.float32)), _p(idx), idx.size // 3, _p(rays), len(rays),
+                          _p(hits))
     return hits
 
 

@@ -66,6 +66,11 @@ void batch_free(Batch *b)
     if (!b) return;
     for (void *p : b->allocs) hipFree(p);
     for (Batch::Shading &s : b->shading) hipFree(s.mats);
+    for (Batch::Bvh &v : b->bvh) {
+        hipFree(v.trav);
+        hipFree(v.tris);
+        hipFree(v.hdr);
+    }
     delete b;
 }
 
@@ -401,8 +406,17 @@ static int check_upload(rt_ctx *ctx, Batch *b, uint32_t i, int scene, uint32_t n
     return RT_OK;
 }
 
+// The BVH is static (built once from the vertices of its build): new
+// vertices drop it and the scene goes back to the LBVH.
+static void drop_bvh(Batch *b, uint32_t i)
+{
+    if (i < b->use_bvh.size()) b->use_bvh[i] = 0;
+    if (i < b->bvh.size()) b->bvh[i].valid = false;
+}
+
 static int finish_upload(rt_ctx *ctx, Batch *b, uint32_t i, uint32_t nv, uint32_t nt)
 {
+    drop_bvh(b, i);
     b->desc[i].nv = nv;
     b->desc[i].nt = nt;
     b->built = false;
@@ -550,6 +564,103 @@ int rt_scene_transform(rt_ctx *ctx, int scene, const float m[12])
     memcpy(x.m, m, sizeof(x.m));
     RT_HIP(ctx, rt::launch_transform(*b, i, x, ctx->stream));
     b->built = false;
+    drop_bvh(b, i);
+    return RT_OK;
+}
+
+int rt_bvh_build_host(const float *xyz, uint32_t nv, const uint32_t *idx, uint32_t nt, uint32_t max_prims,
+                      uint32_t split, rt_node32 *nodes, int32_t *ordered_ids, rt_bvh_info *info)
+{
+    if (!xyz || !idx || nt == 0 || nt > (1u << 25) || max_prims == 0 || split > RT_BVH_EQUAL) return RT_ERR_ARG;
+    for (uint64_t k = 0; k < 3ull * nt; ++k)
+        if (idx[k] >= nv) return RT_ERR_ARG;
+    rt::BvhHost h;
+    rt::bvh_build_host(xyz, idx, nt, max_prims, split, h, info);
+    if (nodes) memcpy(nodes, h.nodes.data(), sizeof(rt_node32) * h.nodes.size());
+    if (ordered_ids) memcpy(ordered_ids, h.ordered.data(), sizeof(int32_t) * h.ordered.size());
+    return RT_OK;
+}
+
+int rt_bvh_build(rt_ctx *ctx, int scene, uint32_t max_prims, uint32_t split, rt_bvh_info *info)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_bvh_build: unknown scene %d", scene);
+    if (max_prims == 0 || split > RT_BVH_EQUAL)
+        return fail(ctx, RT_ERR_ARG, "rt_bvh_build: max_prims >= 1 and a split method 0..2");
+    const SceneDesc &d = b->desc[i];
+    if (d.nt == 0) return fail(ctx, RT_ERR_STATE, "rt_bvh_build: scene %d has no triangles", scene);
+    hipSetDevice(ctx->device);
+    // the scene's current vertices and indices, as the kernels see them
+    std::vector<float> xyz(3 * (size_t)d.nv);
+    std::vector<uint32_t> idx(3 * (size_t)d.nt);
+    RT_HIP(ctx, hipMemcpyAsync(xyz.data(), b->xyz + 3 * (size_t)d.vert_off, 12ull * d.nv, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    RT_HIP(ctx, hipMemcpyAsync(idx.data(), b->idx + 3 * (size_t)d.in_prim_off, 12ull * d.nt,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (b->bvh.size() < b->nscenes) b->bvh.resize(b->nscenes);
+    if (b->use_bvh.size() < b->nscenes) b->use_bvh.assign(b->nscenes, 0);
+    Batch::Bvh &v = b->bvh[i];
+    v.valid = false;
+    b->use_bvh[i] = 0;
+    rt::bvh_build_host(xyz.data(), idx.data(), d.nt, max_prims, split, v.host, info);
+    std::vector<rt::TravNode> trav;
+    std::vector<rt::TravTri> tris;
+    rt::SceneHdr hdr;
+    rt::bvh_device_layout(xyz.data(), idx.data(), v.host, trav, tris, hdr);
+    hipFree(v.trav);
+    hipFree(v.tris);
+    hipFree(v.hdr);
+    v.trav = nullptr;
+    v.tris = nullptr;
+    v.hdr = nullptr;
+    b->bytes -= v.bytes;
+    v.bytes = 0;
+    RT_HIP(ctx, hipMalloc(&v.trav, sizeof(rt::TravNode) * trav.size()));
+    RT_HIP(ctx, hipMalloc(&v.tris, sizeof(rt::TravTri) * tris.size()));
+    RT_HIP(ctx, hipMalloc(&v.hdr, sizeof(rt::SceneHdr)));
+    v.bytes = sizeof(rt::TravNode) * trav.size() + sizeof(rt::TravTri) * tris.size() + sizeof(rt::SceneHdr);
+    b->bytes += v.bytes;
+    RT_HIP(ctx, hipMemcpyAsync(v.trav, trav.data(), sizeof(rt::TravNode) * trav.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    RT_HIP(ctx, hipMemcpyAsync(v.tris, tris.data(), sizeof(rt::TravTri) * tris.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    RT_HIP(ctx, hipMemcpyAsync(v.hdr, &hdr, sizeof(hdr), hipMemcpyHostToDevice, ctx->stream));
+    RT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    v.valid = true;
+    b->use_bvh[i] = 1;
+    return RT_OK;
+}
+
+int rt_scene_select(rt_ctx *ctx, int scene, int structure)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_scene_select: unknown scene %d", scene);
+    if (structure == RT_STRUCTURE_BVH) {
+        if (i >= b->bvh.size() || !b->bvh[i].valid)
+            return fail(ctx, RT_ERR_STATE, "rt_scene_select: scene %d has no BVH (rt_bvh_build)", scene);
+        b->use_bvh[i] = 1;
+        return RT_OK;
+    }
+    if (structure == RT_STRUCTURE_LBVH) {
+        if (i < b->use_bvh.size()) b->use_bvh[i] = 0;
+        return RT_OK;
+    }
+    return fail(ctx, RT_ERR_ARG, "rt_scene_select: structure %d (0 BVH, 9 LBVH)", structure);
+}
+
+int rt_bvh_read(rt_ctx *ctx, int scene, rt_node32 *nodes, int32_t *ordered_ids)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_bvh_read: unknown scene %d", scene);
+    if (i >= b->bvh.size() || !b->bvh[i].valid)
+        return fail(ctx, RT_ERR_STATE, "rt_bvh_read: scene %d has no BVH", scene);
+    const rt::BvhHost &h = b->bvh[i].host;
+    if (nodes) memcpy(nodes, h.nodes.data(), sizeof(rt_node32) * h.nodes.size());
+    if (ordered_ids) memcpy(ordered_ids, h.ordered.data(), sizeof(int32_t) * h.ordered.size());
     return RT_OK;
 }
 
@@ -657,7 +768,7 @@ int rt_trace(rt_ctx *ctx, int scene, const rt_ray48 *rays, uint32_t n, rt_hit24 
     Batch *b;
     uint32_t i;
     if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_trace: unknown scene %d", scene);
-    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_trace: scene %d not built", scene);
+    if (!traversable(*b, i)) return fail(ctx, RT_ERR_STATE, "rt_trace: scene %d not built", scene);
     if (n && (!rays || !hits)) return fail(ctx, RT_ERR_ARG, "rt_trace: null array");
     hipSetDevice(ctx->device);
     const uint64_t rb = (sizeof(rt_ray48) * (uint64_t)n + 255) & ~255ull;
@@ -683,7 +794,7 @@ int rt_trace_device(rt_ctx *ctx, int scene, const rt_ray48 *d_rays, uint32_t n, 
     Batch *b;
     uint32_t i;
     if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_trace_device: unknown scene %d", scene);
-    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_trace_device: scene %d not built", scene);
+    if (!traversable(*b, i)) return fail(ctx, RT_ERR_STATE, "rt_trace_device: scene %d not built", scene);
     if (n && (!d_rays || !d_hits)) return fail(ctx, RT_ERR_ARG, "rt_trace_device: null array");
     if (((uintptr_t)d_rays & 15) || ((uintptr_t)d_hits & 7))
         return fail(ctx, RT_ERR_ARG, "rt_trace_device: rays need 16-byte, hits 8-byte alignment");
@@ -697,7 +808,7 @@ int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, uint32_
     Batch *b;
     uint32_t i;
     if (!get_scene(ctx, scene, &b, &i) || !desc) return fail(ctx, RT_ERR_ARG, "rt_render_direct: bad argument");
-    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_render_direct: scene %d not built", scene);
+    if (!traversable(*b, i)) return fail(ctx, RT_ERR_STATE, "rt_render_direct: scene %d not built", scene);
     if (desc->nlights > 8 || (desc->nlights && !desc->lights))
         return fail(ctx, RT_ERR_ARG, "rt_render_direct: 0..8 lights with a light array");
     if (desc->width == 0 || desc->height == 0 || desc->width > 16384 || desc->height > 16384)
@@ -735,7 +846,7 @@ int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint3
     Batch *b;
     uint32_t i;
     if (!get_scene(ctx, scene, &b, &i) || !desc) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: bad argument");
-    if (!b->built) return fail(ctx, RT_ERR_STATE, "rt_render_whitted: scene %d not built", scene);
+    if (!traversable(*b, i)) return fail(ctx, RT_ERR_STATE, "rt_render_whitted: scene %d not built", scene);
     if (desc->width == 0 || desc->height == 0 || desc->width > 16384 || desc->height > 16384)
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: image %ux%u", desc->width, desc->height);
     if (desc->bounces > 16) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: %u bounces (0..16)", desc->bounces);
@@ -844,7 +955,7 @@ static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, cons
         Batch *b;
         uint32_t i;
         if (!get_scene(ctx, d.scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "job %u: unknown scene %d", k, d.scene);
-        if (!b->built) return fail(ctx, RT_ERR_STATE, "job %u: scene %d not built", k, d.scene);
+        if (!traversable(*b, i)) return fail(ctx, RT_ERR_STATE, "job %u: scene %d not built", k, d.scene);
         if (!d.accum || ((uintptr_t)d.accum & 15))
             return fail(ctx, RT_ERR_ARG, "job %u: accum must be a 16-byte aligned device pointer", k);
         if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || d.spp == 0 ||
@@ -853,7 +964,7 @@ static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, cons
             return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights or mode", k);
         uint32_t tiles = 0;
         rt::fill_render_job(host.data() + jb * k, *b, i, d, t, off, &tiles);
-        if (b->desc[i].depth > *max_depth) *max_depth = b->desc[i].depth;
+        if (trav_depth(*b, i) > *max_depth) *max_depth = trav_depth(*b, i);
         if (tile_offs) tile_offs->push_back(off);
         off += tiles;
     }

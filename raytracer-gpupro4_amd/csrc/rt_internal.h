@@ -103,6 +103,14 @@ constexpr int kCntWork = 1;
 constexpr int kCntTiles = 2;
 constexpr int kCounters = 16;
 
+// SAH BVH of one scene as built on the host (LinearBVHNode order) plus its
+// depth; see bvh.cpp.
+struct BvhHost {
+    std::vector<rt_node32> nodes;
+    std::vector<int32_t> ordered;  // orderedPrimitivesIds
+    uint32_t max_level = 0;        // deepest leaf, root = 0
+};
+
 // A batch: scenes sharing one set of concatenated device arrays, built by one
 // sequence of launches.  A plain scene is a batch of one.
 struct Batch {
@@ -149,7 +157,29 @@ struct Batch {
         bool normals = false, ids = false;
     };
     std::vector<Shading> shading;  // per scene
+    // Second structure (rt_bvh_build): per scene, the host tree and its
+    // device layout; use_bvh selects it for every traversal of the scene.
+    struct Bvh {
+        BvhHost host;
+        TravNode *trav = nullptr;
+        TravTri *tris = nullptr;
+        SceneHdr *hdr = nullptr;
+        uint64_t bytes = 0;
+        bool valid = false;
+    };
+    std::vector<Bvh> bvh;
+    std::vector<uint8_t> use_bvh;
 };
+
+inline bool uses_bvh(const Batch &b, uint32_t s) { return s < b.use_bvh.size() && b.use_bvh[s]; }
+// The scene can be traversed: its selected structure has been built.
+inline bool traversable(const Batch &b, uint32_t s) { return uses_bvh(b, s) || b.built; }
+// Depth of the selected structure (levels of the deepest leaf + 1): a
+// traversal stack holds at most depth - 1 entries.
+inline uint32_t trav_depth(const Batch &b, uint32_t s)
+{
+    return uses_bvh(b, s) ? b.bvh[s].host.max_level + 1 : b.desc[s].depth;
+}
 
 struct SceneRef {
     Batch *batch = nullptr;
@@ -237,6 +267,12 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
                               int cu_count,
                               hipStream_t st);
 hipError_t launch_bump_counter(uint32_t *c, hipStream_t st);
+
+// SAH BVH (bvh.cpp).
+int bvh_build_host(const float *xyz, const uint32_t *idx, uint32_t nt, uint32_t max_prims, uint32_t split,
+                   BvhHost &out, rt_bvh_info *info);
+void bvh_device_layout(const float *xyz, const uint32_t *idx, const BvhHost &h, std::vector<TravNode> &trav,
+                       std::vector<TravTri> &tris, SceneHdr &hdr);
 int tune_index(const rt_tuning &t);
 uint32_t sort_tiles(uint32_t n);
 uint32_t bounds_blocks(uint32_t n);

@@ -1198,6 +1198,10 @@ __global__ void k_bump_counter(uint32_t *c)
 // ---------------------------------------------------------------------------
 static SceneView view_of(const Batch &b, uint32_t scene)
 {
+    if (uses_bvh(b, scene)) {
+        const Batch::Bvh &v = b.bvh[scene];
+        return SceneView{v.trav, v.tris, v.hdr};
+    }
     return SceneView{b.trav, b.tris, b.hdr + scene};
 }
 
@@ -1253,7 +1257,7 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
 {
     if (n == 0) return hipSuccess;
     const SceneView sv = view_of(b, scene);
-    const uint32_t max_depth = b.desc[scene].depth;
+    const uint32_t max_depth = trav_depth(b, scene);
     const bool any = flags & RT_FLAG_ANYHIT, cnt = flags & RT_FLAG_COUNT;
 #define RT_L(B, K, F)                                                                               \
     {                                                                                              \
@@ -1272,7 +1276,7 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st)
 {
-    const uint32_t max_depth = b.desc[scene].depth;
+    const uint32_t max_depth = trav_depth(b, scene);
     RenderParams rp;
     rp.cam = d.cam;
     rp.width = d.width;
@@ -1304,7 +1308,7 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
                                  const rt_tuning &tune, void *queue, uint32_t *qcount, int cu_count,
                                  hipStream_t st)
 {
-    const uint32_t max_depth = b.desc[scene].depth;
+    const uint32_t max_depth = trav_depth(b, scene);
     const SceneDesc &sd = b.desc[scene];
     WhittedParams wp;
     memset(&wp, 0, sizeof(wp));
@@ -1373,9 +1377,10 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
     RenderJob *j = static_cast<RenderJob *>(dst);
     memset(j, 0, sizeof(*j));
     const uint32_t TW = 16, TH = (uint32_t)t.block / 16;
-    j->nodes = b.trav;
-    j->tris = b.tris;
-    j->hdr = b.hdr + scene;
+    const SceneView sv = view_of(b, scene);
+    j->nodes = sv.nodes;
+    j->tris = sv.tris;
+    j->hdr = sv.hdr;
     j->accum = reinterpret_cast<float4 *>(d.accum);
     j->cam = d.cam;
     j->width = d.width;

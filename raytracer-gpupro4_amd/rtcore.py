@@ -98,6 +98,14 @@ class Model(C.Structure):
                 ("materials", C.c_void_p), ("nv", C.c_uint32), ("nt", C.c_uint32), ("nmat", C.c_uint32)]
 
 
+class BvhInfo(C.Structure):
+    _fields_ = [("nodes", C.c_uint32), ("leaves", C.c_uint32), ("max_depth", C.c_uint32),
+                ("forced_medians", C.c_uint32), ("sah_cost", C.c_float), ("build_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("traversal", C.c_int32)]
@@ -196,6 +204,13 @@ def lib() -> C.CDLL:
         "rt_scene_shading": (I32, [P, I32, P, U32, P, U32, P, U32]),
         "rt_render_whitted": (I32, [P, I32, C.POINTER(WhittedDesc), U32, C.POINTER(FrameStats)]),
         "rt_load_model_ex": (I32, [C.c_char_p, U32, C.POINTER(Model)]),
+        "rt_present": (I32, [P, P, U32, U32, U32, P]),
+        "rt_bvh_build": (I32, [P, I32, U32, U32, C.POINTER(BvhInfo)]),
+        "rt_scene_select": (I32, [P, I32, I32]),
+        "rt_bvh_read": (I32, [P, I32, P, P]),
+        "rt_bvh_build_host": (I32, [P, U32, P, U32, U32, U32, P, P, C.POINTER(BvhInfo)]),
+        "rt_image_write": (I32, [C.c_char_p, P, U32, U32]),
+        "rt_image_last_error": (C.c_char_p, []),
         "rt_profile_begin": (I32, [P]),
         "rt_profile_end": (I32, [P, C.POINTER(Profile)]),
         "rt_tuning_get": (I32, [P, C.POINTER(Tuning)]),
@@ -299,6 +314,36 @@ def load_model_shading(path, raw=False) -> dict:
     if rc != RT_OK:
         raise RtError(rc, L.rt_load_last_error().decode())
     return out
+
+
+PRESENT_AVERAGE = 1 << 0
+BVH_SAH, BVH_MIDDLE, BVH_EQUAL = 0, 1, 2
+STRUCTURE_BVH, STRUCTURE_LBVH = 0, 9
+
+
+def bvh_build_host(xyz, idx, max_prims=1, split=BVH_SAH):
+    """The SAH BVH of the arrays on the host: (nodes NODE_DTYPE, ordered ids, info)."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    nt = idx.size // 3
+    nodes = np.zeros(max(2 * nt - 1, 1), NODE_DTYPE)
+    ids = np.empty(nt, np.int32)
+    info = BvhInfo()
+    rc = lib().rt_bvh_build_host(_ptr(xyz), xyz.size // 3, _ptr(idx), nt, max_prims, split, _ptr(nodes),
+                                 _ptr(ids), C.byref(info))
+    if rc != RT_OK:
+        raise RtError(rc, "rt_bvh_build_host")
+    return nodes[:info.nodes].copy(), ids, info.as_dict()
+
+
+def image_write(path, rgba8: np.ndarray):
+    """Writes an (h, w, 4) uint8 image as .png or .ppm (host only)."""
+    img = np.ascontiguousarray(rgba8, np.uint8)
+    h, w = img.shape[:2]
+    L = lib()
+    rc = L.rt_image_write(os.fsencode(path), _ptr(img), w, h)
+    if rc != RT_OK:
+        raise RtError(rc, L.rt_image_last_error().decode())
 
 
 def gen_torus(R=1.0, r=0.3, nu=128, nv=64, jitter=1e-3, seed=1):
@@ -595,6 +640,29 @@ class Context:
         self.check(self.L.rt_render_whitted(self.h, scene, C.byref(d), flags,
                                             C.byref(st) if want_stats else None), "rt_render_whitted")
         return st.as_dict() if want_stats else None
+
+    def bvh_build(self, scene: int, max_prims: int = 1, split: int = BVH_SAH) -> dict:
+        info = BvhInfo()
+        self.check(self.L.rt_bvh_build(self.h, scene, max_prims, split, C.byref(info)), "rt_bvh_build")
+        return info.as_dict()
+
+    def scene_select(self, scene: int, structure: int):
+        self.check(self.L.rt_scene_select(self.h, scene, structure), "rt_scene_select")
+
+    def bvh_read(self, scene: int, nodes: int):
+        nt = self.scene_info(scene)[0]
+        out = np.empty(nodes, NODE_DTYPE)
+        ids = np.empty(nt, np.int32)
+        self.check(self.L.rt_bvh_read(self.h, scene, _ptr(out), _ptr(ids)), "rt_bvh_read")
+        return out, ids
+
+    def present(self, width: int, height: int, d_rgba: int | None = None, average: bool = False) -> np.ndarray:
+        """(h, w, 4) uint8 image of the context's frame buffer, or of a device
+        float RGBA image (an accumulation buffer with average=True)."""
+        out = np.empty((height, width, 4), np.uint8)
+        self.check(self.L.rt_present(self.h, None if d_rgba is None else C.c_void_p(d_rgba), width, height,
+                                     PRESENT_AVERAGE if average else 0, _ptr(out)), "rt_present")
+        return out
 
     def frame_read(self, width: int, height: int, nlights: int, hits: bool = False):
         rgba = np.empty((height * width, 4), np.float32)
