@@ -22,6 +22,10 @@ Other workloads (--workload), each printing one JSON line of the same shape:
            mirror, glass and half-mirror materials, 1920x1080 Whitted frames
            of 3 bounces (primary, closest-hit continuation and shadow rays, the
            reference's ray-tracing colour stage); value = Mrays/s.
+  bvh      the frame's soup (static) traced through the SAH BVH (host build, as
+           the reference builds it) and through the LBVH: direct frames at
+           1920x1080; value = Mrays/s on the BVH, with the LBVH's rate, both
+           structures' SAH costs and build times beside it.
   serve    Poisson 200 requests/s for 60 s over a pool of 500 scenes
            (1k..500k triangles), 512^2 or 1920x1080, direct or 2-bounce GI at
            1 spp with progressive accumulation, 200 GB device budget; p50/p99
@@ -307,7 +311,7 @@ def workload_whitted(args, rtcore):
     s = ctx.add_scene(xyz, idx, DEPTH)
     ctx.build(s)
     ctx.scene_shading(s, None, mat_ids, mats)
-    tuned = ctx.autotune_direct(s, cam, W, H, LIGHTS)
+    tuned, kw["switches"] = ctx.autotune_whitted(s, cam, W, H, LIGHTS[0], **kw)
     for f in range(args.warmup):
         ctx.render_whitted(s, cam, W, H, LIGHTS[0], frame=f, want_stats=False, **kw)
     device_sync(ctx)
@@ -371,6 +375,87 @@ def workload_whitted(args, rtcore):
                                "sample": f"one {sw}x{sh} Whitted frame of the same scene, view and materials: "
                                          f"{n} rays in {dt:.1f} s on {cores} threads"}
     return out
+
+
+# ------------------------------------------------------------------ bvh
+def sah_cost(nodes, heap):
+    """Sum over reachable nodes of area / root area x (0.125 inner, primitive
+    count leaf): the same measure for the LBVH heap (root 1, children 2n and
+    2n + 1, leaf bit 4, count << 3) and the flattened BVH (root 0)."""
+    def area(n):
+        d = (n["mx"] - n["mn"]).astype(np.float64)
+        return 2.0 * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2])
+
+    root = 1 if heap else 0
+    ra = area(nodes[root])
+    cost, stack = 0.0, [root]
+    while stack:
+        i = stack.pop()
+        n = nodes[i]
+        pc = int(n["prim_count"])
+        leaf = (pc & 4) != 0 if heap else pc > 0
+        if leaf:
+            cost += area(n) / ra * ((pc >> 3) if heap else pc)
+        else:
+            cost += area(n) / ra * 0.125
+            stack += [2 * i, 2 * i + 1] if heap else [i + 1, int(n["prim_pos"])]
+    return cost
+
+
+def workload_bvh(args, rtcore):
+    xyz, idx = rtcore.gen_triangle_soup(NTRI, SIGMA, SEED)
+    cam = rtcore.camera_look_at(CAM_POS, CAM_TGT, (0, 1, 0), FOV, W / H)
+    ctx = rtcore.Context(0)
+    s = ctx.add_scene(xyz, idx, DEPTH)
+    lb = ctx.build(s, timed=True)
+    heap = ctx.read(s)[0]
+    lbvh_cost = sah_cost(heap, True)
+    rates = {}
+    for name in ("lbvh", "bvh"):
+        if name == "bvh":
+            info = ctx.bvh_build(s)
+            nodes, _ = ctx.bvh_read(s, info["nodes"])
+            bvh_cost = sah_cost(nodes, False)
+        tuned = ctx.autotune_direct(s, cam, W, H, LIGHTS)
+        for _ in range(args.warmup):
+            ctx.render_direct(s, cam, W, H, LIGHTS, want_stats=False)
+        device_sync(ctx)
+        ctx.profile_begin()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.render_direct(s, cam, W, H, LIGHTS, want_stats=False)
+        device_sync(ctx)
+        wall = time.perf_counter() - t0
+        prof = ctx.profile_end()
+        st = ctx.render_direct(s, cam, W, H, LIGHTS, flags=rtcore.FLAG_COUNT)
+        rays = prof["primary_rays"] + prof["shadow_rays"]
+        per_launch_ms = prof["trace_ms"] / prof["launches"]
+        per_rays = st["primary_rays"] + st["shadow_rays"]
+        alg = RAY_BYTES * per_rays + BOX_BYTES * st["box_tests"] + TRI_BYTES * st["tri_tests"]
+        rates[name] = {"mrays_s": round(rays / (prof["trace_ms"] * 1e-3) / 1e6, 2),
+                       "trace_ms_per_frame": round(per_launch_ms, 4), "wall": wall,
+                       "box_per_ray": round(st["box_tests"] / per_rays, 2),
+                       "tri_per_ray": round(st["tri_tests"] / per_rays, 2), "tuning": list(tuned),
+                       "achieved": alg / (per_launch_ms * 1e-3) / 1e9, "bytes_per_launch": alg}
+    ctx.close()
+    b = rates["bvh"]
+    return result(METRIC, b["mrays_s"], "Mrays/s", args.steps, args.warmup, b.pop("wall"),
+                  {"workload": f"{NTRI}-triangle soup (static), SAH BVH (1 triangle per leaf) against the "
+                               f"LBVH (depth {DEPTH}), {W}x{H} primary + shadow rays, 1 point light",
+                   "tris": NTRI, "width": W, "height": H},
+                  bvh={"mrays_s": b["mrays_s"], "trace_ms_per_frame": b["trace_ms_per_frame"],
+                       "box_per_ray": b["box_per_ray"], "tri_per_ray": b["tri_per_ray"],
+                       "sah_cost": round(bvh_cost, 3), "build_ms_host_1thread": round(info["build_ms"], 2),
+                       "max_depth": info["max_depth"], "forced_medians": info["forced_medians"],
+                       "tuning": b["tuning"]},
+                  lbvh={"mrays_s": rates["lbvh"]["mrays_s"], "trace_ms_per_frame": rates["lbvh"]["trace_ms_per_frame"],
+                        "box_per_ray": rates["lbvh"]["box_per_ray"], "tri_per_ray": rates["lbvh"]["tri_per_ray"],
+                        "sah_cost": round(lbvh_cost, 3), "build_ms_device": round(lb["total_ms"], 4),
+                        "tuning": rates["lbvh"]["tuning"]},
+                  roofline={"bound": "hbm", "achieved": round(b["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(b["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                            "kernel": "k_render_direct", "bytes_per_launch": b["bytes_per_launch"]},
+                  cpu_baseline=None)
 
 
 # ------------------------------------------------------------------ batched
@@ -637,7 +722,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["frame", "sphere", "whitted", "batched", "serve"], default="frame")
+    ap.add_argument("--workload", choices=["frame", "sphere", "whitted", "bvh", "batched", "serve"],
+                    default="frame")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--plain-seconds", type=float, default=10.0,
                     help="serve: seconds of the same arrivals replayed one request per launch")
@@ -652,7 +738,8 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)
         sys.exit(2)
-    defaults = {"frame": (100, 5), "sphere": (10, 1), "whitted": (20, 2), "batched": (3, 1), "serve": (0, 0)}
+    defaults = {"frame": (100, 5), "sphere": (10, 1), "whitted": (20, 2), "bvh": (50, 3), "batched": (3, 1),
+                "serve": (0, 0)}
     ds, dw = defaults[args.workload]
     args.steps = ds if args.steps is None else args.steps
     args.warmup = dw if args.warmup is None else args.warmup
@@ -664,7 +751,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     rtcore.lib()
-    fn = {"frame": workload_frame, "sphere": workload_sphere, "whitted": workload_whitted,
+    fn = {"frame": workload_frame, "sphere": workload_sphere, "whitted": workload_whitted, "bvh": workload_bvh,
           "batched": workload_batched, "serve": workload_serve}[args.workload]
     print(json.dumps(fn(args, rtcore)), flush=True)
 

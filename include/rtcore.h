@@ -292,6 +292,12 @@ typedef struct rt_whitted_desc {
 } rt_whitted_desc;
 int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc,
                       uint32_t flags, rt_frame_stats *stats);
+/* Times every launch shape, traversal form and both Whitted forms on this
+ * frame (renders into the context's frame buffer), keeps the fastest shape in
+ * the context and sets or clears RT_WHITTED_ONE_PASS in desc->switches to the
+ * faster form.  cache_path as for rt_autotune_jobs. */
+int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc,
+                        const char *cache_path, rt_tuning *best);
 
 /* ---- second structure: the SAH BVH of the reference (BVH::Build,
  *      AccelerationStructures/BVH/bvh.cpp:64-349; traversal BVH_IntersectP,

@@ -1148,8 +1148,8 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     if (!get_scene(ctx, scene, &b, &i) || !desc) return fail(ctx, RT_ERR_ARG, "rt_autotune_direct: bad argument");
     if (ctx->profiling) return fail(ctx, RT_ERR_STATE, "rt_autotune_direct: inside a profiling window");
     char key[160];
-    snprintf(key, sizeof(key), "direct:cu%d:t%u:d%u:%ux%u:l%u", ctx->cu_count, b->desc[i].nt, b->desc[i].depth,
-             desc->width, desc->height, desc->nlights);
+    snprintf(key, sizeof(key), "direct:cu%d:%s:t%u:d%u:%ux%u:l%u", ctx->cu_count, uses_bvh(*b, i) ? "bvh" : "lbvh",
+             b->desc[i].nt, trav_depth(*b, i), desc->width, desc->height, desc->nlights);
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "r")) {
             char k[256];
@@ -1194,6 +1194,77 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "a")) {
             fprintf(f, "%s %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal);
+            fclose(f);
+        }
+    }
+    return RT_OK;
+}
+
+int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const char *cache_path, rt_tuning *best)
+{
+    Batch *b;
+    uint32_t i;
+    if (!get_scene(ctx, scene, &b, &i) || !desc) return fail(ctx, RT_ERR_ARG, "rt_autotune_whitted: bad argument");
+    if (ctx->profiling) return fail(ctx, RT_ERR_STATE, "rt_autotune_whitted: inside a profiling window");
+    char key[192];
+    snprintf(key, sizeof(key), "whitted:cu%d:%s:t%u:d%u:%ux%u:b%u:s%u", ctx->cu_count,
+             uses_bvh(*b, i) ? "bvh" : "lbvh", b->desc[i].nt, trav_depth(*b, i), desc->width, desc->height,
+             desc->bounces, desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+    if (cache_path) {
+        if (FILE *f = fopen(cache_path, "r")) {
+            char k[256];
+            rt_tuning t;
+            int one = 0;
+            while (fscanf(f, "%255s %d %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal,
+                          &one) == 6) {
+                if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK) {
+                    fclose(f);
+                    ctx->tune = t;
+                    desc->switches = one ? (desc->switches | RT_WHITTED_ONE_PASS)
+                                         : (desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+                    if (best) *best = t;
+                    return RT_OK;
+                }
+            }
+            fclose(f);
+        }
+    }
+    const rt_tuning saved = ctx->tune;
+    rt_tuning bt = saved;
+    int bone = 0;
+    float bms = 1e30f;
+    int rc = RT_OK;
+    rt_whitted_desc d = *desc;
+    for (int one = 0; one < 2 && rc == RT_OK; ++one) {
+        d.switches = one ? (desc->switches | RT_WHITTED_ONE_PASS) : (desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+        for (int form = 0; form < 2 && rc == RT_OK; ++form) {
+            for (const rt_tuning &shape : kShapes) {
+                rt_tuning t = shape;
+                t.blocks_per_cu = saved.blocks_per_cu;
+                t.traversal = form;
+                ctx->tune = t;
+                float ms = 1e30f;
+                for (int rep = 0; rep < 3 && rc == RT_OK; ++rep) {  // first run warms caches
+                    rt_frame_stats st;
+                    rc = rt_render_whitted(ctx, scene, &d, 0, &st);
+                    if (rep > 0 && st.trace_ms < ms) ms = st.trace_ms;
+                }
+                if (rc) break;
+                if (ms < bms) {
+                    bms = ms;
+                    bt = t;
+                    bone = one;
+                }
+            }
+        }
+    }
+    ctx->tune = rc ? saved : bt;
+    if (rc) return rc;
+    desc->switches = bone ? (desc->switches | RT_WHITTED_ONE_PASS) : (desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+    if (best) *best = bt;
+    if (cache_path) {
+        if (FILE *f = fopen(cache_path, "a")) {
+            fprintf(f, "%s %d %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal, bone);
             fclose(f);
         }
     }

@@ -203,6 +203,7 @@ def lib() -> C.CDLL:
         "rt_frame_read": (I32, [P, P, P, P]),
         "rt_scene_shading": (I32, [P, I32, P, U32, P, U32, P, U32]),
         "rt_render_whitted": (I32, [P, I32, C.POINTER(WhittedDesc), U32, C.POINTER(FrameStats)]),
+        "rt_autotune_whitted": (I32, [P, I32, C.POINTER(WhittedDesc), C.c_char_p, C.POINTER(Tuning)]),
         "rt_load_model_ex": (I32, [C.c_char_p, U32, C.POINTER(Model)]),
         "rt_present": (I32, [P, P, U32, U32, U32, P]),
         "rt_bvh_build": (I32, [P, I32, U32, U32, C.POINTER(BvhInfo)]),
@@ -626,9 +627,8 @@ class Context:
         self.check(self.L.rt_scene_shading(self.h, scene, _ptr(nrm), nv, _ptr(mid), nt, _ptr(mats),
                                            0 if mats is None else len(mats)), "rt_scene_shading")
 
-    def render_whitted(self, scene: int, cam: Camera, width: int, height: int, light, bounces=2,
-                       switches=WHITTED_SHADOWS, env_flag=-1, env=(0.0, 0.0, 0.0), seed=1, frame=0,
-                       flags: int = 0, want_stats: bool = True):
+    @staticmethod
+    def _whitted_desc(cam, width, height, light, bounces, switches, env_flag, env, seed, frame):
         d = WhittedDesc()
         d.cam = cam
         d.width, d.height = width, height
@@ -636,6 +636,23 @@ class Context:
         d.bounces, d.switches, d.env_flag = bounces, switches, env_flag
         d.env[:] = [float(x) for x in env]
         d.frame, d.seed = frame, seed
+        return d
+
+    def autotune_whitted(self, scene: int, cam: Camera, width: int, height: int, light, bounces=2,
+                         switches=WHITTED_SHADOWS, env_flag=-1, env=(0.0, 0.0, 0.0), seed=1, frame=0,
+                         cache_path=None):
+        """Picks the launch shape and Whitted form; returns (tuning tuple, switches to use)."""
+        d = self._whitted_desc(cam, width, height, light, bounces, switches, env_flag, env, seed, frame)
+        t = Tuning()
+        self.check(self.L.rt_autotune_whitted(self.h, scene, C.byref(d),
+                                              cache_path.encode() if cache_path else None, C.byref(t)),
+                   "rt_autotune_whitted")
+        return t.as_tuple(), int(d.switches)
+
+    def render_whitted(self, scene: int, cam: Camera, width: int, height: int, light, bounces=2,
+                       switches=WHITTED_SHADOWS, env_flag=-1, env=(0.0, 0.0, 0.0), seed=1, frame=0,
+                       flags: int = 0, want_stats: bool = True):
+        d = self._whitted_desc(cam, width, height, light, bounces, switches, env_flag, env, seed, frame)
         st = FrameStats()
         self.check(self.L.rt_render_whitted(self.h, scene, C.byref(d), flags,
                                             C.byref(st) if want_stats else None), "rt_render_whitted")

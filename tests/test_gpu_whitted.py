@@ -204,3 +204,23 @@ def test_scene_shading_rejects_bad_input(ctx, rtcore):
         ctx.scene_shading(s, None, np.ones(len(idx), np.uint32), None)
     ctx.scene_shading(s, None, np.zeros(len(idx), np.uint32), None)
     ctx.scene_destroy(s)
+
+
+def test_autotune_whitted(ctx, rtcore, room, tmp_path):
+    """The autotuner returns a valid shape and form, writes the cache, reads
+    it back, and the chosen form renders the same frame as the other."""
+    s = room[0]
+    w, h = 320, 240
+    cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
+    kw = dict(bounces=3, switches=SHADOWS | PHONG, env=(0.2, 0.3, 0.4))
+    cache = tmp_path / "tune.txt"
+    t1, sw1 = ctx.autotune_whitted(s, cam, w, h, (0.8, -0.6, 1.5), cache_path=str(cache), **kw)
+    assert sw1 & ~ONE_PASS == kw["switches"] and len(cache.read_text().splitlines()) == 1
+    t2, sw2 = ctx.autotune_whitted(s, cam, w, h, (0.8, -0.6, 1.5), cache_path=str(cache), **kw)
+    assert (t1, sw1) == (t2, sw2) and len(cache.read_text().splitlines()) == 1
+    ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), **dict(kw, switches=sw1))
+    a = ctx.frame_read(w, h, 1)[0]
+    ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), **dict(kw, switches=sw1 ^ ONE_PASS))
+    b = ctx.frame_read(w, h, 1)[0]
+    assert np.array_equal(a, b)
+    ctx.set_tuning(128, 26, 8, 0)
