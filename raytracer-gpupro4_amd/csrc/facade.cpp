@@ -116,6 +116,15 @@ RayTracerHip::RayTracerHip(Device &dev, LbvhGpu &lbvh, unsigned int width, unsig
     desc_.ambient = 0.1f;
     desc_.albedo = 0.8f;
     desc_.env = 0.0f;
+    // Input's defaults (Common/Input.cpp:23-40): shadows on, Phong on (it
+    // needs normals, so it starts off until SetMaterials brings them),
+    // gloss mapping on, environment-mapping flag -1, no bounces
+    wdesc_.width = width;
+    wdesc_.height = height;
+    wdesc_.switches = RT_WHITTED_SHADOWS | RT_WHITTED_GLOSS;
+    wdesc_.env_flag = -1;
+    wdesc_.bounces = 0;
+    wdesc_.seed = 1;
     const float pos[3] = {0.0f, 0.0f, 3.0f}, tgt[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
     SetCamera(pos, tgt, up, 60.0f);
 }
@@ -125,6 +134,43 @@ void RayTracerHip::SetCamera(const float pos[3], const float target[3], const fl
 {
     rt_camera_look_at(&desc_.cam, pos, target, up, fov_y_deg,
                       (float)desc_.width / (float)desc_.height);
+    wdesc_.cam = desc_.cam;
+}
+
+void RayTracerHip::SetMaterials(const float *normals, const uint32_t *mat_ids, const rt_material *materials,
+                                uint32_t nmat)
+{
+    uint32_t nt = 0, nv = 0, depth = 0;
+    dev_.check(rt_scene_info(dev_.ctx(), lbvh_.scene(), &nt, &nv, &depth), "rt_scene_info");
+    dev_.check(rt_scene_shading(dev_.ctx(), lbvh_.scene(), normals, nv, mat_ids, nt, materials, nmat),
+               "rt_scene_shading");
+    if (normals) set_switch(RT_WHITTED_PHONG, true);
+}
+
+void RayTracerHip::SetEnvironment(float r, float g, float b)
+{
+    wdesc_.env[0] = r;
+    wdesc_.env[1] = g;
+    wdesc_.env[2] = b;
+}
+
+void RayTracerHip::SelectAccelerationStructure(int id, uint32_t max_prims, uint32_t split)
+{
+    if (id == RT_STRUCTURE_BVH && !bvh_built_) {
+        rt_bvh_info info;
+        dev_.check(rt_bvh_build(dev_.ctx(), lbvh_.scene(), max_prims, split, &info), "rt_bvh_build");
+        bvh_built_ = true;
+    }
+    dev_.check(rt_scene_select(dev_.ctx(), lbvh_.scene(), id), "rt_scene_select");
+    structure_ = id;
+}
+
+void RayTracerHip::TakeScreenshot(const char *path)
+{
+    std::vector<uint8_t> img(4ull * desc_.width * desc_.height);
+    dev_.check(rt_present(dev_.ctx(), nullptr, desc_.width, desc_.height, 0, img.data()), "rt_present");
+    const int rc = rt_image_write(path, img.data(), desc_.width, desc_.height);
+    if (rc != RT_OK) throw Error(rc, rt_image_last_error());
 }
 
 void RayTracerHip::SetLights(const std::vector<float> &xyz)
@@ -144,10 +190,19 @@ void RayTracerHip::Render()
 {
     // RayTracerCS::Render (RayTracerCS.cpp:258-295): rebuild when dynamic,
     // then primary rays, intersections and shading with shadow rays.
-    if (dynamic_) build_.GPUBuildStructure(lbvh_.depth(), lbvh_);
-    desc_.lights = lights_.data();
-    desc_.nlights = (uint32_t)(lights_.size() / 3);
-    dev_.check(rt_render_direct(dev_.ctx(), lbvh_.scene(), &desc_, 0, &stats_), "rt_render_direct");
+    // The SAH BVH is static (the reference builds it once on the host), so a
+    // dynamic rebuild applies to the LBVH only.
+    if (dynamic_ && structure_ == RT_STRUCTURE_LBVH) build_.GPUBuildStructure(lbvh_.depth(), lbvh_);
+    if (whitted_) {
+        if (lights_.size() < 3) throw Error(RT_ERR_ARG, "Render: the ray-tracing stage needs a light");
+        for (int k = 0; k < 3; ++k) wdesc_.light[k] = lights_[k];
+        wdesc_.frame = samples_;
+        dev_.check(rt_render_whitted(dev_.ctx(), lbvh_.scene(), &wdesc_, 0, &stats_), "rt_render_whitted");
+    } else {
+        desc_.lights = lights_.data();
+        desc_.nlights = (uint32_t)(lights_.size() / 3);
+        dev_.check(rt_render_direct(dev_.ctx(), lbvh_.scene(), &desc_, 0, &stats_), "rt_render_direct");
+    }
     ++samples_;
 }
 

@@ -116,9 +116,11 @@ class StructureConstructionHip {
     rt_build_times times_{};
 };
 
-// RayTracerCS role: one frame = optional rebuild + primary + shadow rays +
-// direct shading (Whitted shading, textures and reflections are not part of
-// this path).
+// RayTracerCS role: one frame = optional rebuild + primary rays + the colour
+// stage.  By default the direct-lighting frame (white Lambert, any number of
+// lights); SetRayTracing(true) switches to the reference's ray-tracing
+// colour stage with materials, one light and bounces (CSComputeColorRT.hlsl),
+// driven by the toggles of its Input class (Common/Input.h:40-72).
 class RayTracerHip {
    public:
     RayTracerHip(Device &dev, LbvhGpu &lbvh, unsigned int width, unsigned int height,
@@ -126,19 +128,42 @@ class RayTracerHip {
     void SetCamera(const float pos[3], const float target[3], const float up[3], float fov_y_deg);
     void SetLights(const std::vector<float> &xyz);
     void SetShading(float ambient, float albedo, float env);
+    // ray-tracing colour stage and its toggles
+    void SetRayTracing(bool on) { whitted_ = on; }
+    void SetMaterials(const float *normals, const uint32_t *mat_ids, const rt_material *materials,
+                      uint32_t nmat);
+    void SetNumBounces(unsigned int n) { wdesc_.bounces = n; }
+    unsigned int GetNumBounces() const { return wdesc_.bounces; }
+    void SetShadows(bool on) { set_switch(RT_WHITTED_SHADOWS, on); }
+    void SetPhongShading(bool on) { set_switch(RT_WHITTED_PHONG, on); }
+    void SetGlossMapping(bool on) { set_switch(RT_WHITTED_GLOSS, on); }
+    void ChangeEnvMappingFlag() { wdesc_.env_flag = wdesc_.env_flag == 0 ? -1 : 0; }
+    int GetEnvMappingFlag() const { return wdesc_.env_flag; }
+    void SetEnvironment(float r, float g, float b);
+    // acceleration structure switch (g_iAccelerationStructure: 0 BVH, 9 LBVH);
+    // 0 builds the SAH BVH on first use (bvh_max_prims_node, bvh_split_algorithm)
+    void SelectAccelerationStructure(int id, uint32_t max_prims = 1, uint32_t split = RT_BVH_SAH);
+    int GetAccelerationStructure() const { return structure_; }
     void Render();
     void ReadFrame(std::vector<float> &rgba);
+    // RendererDX11::TakeScreenshot: the UNORM8 frame as PNG or PPM
+    void TakeScreenshot(const char *path);
     unsigned int Samples() const { return samples_; }
     const rt_frame_stats &LastStats() const { return stats_; }
     const rt_build_times &LastBuildTimes() const { return build_.LastTimes(); }
 
    private:
+    void set_switch(uint32_t bit, bool on) { wdesc_.switches = on ? (wdesc_.switches | bit) : (wdesc_.switches & ~bit); }
     Device &dev_;
     LbvhGpu &lbvh_;
     StructureConstructionHip build_;
     rt_render_desc desc_{};
+    rt_whitted_desc wdesc_{};
     std::vector<float> lights_;
     bool dynamic_;
+    bool whitted_ = false;
+    bool bvh_built_ = false;
+    int structure_ = RT_STRUCTURE_LBVH;
     unsigned int samples_ = 0;
     rt_frame_stats stats_{};
 };

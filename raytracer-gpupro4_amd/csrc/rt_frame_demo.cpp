@@ -34,6 +34,17 @@ int main(int argc, char **argv)
         rt.ReadFrame(rgba);
         double sum = 0.0;
         for (size_t i = 0; i < rgba.size(); i += 4) sum += rgba[i];
+        // the ray-tracing colour stage with 2 bounces on the SAH BVH, then a
+        // screenshot (argv[2], optional)
+        rt.SetRayTracing(true);
+        rt.SetNumBounces(2);
+        rt.SetEnvironment(0.3f, 0.4f, 0.5f);
+        rt.SelectAccelerationStructure(RT_STRUCTURE_BVH);
+        rt.Render();
+        const uint64_t whitted_rays =
+            rt.LastStats().primary_rays + rt.LastStats().shadow_rays + rt.LastStats().bounce_rays;
+        if (argc > 2) rt.TakeScreenshot(argv[2]);
+        rt.SelectAccelerationStructure(RT_STRUCTURE_LBVH);
         // one ray straight down the +z axis at the torus tube (x = 1)
         rtgpu::Ray r{};
         r.o[0] = 1.0f;
@@ -47,11 +58,11 @@ int main(int argc, char **argv)
         printf("{\"frames\": %u, \"samples\": %u, \"rays\": %llu, \"trace_ms\": %.4f, "
                "\"build_ms\": %.4f, \"image_mean\": %.6f, \"probe_tri\": %d, \"probe_t\": %.6f, "
                "\"root_box\": [%.4f, %.4f, %.4f, %.4f, %.4f, %.4f], \"keyboard_id\": %u, "
-               "\"name\": \"%s\"}\n",
+               "\"name\": \"%s\", \"whitted_rays\": %llu}\n",
                frames, rt.Samples(), (unsigned long long)rays, trace_ms,
                rt.LastBuildTimes().total_ms, sum / (rgba.size() / 4), h.IDTr, h.t, wb.pMin[0],
                wb.pMin[1], wb.pMin[2], wb.pMax[0], wb.pMax[1], wb.pMax[2], lbvh.GetKeyboardId(),
-               lbvh.GetName());
+               lbvh.GetName(), (unsigned long long)whitted_rays);
         return 0;
     } catch (const rtgpu::Error &e) {
         fprintf(stderr, "rt_frame_demo: %s (code %d)\n", e.what(), e.code());

@@ -137,18 +137,23 @@ def test_render_direct_parity(ctx, oracle, rtcore, torus, soup262k, scene, cam_p
     assert st["shadow_rays"] == int((rocc[:, :len(light)] != 2).sum())
 
 
-def test_facade_demo_binary(rtcore):
+def test_facade_demo_binary(rtcore, tmp_path):
+    """The C++ facades end to end: direct frames with rebuilds, a Whitted
+    frame of 2 bounces on the SAH BVH, a PNG screenshot, one IntersectP."""
     import json
     import os
     import subprocess
 
     exe = os.path.join(os.path.dirname(rtcore.LIB_PATH), "rt_frame_demo")
-    out = subprocess.run([exe, "3"], capture_output=True, text=True, timeout=120)
+    shot = tmp_path / "shot.png"
+    out = subprocess.run([exe, "3", str(shot)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     rec = json.loads(out.stdout.strip().splitlines()[-1])
-    assert rec["samples"] == 3 and rec["keyboard_id"] == 9 and rec["name"] == "LBVH"
+    assert rec["samples"] == 4 and rec["keyboard_id"] == 9 and rec["name"] == "LBVH"
     assert rec["probe_tri"] >= 0 and 2.0 < rec["probe_t"] < 3.0
     assert rec["rays"] >= 3 * 512 * 512
+    assert rec["whitted_rays"] > 512 * 512
+    assert shot.read_bytes()[:8] == b"\x89PNG\r\n\x1a\n"
 
 
 @pytest.mark.parametrize("form", [0, 1])
