@@ -26,6 +26,9 @@ Other workloads (--workload), each printing one JSON line of the same shape:
            the reference builds it) and through the LBVH: direct frames at
            1920x1080; value = Mrays/s on the BVH, with the LBVH's rate, both
            structures' SAH costs and build times beside it.
+  build    LBVH rebuilds of triangle soups of 262,144 to 33,554,432 triangles:
+           per-stage device times, Mtris/s, and the onesweep sort's achieved
+           GB/s (4 passes x (8 B read + 8 B written) per key) against 8 TB/s.
   serve    Poisson 200 requests/s for 60 s over a pool of 500 scenes
            (1k..500k triangles), 512^2 or 1920x1080, direct or 2-bounce GI at
            1 spp with progressive accumulation, 200 GB device budget; p50/p99
@@ -377,6 +380,45 @@ def workload_whitted(args, rtcore):
     return out
 
 
+# ------------------------------------------------------------------ build
+BUILD_SIZES = (262144, 2097152, 8388608, 33554432)
+
+
+def workload_build(args, rtcore):
+    sizes = [args.tris] if args.tris else list(BUILD_SIZES)
+    ctx = rtcore.Context(0)
+    rows = []
+    for nt in sizes:
+        xyz, idx = rtcore.gen_triangle_soup(nt, SIGMA * (262144 / nt) ** (1 / 3), SEED)
+        depth = depth_for(nt)
+        s = ctx.add_scene(xyz, idx, depth)
+        del xyz, idx
+        for _ in range(max(args.warmup, 1)):
+            ctx.build(s)
+        ctx.sync()
+        times = [ctx.build(s, timed=True) for _ in range(args.steps)]
+        med = {k: float(np.median([t[k] for t in times])) for k in times[0]}
+        sort_bytes = 4 * 16 * nt
+        rows.append({"tris": nt, "depth": depth, "build_ms": round(med["total_ms"], 4),
+                     "stages_ms": {k.replace("_ms", ""): round(v, 4) for k, v in med.items() if k != "total_ms"},
+                     "mtris_s": round(nt / (med["total_ms"] * 1e-3) / 1e6, 1),
+                     "sort_gb_s": round(sort_bytes / (med["sort_ms"] * 1e-3) / 1e9, 1),
+                     "sort_frac": round(sort_bytes / (med["sort_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)})
+        log(f"build {nt}: {rows[-1]}")
+        ctx.scene_destroy(s)
+    ctx.close()
+    first, last = rows[0], rows[-1]
+    return result(METRIC, first["build_ms"], "ms", args.steps, args.warmup, first["build_ms"] * args.steps * 1e-3,
+                  {"workload": "LBVH rebuild of seeded triangle soups (edges scaled to keep density), "
+                               + ", ".join(str(r["tris"]) for r in rows) + " triangles",
+                   "tris": [r["tris"] for r in rows]},
+                  higher_is_better=False, builds=rows,
+                  roofline={"bound": "hbm", "achieved": last["sort_gb_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": last["sort_frac"], "traffic": None, "kernel": "k_onesweep",
+                            "bytes_per_launch": 16 * last["tris"], "tris": last["tris"]},
+                  cpu_baseline=None)
+
+
 # ------------------------------------------------------------------ bvh
 def sah_cost(nodes, heap):
     """Sum over reachable nodes of area / root area x (0.125 inner, primitive
@@ -722,8 +764,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["frame", "sphere", "whitted", "bvh", "batched", "serve"],
+    ap.add_argument("--workload", choices=["frame", "sphere", "whitted", "bvh", "build", "batched", "serve"],
                     default="frame")
+    ap.add_argument("--tris", type=int, default=0, help="build: one soup size instead of the list")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--plain-seconds", type=float, default=10.0,
                     help="serve: seconds of the same arrivals replayed one request per launch")
@@ -738,8 +781,8 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)
         sys.exit(2)
-    defaults = {"frame": (100, 5), "sphere": (10, 1), "whitted": (20, 2), "bvh": (50, 3), "batched": (3, 1),
-                "serve": (0, 0)}
+    defaults = {"frame": (100, 5), "sphere": (10, 1), "whitted": (20, 2), "bvh": (50, 3), "build": (20, 2),
+                "batched": (3, 1), "serve": (0, 0)}
     ds, dw = defaults[args.workload]
     args.steps = ds if args.steps is None else args.steps
     args.warmup = dw if args.warmup is None else args.warmup
@@ -752,7 +795,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     rtcore.lib()
     fn = {"frame": workload_frame, "sphere": workload_sphere, "whitted": workload_whitted, "bvh": workload_bvh,
-          "batched": workload_batched, "serve": workload_serve}[args.workload]
+          "build": workload_build, "batched": workload_batched, "serve": workload_serve}[args.workload]
     print(json.dumps(fn(args, rtcore)), flush=True)
 
 

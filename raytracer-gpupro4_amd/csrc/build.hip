@@ -325,7 +325,7 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 
-template <bool SCENE_PASS>
+template <bool SCENE_PASS, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin, uint32_t *__restrict__ kout,
     int32_t *__restrict__ vout, uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
@@ -338,8 +338,8 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     __shared__ uint32_t s_excl[kRadix];
     __shared__ uint32_t s_gofs[kRadix];
     __shared__ uint32_t s_scan[kWaves];
-    __shared__ uint32_t s_keys[kSortTile];
-    __shared__ int32_t s_vals[kSortTile];
+    __shared__ uint32_t s_keys[kSortThreads * ITEMS];
+    __shared__ int32_t s_vals[kSortThreads * ITEMS];
     __shared__ uint32_t s_soff[SCENE_PASS ? kRadix : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -349,20 +349,20 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         for (int j = tid; j < kRadix; j += kSortThreads) s_soff[j] = j < (int)S ? desc[j].prim_off : 0xFFFFFFFFu;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t base = tile * (uint32_t)kSortTile;
+    const uint32_t base = tile * (uint32_t)(kSortThreads * ITEMS);
     auto digit_of = [&](uint32_t key, int32_t val) -> uint32_t {
         if (SCENE_PASS) return upper_scene(S, (uint32_t)val, [&](uint32_t i) { return s_soff[i]; });
         return (key >> shift) & (kRadix - 1);
     };
 
-    uint32_t key[kSortItems];
-    int32_t val[kSortItems];
-    uint32_t rank[kSortItems];
-    uint32_t dig[kSortItems];
-    bool valid[kSortItems];
+    uint32_t key[ITEMS];
+    int32_t val[ITEMS];
+    uint32_t rank[ITEMS];
+    uint32_t dig[ITEMS];
+    bool valid[ITEMS];
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        const uint32_t i = base + wave * (64 * kSortItems) + r * 64 + lane;
+    for (int r = 0; r < ITEMS; ++r) {
+        const uint32_t i = base + wave * (64 * ITEMS) + r * 64 + lane;
         valid[r] = i < n;
         key[r] = valid[r] ? kin[i] : 0u;
         val[r] = valid[r] ? vin[i] : 0;
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     }
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         const uint32_t d = dig[r];
         uint64_t peers = __ballot(valid[r]);
 #pragma unroll
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     __syncthreads();
 
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         if (valid[r]) {
             const uint32_t d = dig[r];
             const uint32_t lp = s_excl[d] + s_wave[wave][d] + rank[r];
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         }
     }
     __syncthreads();
-    const uint32_t tile_n = (n - base) < (uint32_t)kSortTile ? (n - base) : (uint32_t)kSortTile;
+    const uint32_t tile_n = (n - base) < (uint32_t)(kSortThreads * ITEMS) ? (n - base) : (uint32_t)(kSortThreads * ITEMS);
     for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
         const uint32_t k = s_keys[i];
         const int32_t v = s_vals[i];
@@ -919,14 +919,20 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
             const int src = (start + p) & 1, dst = src ^ 1;
             uint32_t *status = b.status + (size_t)p * tiles * kRadix;
             uint32_t *tc = b.counters + kCntTiles + p;
-            if (p < (uint32_t)kCodePasses)
-                hipLaunchKernelGGL((k_onesweep<false>), dim3(tiles), dim3(kSortThreads), 0, st,
-                                   b.keys[src], b.vals[src], b.keys[dst], b.vals[dst], n,
-                                   p * kRadixBits, b.hist + p * kRadix, b.d_desc, S, status, tc);
-            else
-                hipLaunchKernelGGL((k_onesweep<true>), dim3(tiles), dim3(kSortThreads), 0, st,
-                                   b.keys[src], b.vals[src], b.keys[dst], b.vals[dst], n, 0u,
-                                   b.hist, b.d_desc, S, status, tc);
+            const bool large = n >= kSortLargeN;
+            const uint32_t lt = large ? (n + kSortThreads * kSortItemsLarge - 1) / (kSortThreads * kSortItemsLarge)
+                                      : tiles;
+#define RT_SORT(SC, IT, SHIFT, HIST)                                                                      \
+    hipLaunchKernelGGL((k_onesweep<SC, IT>), dim3(lt), dim3(kSortThreads), 0, st, b.keys[src], b.vals[src], \
+                       b.keys[dst], b.vals[dst], n, SHIFT, HIST, b.d_desc, S, status, tc)
+            if (p < (uint32_t)kCodePasses) {
+                if (large) RT_SORT(false, kSortItemsLarge, p * kRadixBits, b.hist + p * kRadix);
+                else RT_SORT(false, kSortItemsSmall, p * kRadixBits, b.hist + p * kRadix);
+            } else {
+                if (large) RT_SORT(true, kSortItemsLarge, 0u, b.hist);
+                else RT_SORT(true, kSortItemsSmall, 0u, b.hist);
+            }
+#undef RT_SORT
         }
     }
     if (ev) hipEventRecord(ev[3], st);

@@ -16,9 +16,14 @@ namespace rt {
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kCodePasses = 4;
+// Onesweep tiles of 256 threads x ITEMS keys: 8 keys per thread below
+// kSortLargeN keys (enough tiles to fill the chip), 32 above (long per-digit
+// runs, so the scatter writes coalesce).  Scratch is sized for the smaller tile.
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 4;
-constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kSortItemsSmall = 8;
+constexpr int kSortItemsLarge = 32;
+constexpr uint32_t kSortLargeN = 1u << 20;
+constexpr int kSortTile = kSortThreads * kSortItemsSmall;
 constexpr int kMaxBatchScenes = kRadix;  // the scene pass is one 8-bit digit
 
 // Bounds / Morton kernels: 256 threads x 4 triangles per block.

@@ -233,3 +233,24 @@ def test_build_loaded_obj_model(ctx, oracle, rtcore, tmp_path):
     assert len(mi) == len(idx)
     assert abs(np.sqrt((mx.astype(np.float64) ** 2).sum(1)).max() - 0.49) < 1e-6
     check_build(ctx, oracle, mx, mi, 18, check_partition_mode=True)
+
+
+def test_batch_above_the_large_tile_threshold(ctx, oracle, rtcore):
+    """A batch of 1.2 M triangles sorts with the large tiles, scene pass
+    included; every scene's codes, ids and reachable nodes stay bit-exact."""
+    sizes = [600011, 499999, 100003, 7]
+    depths = [21, 21, 18, 4]
+    scenes = [soup(n, sigma=0.005, seed=300 + i) for i, n in enumerate(sizes)]
+    handles = ctx.batch_create(sizes, [len(x) for x, _ in scenes], depths)
+    for h, (xyz, idx) in zip(handles, scenes):
+        ctx.scene_upload(h, xyz, idx)
+    ctx.build(handles[0])
+    for h, (xyz, idx), d in zip(handles, scenes, depths):
+        nodes, codes, ids, box = ctx.read(h)
+        ref = oracle.build_scene(xyz, idx, d)
+        assert np.array_equal(codes, ref["codes"]) and np.array_equal(ids, ref["ids"]), len(idx)
+        reach = oracle.reachable(ref["nodes"])
+        assert np.array_equal(nodes[reach]["prim_count"], ref["nodes"][reach]["prim_count"])
+        assert np.array_equal(nodes[reach]["prim_pos"], ref["nodes"][reach]["prim_pos"])
+    for h in handles:
+        ctx.scene_destroy(h)
