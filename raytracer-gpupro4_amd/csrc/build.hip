@@ -649,7 +649,7 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
                     write_heap_word(&sheap[2 * h], kInvalid, kInvalid);
                     write_heap_word(&sheap[2 * h + 1], kInvalid, kInvalid);
                 }
-                const int32_t ref = (int32_t)(kLeafBit | rfirst);
+                const int32_t ref = (int32_t)leaf_ref(rfirst, (uint32_t)rcnt);
                 if (rpr < 0) {
                     hdr[s].root_ref = ref;
                     for (int k = 0; k < 3; ++k) {
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
                         L.cbox[ploc][side * 6 + k] = lo[k];
                         L.cbox[ploc][side * 6 + 3 + k] = hi[k];
                     }
-                    L.cref[ploc][side] = (int32_t)(kLeafBit | first);
+                    L.cref[ploc][side] = (int32_t)leaf_ref(first, (uint32_t)cnt);
                     continue;
                 }
                 const uint32_t X = ((L.codes[lf] >> p) | 1u) << p;

@@ -327,7 +327,7 @@ void bvh_device_layout(const float *xyz, const uint32_t *idx, const BvhHost &h, 
         if (h.nodes[i].prim_count <= 0) inner[i] = k++;
     auto ref_of = [&](uint32_t i) -> int32_t {
         const rt_node32 &n = h.nodes[i];
-        return n.prim_count > 0 ? (int32_t)(kLeafBit | (uint32_t)n.prim_pos) : inner[i];
+        return n.prim_count > 0 ? (int32_t)leaf_ref((uint32_t)n.prim_pos, (uint32_t)n.prim_count) : inner[i];
     };
     trav.assign(k > 0 ? (size_t)k : 1, TravNode{});
     for (size_t i = 0; i < nn; ++i) {

@@ -80,6 +80,78 @@ __device__ __forceinline__ void box_test(const float *bx, float ox, float oy, fl
     t1 = b;
 }
 
+// One triangle test: getIntersection (Intersection.hlsl:47-64) and the
+// acceptance rule of LBVH_IntersectP (LBVH.hlsl:400-420).  Returns true when
+// the triangle becomes the current hit.
+__device__ __forceinline__ bool tri_test(TraceOut &r, const float4 a, const float4 e1, const float4 e2, float ox,
+                                         float oy, float oz, float dx, float dy, float dz, int32_t self,
+                                         int32_t heap, int32_t pos)
+{
+    r.tris += 1;
+    const int32_t id = __float_as_int(a.w);
+    const float px = (dy * e2.z) - (dz * e2.y);
+    const float py = (dz * e2.x) - (dx * e2.z);
+    const float pz = (dx * e2.y) - (dy * e2.x);
+    const float dd = (e1.x * px + e1.y * py) + e1.z * pz;
+    const float det = 1.0f / dd;
+    const float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
+    const float u = ((tx * px + ty * py) + tz * pz) * det;
+    const float qx = (ty * e1.z) - (tz * e1.y);
+    const float qy = (tz * e1.x) - (tx * e1.z);
+    const float qz = (tx * e1.y) - (ty * e1.x);
+    const float v = ((dx * qx + dy * qy) + dz * qz) * det;
+    const float t = ((e2.x * qx + e2.y * qy) + e2.z * qz) * det;
+    if (id != self && u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f && t >= 0.0f && t < r.t) {
+        r.tri = id;
+        r.u = u;
+        r.v = v;
+        r.t = t;
+        r.leaf_heap = heap;
+        r.pos = pos;
+        return true;
+    }
+    return false;
+}
+
+// The triangles of a leaf, in order.  The count hint in the reference lets
+// the first two triangles' loads go out together; later ones go in pairs.
+// Returns true when an any-hit query accepted a triangle.
+__device__ __forceinline__ bool leaf_test(TraceOut &r, const TravTri *__restrict__ tris, uint32_t ref, float ox,
+                                          float oy, float oz, float dx, float dy, float dz, int32_t self,
+                                          bool anyhit)
+{
+    const uint32_t pos = ref & kLeafPosMask;
+    const uint32_t hint = (ref >> kLeafCountShift) & 3u;
+    const TravTri *tp = &tris[pos];
+    float4 a0 = tp[0].a, b0 = tp[0].e1, c0 = tp[0].e2;
+    float4 a1 = a0, b1 = b0, c1 = c0;
+    if (hint != 1u) {
+        a1 = tp[1].a;
+        b1 = tp[1].e1;
+        c1 = tp[1].e2;
+    }
+    const int32_t cnt = hint == 3u ? __float_as_int(b0.w) : (int32_t)hint;
+    const int32_t heap = __float_as_int(c0.w);
+    if (tri_test(r, a0, b0, c0, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos) && anyhit) return true;
+    if (cnt > 1 && tri_test(r, a1, b1, c1, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + 1) && anyhit)
+        return true;
+    for (int32_t j = 2; j < cnt; j += 2) {
+        a0 = tp[j].a;
+        b0 = tp[j].e1;
+        c0 = tp[j].e2;
+        const bool two = j + 1 < cnt;
+        if (two) {
+            a1 = tp[j + 1].a;
+            b1 = tp[j + 1].e1;
+            c1 = tp[j + 1].e2;
+        }
+        if (tri_test(r, a0, b0, c0, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + j) && anyhit) return true;
+        if (two && tri_test(r, a1, b1, c1, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + j + 1) && anyhit)
+            return true;
+    }
+    return false;
+}
+
 template <bool ANYHIT, class Stack>
 __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
                                              const TravTri *__restrict__ tris, int32_t root_ref,
@@ -135,40 +207,7 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
                 continue;
             }
         } else {
-            const uint32_t pos = (uint32_t)cur & ~kLeafBit;
-            const TravTri *tp = &tris[pos];
-            int32_t cnt = 1;
-            for (int32_t j = 0; j < cnt; ++j) {
-                const float4 a = tp[j].a;
-                const float4 e1 = tp[j].e1;
-                const float4 e2 = tp[j].e2;
-                if (j == 0) cnt = __float_as_int(e1.w);
-                r.tris += 1;
-                const int32_t id = __float_as_int(a.w);
-                // getIntersection (Intersection.hlsl:47-64)
-                const float px = (dy * e2.z) - (dz * e2.y);
-                const float py = (dz * e2.x) - (dx * e2.z);
-                const float pz = (dx * e2.y) - (dy * e2.x);
-                const float dd = (e1.x * px + e1.y * py) + e1.z * pz;
-                const float det = 1.0f / dd;
-                const float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
-                const float u = ((tx * px + ty * py) + tz * pz) * det;
-                const float qx = (ty * e1.z) - (tz * e1.y);
-                const float qy = (tz * e1.x) - (tx * e1.z);
-                const float qz = (tx * e1.y) - (ty * e1.x);
-                const float v = ((dx * qx + dy * qy) + dz * qz) * det;
-                const float t = ((e2.x * qx + e2.y * qy) + e2.z * qz) * det;
-                if (id != self && u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f && t >= 0.0f &&
-                    t < r.t) {
-                    r.tri = id;
-                    r.u = u;
-                    r.v = v;
-                    r.t = t;
-                    r.leaf_heap = __float_as_int(tp[0].e2.w);
-                    r.pos = (int32_t)pos + j;
-                    if (ANYHIT) return r;
-                }
-            }
+            if (leaf_test(r, tris, (uint32_t)cur, ox, oy, oz, dx, dy, dz, self, ANYHIT)) return r;
         }
         // pop the next live entry
         bool found = false;
@@ -271,40 +310,9 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
             return false;
         }
     } else {
-        const uint32_t pos = (uint32_t)T.cur & ~kLeafBit;
-        const TravTri *tp = &tris[pos];
-        const float dx = T.dx, dy = T.dy, dz = T.dz;
-        int32_t cnt = 1;
-        for (int32_t j = 0; j < cnt; ++j) {
-            const float4 a = tp[j].a;
-            const float4 e1 = tp[j].e1;
-            const float4 e2 = tp[j].e2;
-            if (j == 0) cnt = __float_as_int(e1.w);
-            r.tris += 1;
-            const int32_t id = __float_as_int(a.w);
-            // getIntersection (Intersection.hlsl:47-64)
-            const float px = (dy * e2.z) - (dz * e2.y);
-            const float py = (dz * e2.x) - (dx * e2.z);
-            const float pz = (dx * e2.y) - (dy * e2.x);
-            const float dd = (e1.x * px + e1.y * py) + e1.z * pz;
-            const float det = 1.0f / dd;
-            const float tx = T.ox - a.x, ty = T.oy - a.y, tz = T.oz - a.z;
-            const float u = ((tx * px + ty * py) + tz * pz) * det;
-            const float qx = (ty * e1.z) - (tz * e1.y);
-            const float qy = (tz * e1.x) - (tx * e1.z);
-            const float qz = (tx * e1.y) - (ty * e1.x);
-            const float v = ((dx * qx + dy * qy) + dz * qz) * det;
-            const float t = ((e2.x * qx + e2.y * qy) + e2.z * qz) * det;
-            if (id != T.self && u >= 0.0f && v >= 0.0f && (u + v) <= 1.0f && t >= 0.0f && t < r.t) {
-                r.tri = id;
-                r.u = u;
-                r.v = v;
-                r.t = t;
-                r.leaf_heap = __float_as_int(tp[0].e2.w);
-                r.pos = (int32_t)pos + j;
-                if (MODE == 1 || (MODE == 2 && T.anyhit)) return true;
-            }
-        }
+        if (leaf_test(r, tris, (uint32_t)T.cur, T.ox, T.oy, T.oz, T.dx, T.dy, T.dz, T.self,
+                      MODE == 1 || (MODE == 2 && T.anyhit)))
+            return true;
     }
     // pop the next live entry
     while (stk.sp > 0) {
