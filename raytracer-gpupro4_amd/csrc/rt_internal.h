@@ -39,16 +39,17 @@ constexpr int kSubtreeThreads = 256;
 constexpr int kTopThreads = 1024;
 constexpr int kMaxLevels = 32;
 
-// Leaf references in traversal nodes carry bit 31; bits 29-30 hold the leaf's
-// triangle count capped at 3 (so a traversal can fetch a leaf's first two
-// triangles together); the low 29 bits are the first sorted primitive of the
-// leaf (a position in the batch's sorted arrays, below 2^25).
+// Leaf references in traversal nodes carry bit 31; bits 28-30 hold the leaf's
+// triangle count capped at 7 (so a traversal can fetch a leaf's triangles
+// without first reading the count); the low 28 bits are the first sorted
+// primitive of the leaf (a position in the batch's sorted arrays, below 2^25).
 constexpr uint32_t kLeafBit = 0x80000000u;
-constexpr uint32_t kLeafPosMask = 0x1FFFFFFFu;
-constexpr int kLeafCountShift = 29;
+constexpr uint32_t kLeafPosMask = 0x0FFFFFFFu;
+constexpr int kLeafCountShift = 28;
+constexpr uint32_t kLeafCountCap = 7u;
 __host__ __device__ constexpr uint32_t leaf_ref(uint32_t first, uint32_t count)
 {
-    return kLeafBit | ((count < 3u ? count : 3u) << kLeafCountShift) | first;
+    return kLeafBit | ((count < kLeafCountCap ? count : kLeafCountCap) << kLeafCountShift) | first;
 }
 
 constexpr int32_t kInvalid = -2147483647;

@@ -114,40 +114,40 @@ __device__ __forceinline__ bool tri_test(TraceOut &r, const float4 a, const floa
 }
 
 // The triangles of a leaf, in order.  The count hint in the reference lets
-// the first two triangles' loads go out together; later ones go in pairs.
-// Returns true when an any-hit query accepted a triangle.
+// the loads of up to four triangles go out together, before any test; longer
+// leaves continue in chunks of four.  Returns true when an any-hit query
+// accepted a triangle.
 __device__ __forceinline__ bool leaf_test(TraceOut &r, const TravTri *__restrict__ tris, uint32_t ref, float ox,
                                           float oy, float oz, float dx, float dy, float dz, int32_t self,
                                           bool anyhit)
 {
+    constexpr int kChunk = 4;
     const uint32_t pos = ref & kLeafPosMask;
-    const uint32_t hint = (ref >> kLeafCountShift) & 3u;
+    const uint32_t hint = (ref >> kLeafCountShift) & 7u;
     const TravTri *tp = &tris[pos];
-    float4 a0 = tp[0].a, b0 = tp[0].e1, c0 = tp[0].e2;
-    float4 a1 = a0, b1 = b0, c1 = c0;
-    if (hint != 1u) {
-        a1 = tp[1].a;
-        b1 = tp[1].e1;
-        c1 = tp[1].e2;
-    }
-    const int32_t cnt = hint == 3u ? __float_as_int(b0.w) : (int32_t)hint;
-    const int32_t heap = __float_as_int(c0.w);
-    if (tri_test(r, a0, b0, c0, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos) && anyhit) return true;
-    if (cnt > 1 && tri_test(r, a1, b1, c1, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + 1) && anyhit)
-        return true;
-    for (int32_t j = 2; j < cnt; j += 2) {
-        a0 = tp[j].a;
-        b0 = tp[j].e1;
-        c0 = tp[j].e2;
-        const bool two = j + 1 < cnt;
-        if (two) {
-            a1 = tp[j + 1].a;
-            b1 = tp[j + 1].e1;
-            c1 = tp[j + 1].e2;
+    int32_t cnt = (int32_t)hint;  // exact below the cap; the cap reads the count from the first record
+    int32_t heap = 0;
+    for (int32_t j = 0; j < cnt; j += kChunk) {
+        const int32_t m = cnt - j < kChunk ? cnt - j : kChunk;
+        float4 A[kChunk], B[kChunk], C[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            if (k < m) {
+                A[k] = tp[j + k].a;
+                B[k] = tp[j + k].e1;
+                C[k] = tp[j + k].e2;
+            }
         }
-        if (tri_test(r, a0, b0, c0, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + j) && anyhit) return true;
-        if (two && tri_test(r, a1, b1, c1, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + j + 1) && anyhit)
-            return true;
+        if (j == 0) {
+            heap = __float_as_int(C[0].w);
+            if (hint == kLeafCountCap) cnt = __float_as_int(B[0].w);
+        }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            if (k < m && tri_test(r, A[k], B[k], C[k], ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + j + k) &&
+                anyhit)
+                return true;
+        }
     }
     return false;
 }
