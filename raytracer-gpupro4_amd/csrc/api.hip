@@ -171,7 +171,7 @@ uint64_t scene_bytes(uint32_t max_tris, uint32_t max_verts, uint32_t depth)
     const uint64_t tiles = sort_tiles((uint32_t)nt);
     const uint64_t words = kCodePasses * kRadix + kCounters + (uint64_t)kCodePasses * tiles * kRadix;
     const uint64_t table = ((uint64_t)1 << table_bits(max_tris)) + 1;
-    return 12 * nv + 12 * nt + sizeof(SceneDesc) + sizeof(SceneHdr) + sizeof(PrimBox) * nt + 12 * nt +
+    return 12 * nv + 12 * nt + sizeof(SceneDesc) + sizeof(SceneHdr) + sizeof(TriRec) * nt + 12 * nt +
            24 * (uint64_t)bounds_blocks((uint32_t)nt) + 16 * nt + 4 * words + 4 * table + 16 * 4 * nt +
            4 * kMaxLevels + sizeof(rt_node32) * ((uint64_t)1 << depth) + sizeof(TravNode) * nt +
            sizeof(TravTri) * nt;
@@ -239,7 +239,7 @@ int create_batch(rt_ctx *ctx, uint32_t n, const uint32_t *max_tris, const uint32
     ALLOC(b->idx, 3 * (size_t)nt);
     ALLOC(b->d_desc, n);
     ALLOC(b->hdr, n);
-    ALLOC(b->pbox, nt);
+    ALLOC(b->trec, nt);
     ALLOC(b->cent, 3 * (size_t)nt);
     ALLOC(b->partials, 6 * (size_t)(blocks ? blocks : 1));
     ALLOC(b->keys[0], nt);

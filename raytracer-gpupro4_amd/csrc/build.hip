@@ -166,30 +166,21 @@ __device__ __forceinline__ void write_heap_word(rt_node32 *n, int32_t prim_count
 }
 
 // Box and sorted-order triangle record of one leaf primitive.
-__device__ __forceinline__ void leaf_prim(const SceneDesc &D, const int32_t *__restrict__ vals,
-                                          const PrimBox *__restrict__ pbox,
-                                          const float *__restrict__ xyz,
-                                          const uint32_t *__restrict__ idx, TravTri *__restrict__ tris,
-                                          uint32_t pos, int32_t j, int32_t cnt, uint32_t h,
-                                          float lo[3], float hi[3])
+__device__ __forceinline__ void leaf_prim(const int32_t *__restrict__ vals, const TriRec *__restrict__ trec,
+                                          TravTri *__restrict__ tris, uint32_t pos, int32_t j, int32_t cnt,
+                                          uint32_t h, float lo[3], float hi[3])
 {
-    const int32_t g = vals[pos];
-    const PrimBox b = pbox[g];
+    const TriRec T = trec[vals[pos]];
+    const float A[3] = {T.a.x, T.a.y, T.a.z}, B[3] = {T.b.x, T.b.y, T.b.z}, C[3] = {T.c.x, T.c.y, T.c.z};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = fminf(lo[k], b.mn[k]);
-        hi[k] = fmaxf(hi[k], b.mx[k]);
+    for (int k = 0; k < 3; ++k) {  // the triangle's box, as the bounds pass computed it
+        lo[k] = fminf(lo[k], fminf(A[k], fminf(B[k], C[k])));
+        hi[k] = fmaxf(hi[k], fmaxf(A[k], fmaxf(B[k], C[k])));
     }
-    const uint32_t local = (uint32_t)g - D.prim_off;
-    const uint32_t *tri = idx + 3 * ((size_t)D.in_prim_off + local);
-    const float *va = xyz + 3 * ((size_t)D.vert_off + tri[0]);
-    const float *vb = xyz + 3 * ((size_t)D.vert_off + tri[1]);
-    const float *vc = xyz + 3 * ((size_t)D.vert_off + tri[2]);
-    const float ax = va[0], ay = va[1], az = va[2];
     TravTri t;
-    t.a = make_float4(ax, ay, az, __int_as_float((int32_t)local));
-    t.e1 = make_float4(vb[0] - ax, vb[1] - ay, vb[2] - az, __int_as_float(j == 0 ? cnt : 0));
-    t.e2 = make_float4(vc[0] - ax, vc[1] - ay, vc[2] - az, __int_as_float(j == 0 ? (int32_t)h : 0));
+    t.a = T.a;
+    t.e1 = make_float4(B[0] - A[0], B[1] - A[1], B[2] - A[2], __int_as_float(j == 0 ? cnt : 0));
+    t.e2 = make_float4(C[0] - A[0], C[1] - A[1], C[2] - A[2], __int_as_float(j == 0 ? (int32_t)h : 0));
     tris[pos] = t;
 }
 
@@ -199,7 +190,7 @@ __device__ __forceinline__ void leaf_prim(const SceneDesc &D, const int32_t *__r
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBoundsThreads) void k_bounds(
     const SceneDesc *__restrict__ desc, uint32_t S, uint32_t total_blocks,
-    const float *__restrict__ xyz, const uint32_t *__restrict__ idx, PrimBox *__restrict__ pbox,
+    const float *__restrict__ xyz, const uint32_t *__restrict__ idx, TriRec *__restrict__ trec,
     float *__restrict__ cent, float *__restrict__ partials, uint32_t *__restrict__ zero_words,
     uint32_t nzero)
 {
@@ -222,18 +213,19 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(
             const float *vb = xyz + 3 * ((size_t)D.vert_off + tri[1]);
             const float *vc = xyz + 3 * ((size_t)D.vert_off + tri[2]);
             const size_t g = (size_t)D.prim_off + i;
-            PrimBox b;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const float a = va[k], bb = vb[k], c = vc[k];
-                b.mn[k] = fminf(a, fminf(bb, c));
-                b.mx[k] = fmaxf(a, fmaxf(bb, c));
                 const float sum = a + bb;
                 cent[3 * g + k] = sum + c;
-                lo[k] = fminf(lo[k], b.mn[k]);
-                hi[k] = fmaxf(hi[k], b.mx[k]);
+                lo[k] = fminf(lo[k], fminf(a, fminf(bb, c)));
+                hi[k] = fmaxf(hi[k], fmaxf(a, fmaxf(bb, c)));
             }
-            pbox[g] = b;
+            TriRec t;
+            t.a = make_float4(va[0], va[1], va[2], __int_as_float((int32_t)i));
+            t.b = make_float4(vb[0], vb[1], vb[2], 0.0f);
+            t.c = make_float4(vc[0], vc[1], vc[2], 0.0f);
+            trec[g] = t;
         }
     }
     block_box_reduce<4>(lo, hi, red);
@@ -610,8 +602,7 @@ struct SubtreeLds {
 __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
     const SceneDesc *__restrict__ desc, uint32_t S, SceneHdr *__restrict__ hdr,
     const uint32_t *__restrict__ codes, const int32_t *__restrict__ vals,
-    const PrimBox *__restrict__ pbox, const float *__restrict__ xyz,
-    const uint32_t *__restrict__ idx, const int4 *__restrict__ small, uint32_t *__restrict__ counters,
+    const TriRec *__restrict__ trec, const int4 *__restrict__ small, uint32_t *__restrict__ counters,
     rt_node32 *__restrict__ heap, TravNode *__restrict__ trav, TravTri *__restrict__ tris)
 {
     __shared__ SubtreeLds L;
@@ -640,7 +631,7 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
             float lo[3] = {INFINITY, INFINITY, INFINITY};
             float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
             for (int32_t j = tid; j < rcnt; j += kSubtreeThreads)
-                leaf_prim(D, vals, pbox, xyz, idx, tris, rfirst + j, j, rcnt, (uint32_t)root.x, lo, hi);
+                leaf_prim(vals, trec, tris, rfirst + j, j, rcnt, (uint32_t)root.x, lo, hi);
             block_box_reduce<kSubtreeThreads / 64>(lo, hi, L.red);
             if (tid == 0) {
                 const uint32_t h = (uint32_t)root.x;
@@ -695,7 +686,7 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
                     float lo[3] = {INFINITY, INFINITY, INFINITY};
                     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
                     for (int32_t j = 0; j < cnt; ++j)
-                        leaf_prim(D, vals, pbox, xyz, idx, tris, first + j, j, cnt, h, lo, hi);
+                        leaf_prim(vals, trec, tris, first + j, j, cnt, h, lo, hi);
                     write_heap(&sheap[h], lo, hi, (cnt << 3) | 4 | axis, (int32_t)(first - D.prim_off));
                     if (h < leaf_level) {
                         write_heap_word(&sheap[2 * h], kInvalid, kInvalid);
@@ -907,7 +898,7 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
     if (ev) hipEventRecord(ev[0], st);
     const uint32_t nb = b.total_blocks > 0 ? b.total_blocks : 1;
     hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(kBoundsThreads), 0, st, b.d_desc, S, b.total_blocks,
-                       b.xyz, b.idx, b.pbox, b.cent, b.partials, b.hist, nzero);
+                       b.xyz, b.idx, b.trec, b.cent, b.partials, b.hist, nzero);
     if (ev) hipEventRecord(ev[1], st);
     const int start = (int)(passes & 1);  // the sorted result ends in buffer 0
     if (b.total_blocks > 0)
@@ -948,7 +939,7 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
     if (n > 0) {
         const uint32_t blocks = min_u32(2u * (uint32_t)cu_count, (n + 63) / 64);
         hipLaunchKernelGGL(k_subtree, dim3(blocks > 0 ? blocks : 1), dim3(kSubtreeThreads), 0, st,
-                           b.d_desc, S, b.hdr, b.keys[0], b.vals[0], b.pbox, b.xyz, b.idx, b.small,
+                           b.d_desc, S, b.hdr, b.keys[0], b.vals[0], b.trec, b.small,
                            b.counters, b.heap, b.trav, b.tris);
     }
     hipLaunchKernelGGL(k_top_refit, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.top_nodes,

@@ -77,10 +77,15 @@ struct alignas(16) TravTri {
 };
 static_assert(sizeof(TravTri) == 48, "TravTri must be 48 bytes");
 
-struct PrimBox {
-    float mn[3];
-    float mx[3];
+// Triangle in input order, written by the bounds pass where it reads the
+// vertices anyway: A.xyz + scene-local id, B.xyz, C.xyz.  The emit pass
+// gathers one record per sorted triangle for its box and traversal record.
+struct alignas(16) TriRec {
+    float4 a;
+    float4 b;
+    float4 c;
 };
+static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
 
 // Per scene of a batch, device side.
 // Input arrays (vertices, indices) are laid out by capacity; build arrays
@@ -141,7 +146,7 @@ struct Batch {
     uint32_t *idx = nullptr;    // per-scene vertex indices (3 per triangle)
     SceneDesc *d_desc = nullptr;
     SceneHdr *hdr = nullptr;
-    PrimBox *pbox = nullptr;
+    TriRec *trec = nullptr;
     float *cent = nullptr;      // 3 floats per triangle (3x centroid)
     float *partials = nullptr;  // 6 floats per bounds block
     uint32_t *keys[2] = {nullptr, nullptr};
