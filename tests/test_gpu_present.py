@@ -48,3 +48,15 @@ def test_present_accumulation_average(ctx, rtcore):
     assert np.array_equal(img, want)
     ctx.free(acc)
     ctx.scene_destroy(s)
+
+
+def test_present_rejects_mismatched_size(ctx, rtcore):
+    xyz, idx = rtcore.gen_torus(0.4, 0.15, 16, 8, 0.0, 2)
+    s = ctx.add_scene(xyz, idx, 10)
+    ctx.build(s)
+    cam = rtcore.camera_look_at((0.2, -1.4, 0.6), (0, 0, 0), (0, 0, 1), 55.0, 1.0)
+    ctx.render_whitted(s, cam, 1, 1, (0.5, -0.5, 1.0), bounces=0)
+    assert ctx.present(1, 1).shape == (1, 1, 4)
+    with pytest.raises(rtcore.RtError):
+        ctx.present(2, 2)  # the frame buffer holds a 1 x 1 frame
+    ctx.scene_destroy(s)

@@ -224,3 +224,37 @@ def test_autotune_whitted(ctx, rtcore, room, tmp_path):
     b = ctx.frame_read(w, h, 1)[0]
     assert np.array_equal(a, b)
     ctx.set_tuning(128, 26, 8, 0)
+
+
+def test_whitted_edge_cases(ctx, oracle, rtcore):
+    """An empty scene (every ray misses), a 1 x 1 image, one bounce in the
+    wavefront form, and a single-triangle scene on both structures."""
+    empty = ctx.scene_create(16, 16, 8)
+    ctx.scene_upload(empty, np.zeros((0, 3), F), np.zeros((0, 3), np.uint32))
+    ctx.build(empty)
+    cam = rtcore.camera_look_at((0, -2, 0.5), (0, 0, 0), (0, 0, 1), 50.0, 1.0)
+    ctx.render_whitted(empty, cam, 17, 9, (0, 0, 2), bounces=3, env=(0.25, 0.5, 0.75))
+    img = ctx.frame_read(17, 9, 1)[0]
+    assert np.array_equal(img, np.tile(np.array([0.25, 0.5, 0.75, 1.0], F), (17 * 9, 1)))
+    ctx.render_whitted(empty, cam, 5, 5, (0, 0, 2), bounces=2, env_flag=0)
+    assert (ctx.frame_read(5, 5, 1)[0] == 1).all()
+    ctx.scene_destroy(empty)
+    xyz = np.array([[-1, 0, -1], [1, 0, -1], [0, 0, 1]], F)
+    idx = np.array([[0, 1, 2]], np.uint32)
+    one = ctx.add_scene(xyz, idx, 4)
+    ctx.build(one)
+    ref = oracle.build_scene(xyz, idx, 4)
+    for w, h, bounces in ((1, 1, 0), (1, 1, 1), (7, 3, 1), (64, 48, 5)):
+        cam = rtcore.camera_look_at((0.1, -2, 0.2), (0, 0, 0), (0, 0, 1), 50.0, w / h)
+        for structure in (rtcore.STRUCTURE_LBVH, rtcore.STRUCTURE_BVH):
+            if structure == rtcore.STRUCTURE_BVH:
+                ctx.bvh_build(one)
+            else:
+                ctx.scene_select(one, rtcore.STRUCTURE_LBVH)
+            ctx.render_whitted(one, cam, w, h, (0.5, -1.0, 1.0), bounces=bounces, switches=SHADOWS,
+                               env=(0.1, 0.2, 0.3))
+            g = ctx.frame_read(w, h, 1)[0]
+            r = oracle.render_whitted(ref["nodes"], ref["ids"], xyz, idx, cam.as_array(), w, h, (0.5, -1.0, 1.0),
+                                      bounces=bounces, switches=SHADOWS, env=(0.1, 0.2, 0.3))[0]
+            assert np.array_equal(g, r), (w, h, bounces, structure)
+    ctx.scene_destroy(one)
