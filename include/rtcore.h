@@ -120,8 +120,9 @@ typedef struct rt_tuning {
     int32_t block;         /* threads per block: 64, 128 or 256 (tile 16 x block/16) */
     int32_t lds_stack;     /* stack entries per lane kept in LDS (8..26); deeper ones spill */
     int32_t blocks_per_cu; /* resident blocks per CU for persistent job launches */
-    int32_t traversal;     /* 0: loop form, 1: step form (same visit order and results;
-                              the faster one depends on the scene) */
+    int32_t traversal;     /* 0: loop form, 1: step form, 2: loop form fetching nodes and
+                              leaves with one shared load (same visit order and results;
+                              the fastest one depends on the scene) */
 } rt_tuning;
 
 /* One image of one scene for rt_render_jobs: spp samples per pixel are added

@@ -906,6 +906,8 @@ int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint3
 // ---------------------------------------------------------------------------
 // Render jobs, tuning, graphs
 // ---------------------------------------------------------------------------
+static const int kForms = 3;  // traversal forms: loop, step, loop with the shared fetch
+
 static int check_tuning(rt_ctx *ctx, const rt_tuning &t)
 {
     if (t.block != 64 && t.block != 128 && t.block != 256)
@@ -913,7 +915,7 @@ static int check_tuning(rt_ctx *ctx, const rt_tuning &t)
     if (t.lds_stack < 1 || t.lds_stack > 26) return fail(ctx, RT_ERR_ARG, "tuning: lds_stack in 1..26");
     if (t.blocks_per_cu < 1 || t.blocks_per_cu > 32)
         return fail(ctx, RT_ERR_ARG, "tuning: blocks_per_cu in 1..32");
-    if (t.traversal != 0 && t.traversal != 1) return fail(ctx, RT_ERR_ARG, "tuning: traversal 0 or 1");
+    if (t.traversal < 0 || t.traversal >= kForms) return fail(ctx, RT_ERR_ARG, "tuning: traversal 0, 1 or 2");
     return RT_OK;
 }
 
@@ -1107,7 +1109,7 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
     float bms = 1e30f;
     const int bpcs[] = {2, 4, 8};
     int rc = RT_OK;
-    for (int form = 0; form < 2 && rc == RT_OK; ++form) {
+    for (int form = 0; form < kForms && rc == RT_OK; ++form) {
         for (const rt_tuning &shape : kShapes) {
             for (int bpc : bpcs) {
                 rt_tuning t = shape;
@@ -1169,7 +1171,7 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     rt_tuning bt = saved;
     float bms = 1e30f;
     int rc = RT_OK;
-    for (int form = 0; form < 2 && rc == RT_OK; ++form) {
+    for (int form = 0; form < kForms && rc == RT_OK; ++form) {
         for (const rt_tuning &shape : kShapes) {
             rt_tuning t = shape;
             t.blocks_per_cu = saved.blocks_per_cu;
@@ -1237,7 +1239,7 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     rt_whitted_desc d = *desc;
     for (int one = 0; one < 2 && rc == RT_OK; ++one) {
         d.switches = one ? (desc->switches | RT_WHITTED_ONE_PASS) : (desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
-        for (int form = 0; form < 2 && rc == RT_OK; ++form) {
+        for (int form = 0; form < kForms && rc == RT_OK; ++form) {
             for (const rt_tuning &shape : kShapes) {
                 rt_tuning t = shape;
                 t.blocks_per_cu = saved.blocks_per_cu;

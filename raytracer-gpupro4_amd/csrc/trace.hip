@@ -152,6 +152,66 @@ __device__ __forceinline__ bool leaf_test(TraceOut &r, const TravTri *__restrict
     return false;
 }
 
+// leaf_test with the leaf's first triangle (A0, B0, C0) already fetched by
+// fetch_step; the rest follow in chunks of four, tested in the same order.
+__device__ __forceinline__ bool leaf_test_first(TraceOut &r, const TravTri *__restrict__ tris, uint32_t ref,
+                                                const float4 A0, const float4 B0, const float4 C0, float ox,
+                                                float oy, float oz, float dx, float dy, float dz, int32_t self,
+                                                bool anyhit)
+{
+    constexpr int kChunk = 4;
+    const uint32_t pos = ref & kLeafPosMask;
+    const uint32_t hint = (ref >> kLeafCountShift) & 7u;
+    const int32_t cnt = hint == kLeafCountCap ? __float_as_int(B0.w) : (int32_t)hint;
+    const int32_t heap = __float_as_int(C0.w);
+    if (tri_test(r, A0, B0, C0, ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos) && anyhit) return true;
+    const TravTri *tp = &tris[pos];
+    for (int32_t j = 1; j < cnt; j += kChunk) {
+        const int32_t m = cnt - j < kChunk ? cnt - j : kChunk;
+        float4 A[kChunk], B[kChunk], C[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            if (k < m) {
+                A[k] = tp[j + k].a;
+                B[k] = tp[j + k].e1;
+                C[k] = tp[j + k].e2;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            if (k < m && tri_test(r, A[k], B[k], C[k], ox, oy, oz, dx, dy, dz, self, heap, (int32_t)pos + j + k) &&
+                anyhit)
+                return true;
+        }
+    }
+    return false;
+}
+
+// One fetch per iteration for every lane: the 64-byte node of a lane at an
+// internal node, or the first triangle record of a lane at a leaf (its fourth
+// quad re-reads the record's first).  Lanes at nodes and lanes at leaves then
+// wait on the same loads, so a wave holding both kinds pays one memory round
+// trip per iteration instead of one per branch.  The empty asm reads the
+// last quad, which keeps all four loads ahead of the branch.
+__device__ __forceinline__ void fetch_step(const TravNode *__restrict__ nodes, const TravTri *__restrict__ tris,
+                                           int32_t cur, float4 &q0, float4 &q1, float4 &q2, float4 &q3)
+{
+    const bool at_node = cur >= 0;
+    const float4 *p = at_node ? reinterpret_cast<const float4 *>(&nodes[cur])
+                              : reinterpret_cast<const float4 *>(&tris[(uint32_t)cur & kLeafPosMask]);
+    const float4 *p3 = at_node ? p + 3 : p;
+    q0 = p[0];
+    q1 = p[1];
+    q2 = p[2];
+    q3 = *p3;
+    asm volatile("" ::"v"(q3.x), "v"(q3.y), "v"(q3.z), "v"(q3.w));
+}
+
+__device__ __forceinline__ int4 as_int4(const float4 f)
+{
+    return make_int4(__float_as_int(f.x), __float_as_int(f.y), __float_as_int(f.z), __float_as_int(f.w));
+}
+
 template <bool ANYHIT, class Stack>
 __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
                                              const TravTri *__restrict__ tris, int32_t root_ref,
@@ -224,14 +284,89 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
     return r;
 }
 
+// traverse() with fetch_step: one round trip per iteration for waves whose
+// lanes sit at nodes and at leaves, instead of a node fetch and a leaf fetch
+// in their own branches.  Same visits, same results.
+template <bool ANYHIT, class Stack>
+__device__ __forceinline__ TraceOut traverse_shared(const TravNode *__restrict__ nodes,
+                                             const TravTri *__restrict__ tris, int32_t root_ref,
+                                             const float *root_box, float ox, float oy, float oz,
+                                             float dx, float dy, float dz, float tmax, int32_t self,
+                                             Stack &stk)
+{
+    TraceOut r;
+    r.tri = -1;
+    r.u = -1.0f;
+    r.v = -1.0f;
+    r.t = tmax;
+    r.boxes = 0;
+    r.tris = 0;
+    r.leaf_heap = 1;
+    r.pos = -1;
+    if (root_ref == -1) return r;
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const uint32_t neg = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
+    float t0, t1;
+    box_test(root_box, ox, oy, oz, ix, iy, iz, t0, t1);
+    r.boxes = 1;
+    if (t0 > r.t || t1 < 0.0f) return r;
+    int32_t cur = root_ref;
+    stk.sp = 0;
+    while (true) {
+        float4 q0, q1, q2, q3f;
+        fetch_step(nodes, tris, cur, q0, q1, q2, q3f);
+        if (cur >= 0) {
+            const int4 q3 = as_int4(q3f);
+            const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+            const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            float a0, a1, b0, b1;
+            box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
+            box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+            r.boxes += 2;
+            const int first_is_b = (neg >> q3.z) & 1u;  // the child on side dirIsNeg[axis]
+            const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
+            const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+            const int32_t fref = first_is_b ? q3.y : q3.x;
+            const int32_t sref = first_is_b ? q3.x : q3.y;
+            const bool f_ok = !(f0 > r.t || f1 < 0.0f);
+            const bool s_live = !(s1 < 0.0f);
+            if (f_ok) {
+                if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+                cur = fref;
+                continue;
+            }
+            if (s_live && !(s0 > r.t)) {
+                cur = sref;
+                continue;
+            }
+        } else {
+            if (leaf_test_first(r, tris, (uint32_t)cur, q0, q1, q2, ox, oy, oz, dx, dy, dz, self, ANYHIT)) return r;
+        }
+        // pop the next live entry
+        bool found = false;
+        while (stk.sp > 0) {
+            const uint2 e = stk.pop();
+            if (!(__uint_as_float(e.y) > r.t)) {
+                cur = (int32_t)e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return r;
+}
+
 // The same traversal written as a step function: trav_begin tests the root
-// box; each trav_step handles one internal node (both child boxes) or one
-// leaf and pops the stack; a ray is finished when either returns true.  It
-// visits the same nodes in the same order as traverse(), but the compiler
-// lays the loop out differently, and which layout is faster depends on the
-// scene (deep leaves of many triangles favour traverse(), incoherent any-hit
-// rays through thin geometry favour this one), so the launch shape's
-// traversal field picks one per launch.
+// box; each trav_step fetches with fetch_step, handles one internal node (both
+// child boxes) or one leaf and pops the stack; a ray is finished when either
+// returns true.  It visits the same nodes in the same order as traverse(),
+// but the compiler lays the loop out differently, and which layout is faster
+// depends on the scene (coherent rays over deep leaves favour the loop with
+// split fetches; incoherent rays through thin geometry, whose waves mix lanes
+// at nodes and at leaves, favour the shared fetch), so the launch shape's
+// traversal field picks one per launch: 0 loop, 1 step, 2 loop with the
+// shared fetch.
 struct Trav {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
     int32_t self, cur;
@@ -281,12 +416,10 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
                                           const TravTri *__restrict__ tris, Stack &stk)
 {
     TraceOut &r = T.r;
+    float4 q0, q1, q2, q3f;
+    fetch_step(nodes, tris, T.cur, q0, q1, q2, q3f);
     if (T.cur >= 0) {
-        const TravNode *n = &nodes[T.cur];
-        const float4 q0 = *reinterpret_cast<const float4 *>(&n->box[0]);
-        const float4 q1 = *reinterpret_cast<const float4 *>(&n->box[4]);
-        const float4 q2 = *reinterpret_cast<const float4 *>(&n->box[8]);
-        const int4 q3 = *reinterpret_cast<const int4 *>(&n->ref[0]);
+        const int4 q3 = as_int4(q3f);
         const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
         const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
         float a0, a1, b0, b1;
@@ -310,7 +443,7 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
             return false;
         }
     } else {
-        if (leaf_test(r, tris, (uint32_t)T.cur, T.ox, T.oy, T.oz, T.dx, T.dy, T.dz, T.self,
+        if (leaf_test_first(r, tris, (uint32_t)T.cur, q0, q1, q2, T.ox, T.oy, T.oz, T.dx, T.dy, T.dz, T.self,
                       MODE == 1 || (MODE == 2 && T.anyhit)))
             return true;
     }
@@ -349,6 +482,8 @@ __device__ __forceinline__ TraceOut traverse_form(const TravNode *__restrict__ n
 {
     if (FORM == 1)
         return traverse_steps<ANYHIT>(nodes, tris, root_ref, root_box, ox, oy, oz, dx, dy, dz, tmax, self, stk);
+    if (FORM == 2)
+        return traverse_shared<ANYHIT>(nodes, tris, root_ref, root_box, ox, oy, oz, dx, dy, dz, tmax, self, stk);
     return traverse<ANYHIT>(nodes, tris, root_ref, root_box, ox, oy, oz, dx, dy, dz, tmax, self, stk);
 }
 
@@ -1227,6 +1362,8 @@ static SceneView view_of(const Batch &b, uint32_t scene)
 #define RT_TUNED(MACRO)                                  \
     if (tune.traversal == 1) {                           \
         RT_SHAPES(MACRO, 1)                              \
+    } else if (tune.traversal == 2) {                    \
+        RT_SHAPES(MACRO, 2)                              \
     } else {                                             \
         RT_SHAPES(MACRO, 0)                              \
     }

@@ -99,7 +99,7 @@ def test_jobs_mix_and_launch_mode_invariance(ctx, rtcore):
     alone = [run([j], False)[k] for k, j in enumerate(jobs)]
     saved = ctx.tuning()
     try:
-        for shape in [(64, 16, 0), (64, 20, 1), (128, 12, 0), (128, 20, 1), (128, 26, 0), (256, 12, 1)]:
+        for shape in [(64, 16, 0), (64, 20, 1), (128, 12, 2), (128, 20, 1), (128, 26, 0), (128, 22, 2), (256, 12, 1)]:
             ctx.set_tuning(shape[0], shape[1], 2, shape[2])
             for persistent in (False, True):
                 together = run(jobs, persistent)
@@ -123,7 +123,7 @@ def test_trace_parity_under_every_launch_shape(ctx, oracle, rtcore):
     r = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays, nthreads=8)
     saved = ctx.tuning()
     try:
-        for form in (0, 1):
+        for form in (0, 1, 2):
             for block, lds in [(64, 16), (64, 20), (128, 12), (128, 16), (128, 20), (128, 22), (128, 26), (256, 12)]:
                 ctx.set_tuning(block, lds, 4, form)
                 assert np.array_equal(ctx.trace(s, rays), r), (block, lds, form)
@@ -184,7 +184,7 @@ def test_autotuner_picks_and_caches(ctx, rtcore, tmp_path):
     jobs = [ctx.job(s, cam, 128, 128, acc, rtcore.MODE_GI, 1, 2, 0.7, 1.0)]
     cache = str(tmp_path / "tune.txt")
     best = ctx.autotune(jobs, cache)
-    assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] in (2, 4, 8) and best[3] in (0, 1)
+    assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] in (2, 4, 8) and best[3] in (0, 1, 2)
     assert ctx.tuning() == best
     assert open(cache).read().strip().endswith(" ".join(str(v) for v in best))
     ctx.set_tuning(128, 26, 4)
@@ -224,7 +224,7 @@ def test_autotune_direct_picks_and_caches(ctx, rtcore, mix, tmp_path):
     saved = ctx.tuning()
     try:
         best = ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache)
-        assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] == saved[2] and best[3] in (0, 1)
+        assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] == saved[2] and best[3] in (0, 1, 2)
         assert ctx.tuning() == best
         assert open(cache).read().split()[-4:] == [str(v) for v in best]
         ctx.set_tuning(64, 8, saved[2])
