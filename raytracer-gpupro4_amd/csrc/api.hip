@@ -119,16 +119,21 @@ int ensure_frame(rt_ctx *ctx, uint32_t w, uint32_t h, uint32_t nl)
     hipFree(f.rgba);
     hipFree(f.hits);
     hipFree(f.occ);
+    hipFree(f.tile_order);
+    hipFree(f.tile_cost);
     ctx->bytes -= f.bytes;
     f = rt::FrameBuffers();
     const uint32_t nlc = nl > 0 ? nl : 1;
+    const uint64_t tiles = (uint64_t)((w + 15) / 16) * ((h + 3) / 4);
     RT_HIP(ctx, hipMalloc(&f.rgba, sizeof(float4) * (px ? px : 1)));
     RT_HIP(ctx, hipMalloc(&f.hits, sizeof(rt_hit24) * (px ? px : 1)));
     RT_HIP(ctx, hipMalloc(&f.occ, (px ? px : 1) * nlc));
+    RT_HIP(ctx, hipMalloc(&f.tile_order, sizeof(uint32_t) * (tiles ? tiles : 1)));
+    RT_HIP(ctx, hipMalloc(&f.tile_cost, sizeof(uint32_t) * (tiles ? tiles : 1)));
     f.width = w;
     f.height = h;
     f.nlights = nlc;
-    f.bytes = (sizeof(float4) + sizeof(rt_hit24) + nlc) * (px ? px : 1);
+    f.bytes = (sizeof(float4) + sizeof(rt_hit24) + nlc) * (px ? px : 1) + 2 * sizeof(uint32_t) * (tiles ? tiles : 1);
     ctx->bytes += f.bytes;
     return RT_OK;
 }
@@ -321,6 +326,8 @@ void rt_destroy(rt_ctx *ctx)
     hipFree(ctx->frame.rgba);
     hipFree(ctx->frame.hits);
     hipFree(ctx->frame.occ);
+    hipFree(ctx->frame.tile_order);
+    hipFree(ctx->frame.tile_cost);
     for (rt_ctx::Graph &g : ctx->graphs) {
         if (g.exec) hipGraphExecDestroy(g.exec);
         if (g.graph) hipGraphDestroy(g.graph);

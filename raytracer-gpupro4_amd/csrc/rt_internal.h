@@ -210,6 +210,14 @@ struct FrameBuffers {
     float4 *rgba = nullptr;
     rt_hit24 *hits = nullptr;
     uint8_t *occ = nullptr;
+    // rt_render_direct's tile order (heaviest tiles of the last frame first)
+    // and the per-tile costs it is made from; sized for the smallest tile
+    // (16 x 4, block 64), valid for one (scene, block size, tile count).
+    uint32_t *tile_order = nullptr;
+    uint32_t *tile_cost = nullptr;
+    bool order_valid = false;
+    uint64_t order_key = 0;
+    uint32_t order_block = 0, order_tiles = 0;
     uint64_t bytes = 0;
 };
 

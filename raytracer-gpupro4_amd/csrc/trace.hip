@@ -652,7 +652,9 @@ struct RenderParams {
     rt_camera cam;
     uint32_t width, height, nlights;
     float ambient, albedo, env;
-    float lights[24];  // up to 8 point lights
+    float lights[24];           // up to 8 point lights
+    const uint32_t *order;      // tile of each block, heaviest first; null: block index order
+    uint32_t *cost;             // per tile: max over its lanes of box + triangle tests (atomicMax)
 };
 
 // A block renders a tile of 16 x BLOCK/16 pixels.  With two or more waves
@@ -687,7 +689,8 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
     stk.lds = &lds[threadIdx.x];
     constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
     const uint32_t tiles_x = (rp.width + TW - 1) / TW;
-    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const uint32_t tile = rp.order ? rp.order[blockIdx.x] : blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     uint32_t ox_, oy_;
     Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
     const uint32_t px = tx * TW + ox_;
@@ -724,6 +727,55 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
         count_add(stats, 1, ntris);
     }
     count_add(stats, 2, nshadow);
+    if (rp.cost) {  // the wave's cost: its longest lane (the waves of a tile take the max)
+        uint32_t c = boxes + ntris;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t y = __shfl_xor(c, o);
+            c = y > c ? y : c;
+        }
+        if (__lane_id() == 0 && c) atomicMax(&rp.cost[tile], c);
+    }
+}
+
+// The next frame's tile order from this frame's tile costs, heaviest first:
+// a wave's time is the length of its longest lane's chain of dependent
+// iterations, and the frame ends with its last wave, so the longest tiles
+// should start first.  One block: a counting sort over 16 buckets per octave
+// of cost (descending; the order inside a bucket is whatever the atomics
+// give), then the costs are cleared for the next frame.
+constexpr int kOrderBuckets = 33 * 16;
+
+__device__ __forceinline__ int order_bucket(uint32_t c)
+{
+    if (c == 0) return kOrderBuckets - 1;
+    const int e = 31 - __clz(c);                            // 0..31
+    const uint32_t m = e >= 4 ? (c >> (e - 4)) & 15u : (c << (4 - e)) & 15u;
+    return kOrderBuckets - 1 - (e + 1) * 16 - (int)m;       // larger cost, smaller bucket
+}
+
+__global__ __launch_bounds__(1024) void k_tile_order(uint32_t *__restrict__ cost, uint32_t n,
+                                                     uint32_t *__restrict__ order)
+{
+    __shared__ uint32_t hist[kOrderBuckets];
+    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[order_bucket(cost[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive prefix sum, in place
+        uint32_t run = 0;
+        for (int b = 0; b < kOrderBuckets; ++b) {
+            const uint32_t h = hist[b];
+            hist[b] = run;
+            run += h;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t pos = atomicAdd(&hist[order_bucket(cost[i])], 1u);
+        order[pos] = i;
+        cost[i] = 0u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1417,10 +1469,25 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
     return hipGetLastError();
 }
 
+// The tile order kept with the frame buffers belongs to one (scene, block
+// size, tile count); any other launch starts from block order with cleared
+// costs.  Returns false if clearing failed.
+static bool order_ready(FrameBuffers &fb, uint64_t key, int block, uint32_t tiles, hipStream_t st)
+{
+    if (fb.order_valid && fb.order_key == key && fb.order_block == (uint32_t)block && fb.order_tiles == tiles)
+        return true;
+    fb.order_valid = false;
+    fb.order_key = key;
+    fb.order_block = (uint32_t)block;
+    fb.order_tiles = tiles;
+    return hipMemsetAsync(fb.tile_cost, 0, sizeof(uint32_t) * tiles, st) == hipSuccess;
+}
+
 hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st)
 {
+    const uint64_t scene_key = ((uint64_t)(uintptr_t)&b << 16) ^ scene;
     const uint32_t max_depth = trav_depth(b, scene);
     RenderParams rp;
     rp.cam = d.cam;
@@ -1433,11 +1500,16 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     for (uint32_t i = 0; i < 24; ++i) rp.lights[i] = (i < 3 * d.nlights) ? d.lights[i] : 0.0f;
     const SceneView sv = view_of(b, scene);
     const bool keep = flags & RT_FLAG_KEEP_HITS, cnt = flags & RT_FLAG_COUNT;
+    uint32_t ntiles = 0;
 #define RT_L(B, K, F)                                                                               \
     {                                                                                              \
         constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                            \
         const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);             \
         const dim3 grid(tiles), block(B);                                                          \
+        ntiles = tiles;                                                                            \
+        if (!order_ready(fb, scene_key, B, tiles, st)) return hipGetLastError();                   \
+        rp.order = fb.order_valid ? fb.tile_order : nullptr;                                       \
+        rp.cost = fb.tile_cost;                                                                    \
         if (keep && cnt) hipLaunchKernelGGL((k_render_direct<true, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
         else if (keep) hipLaunchKernelGGL((k_render_direct<true, false, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
         else if (cnt) hipLaunchKernelGGL((k_render_direct<false, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
@@ -1445,6 +1517,8 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     }
     RT_TUNED(RT_L)
 #undef RT_L
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, fb.tile_cost, ntiles, fb.tile_order);
+    fb.order_valid = true;
     return hipGetLastError();
 }
 

@@ -179,6 +179,45 @@ def test_render_direct_forms_equal(ctx, rtcore, soup262k, form):
         assert st0[k] == st1[k], k
 
 
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_render_direct_tile_order_keeps_the_frame(ctx, rtcore, soup262k, block):
+    """rt_render_direct dispatches the tiles of a repeated frame heaviest
+    first (from the last frame's per-tile costs): the first frame of a shape
+    runs in block order, the next ones in cost order, and every frame is the
+    same bit for bit, counters included.  A camera move, a size change and a
+    shape change keep it so."""
+    s = soup262k[0]
+    W, H = 401, 233
+    cams = [rtcore.camera_look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0, 1, 0), 60.0, W / H),
+            rtcore.camera_look_at((1.8, 0.9, -0.6), (0.5, 0.5, 0.5), (0, 1, 0), 50.0, W / H)]
+    lights = [(0.5, 2.0, -1.0), (1.5, 0.5, 0.5)]
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(block, 26 if block == 128 else 12, 4, 2)
+        for cam in cams:
+            ref = None
+            for rep in range(3):
+                st = ctx.render_direct(s, cam, W, H, lights, flags=rtcore.FLAG_KEEP_HITS | rtcore.FLAG_COUNT)
+                out = ctx.frame_read(W, H, len(lights), hits=True)
+                if ref is None:
+                    ref = (st, out)
+                    continue
+                for x, y in zip(ref[1], out):
+                    assert np.array_equal(x, y), (block, rep)
+                for k in ("primary_rays", "shadow_rays", "box_tests", "tri_tests"):
+                    assert ref[0][k] == st[k], (block, rep, k)
+        # a smaller frame in the same buffers' place, then the first size again
+        ctx.render_direct(s, cams[0], 97, 55, lights)
+        ctx.render_direct(s, cams[0], W, H, lights, flags=rtcore.FLAG_KEEP_HITS)
+        again = ctx.frame_read(W, H, len(lights), hits=True)
+        ctx.render_direct(s, cams[0], W, H, lights, flags=rtcore.FLAG_KEEP_HITS)
+        twice = ctx.frame_read(W, H, len(lights), hits=True)
+        for x, y in zip(again, twice):
+            assert np.array_equal(x, y)
+    finally:
+        ctx.set_tuning(*saved)
+
+
 def chain_scene():
     """Triangles whose Morton codes are 2^29, 2^28, ..., 1, 0: each split
     peels one off, so the tree is a chain as deep as the heap allows."""
