@@ -676,6 +676,18 @@ struct Tile {
     }
 };
 
+// A tile's cost for the next frame's order: the box + triangle tests of the
+// longest lane of its waves (a wave runs as long as its longest lane).
+__device__ __forceinline__ void tile_cost_max(uint32_t *cost, uint32_t tile, uint32_t c)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(c, o);
+        c = y > c ? y : c;
+    }
+    if (__lane_id() == 0 && c) atomicMax(&cost[tile], c);
+}
+
 // Primary + shadow rays + direct term for one image, one tile per block.
 template <bool KEEP, bool COUNT, int BLOCK, int KLDS, int FORM>
 __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderParams rp,
@@ -727,15 +739,7 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
         count_add(stats, 1, ntris);
     }
     count_add(stats, 2, nshadow);
-    if (rp.cost) {  // the wave's cost: its longest lane (the waves of a tile take the max)
-        uint32_t c = boxes + ntris;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const uint32_t y = __shfl_xor(c, o);
-            c = y > c ? y : c;
-        }
-        if (__lane_id() == 0 && c) atomicMax(&rp.cost[tile], c);
-    }
+    if (rp.cost) tile_cost_max(rp.cost, tile, boxes + ntris);
 }
 
 // The next frame's tile order from this frame's tile costs, heaviest first:
@@ -873,6 +877,8 @@ struct WhittedParams {
     uint32_t frame;
     uint64_t seed;
     rt_material defmat;
+    const uint32_t *order;  // tile of each block, heaviest first; null: block index order
+    uint32_t *cost;         // per tile: max over its lanes of box + triangle tests (atomicMax)
 };
 
 // A live path between passes: origin, direction, reflective factor, the id
@@ -1131,7 +1137,8 @@ __global__ __launch_bounds__(BLOCK) void k_render_whitted(SceneView sv, ShadeVie
     stk.lds = &lds[threadIdx.x];
     constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
     const uint32_t tiles_x = (wp.width + TW - 1) / TW;
-    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const uint32_t tile = wp.order ? wp.order[blockIdx.x] : blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     uint32_t ox_, oy_;
     Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
     const uint32_t px = tx * TW + ox_;
@@ -1160,6 +1167,7 @@ __global__ __launch_bounds__(BLOCK) void k_render_whitted(SceneView sv, ShadeVie
     }
     count_add(stats, 2, cnt.nshadow);
     count_add(stats, 3, nbounce);
+    if (wp.cost) tile_cost_max(wp.cost, tile, cnt.boxes + cnt.ntris);
 }
 
 // Wavefront form, pass 0: one lane per pixel; live paths go to queue q.
@@ -1175,7 +1183,8 @@ __global__ __launch_bounds__(BLOCK) void k_whitted_first(SceneView sv, ShadeView
     stk.lds = &lds[threadIdx.x];
     constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
     const uint32_t tiles_x = (wp.width + TW - 1) / TW;
-    const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const uint32_t tile = wp.order ? wp.order[blockIdx.x] : blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     uint32_t ox_, oy_;
     Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
     const uint32_t px = tx * TW + ox_;
@@ -1202,6 +1211,7 @@ __global__ __launch_bounds__(BLOCK) void k_whitted_first(SceneView sv, ShadeView
         count_add(stats, 1, cnt.ntris);
     }
     count_add(stats, 2, cnt.nshadow);
+    if (wp.cost) tile_cost_max(wp.cost, tile, cnt.boxes + cnt.ntris);
 }
 
 // Wavefront form, a bounce pass: the *in_count rays of queue in (at most one
@@ -1487,7 +1497,7 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st)
 {
-    const uint64_t scene_key = ((uint64_t)(uintptr_t)&b << 16) ^ scene;
+    const uint64_t scene_key = (((uint64_t)(uintptr_t)&b << 16) ^ scene) + 1u;
     const uint32_t max_depth = trav_depth(b, scene);
     RenderParams rp;
     rp.cam = d.cam;
@@ -1562,10 +1572,17 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     const uint64_t px = (uint64_t)d.width * d.height;
     RayRec *q[2] = {static_cast<RayRec *>(queue), static_cast<RayRec *>(queue) + px};
     if (!one_pass) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, qcount, d.bounces + 1);
+    // the tile order of the one-pass kernel or of the wavefront's first pass
+    const uint64_t order_key = (((uint64_t)(uintptr_t)&b << 16) ^ scene) + (one_pass ? 2u : 3u);
+    uint32_t ntiles = 0;
 #define RT_L(B, K, F)                                                                                      \
     {                                                                                                     \
         constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                                   \
         const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);                    \
+        ntiles = tiles;                                                                                   \
+        if (!order_ready(fb, order_key, B, tiles, st)) return hipGetLastError();                          \
+        wp.order = fb.order_valid ? fb.tile_order : nullptr;                                              \
+        wp.cost = fb.tile_cost;                                                                           \
         if (one_pass) {                                                                                   \
             hipLaunchKernelGGL((k_render_whitted<B, K, F>), dim3(tiles), dim3(B), 0, st, sv, sh, wp,      \
                                fb.rgba, fb.hits, fb.occ, keep, cnt, stats);                               \
@@ -1583,6 +1600,8 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     }
     RT_TUNED(RT_L)
 #undef RT_L
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, fb.tile_cost, ntiles, fb.tile_order);
+    fb.order_valid = true;
     return hipGetLastError();
 }
 

@@ -131,6 +131,26 @@ def test_whitted_forms_identical_at_1080p(ctx, rtcore, room):
             assert a[k] == b[k], k
 
 
+def test_whitted_repeated_frames_identical(ctx, rtcore, room):
+    """A repeated frame runs its tiles heaviest first (the last frame's tile
+    costs): both forms give the same frame and counters every time."""
+    s = room[0]
+    w, h = 640, 360
+    cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
+    kw = dict(bounces=3, env=(0.2, 0.3, 0.4), seed=5)
+    for sw in (SHADOWS | PHONG, SHADOWS | PHONG | ONE_PASS):
+        ref = None
+        for rep in range(3):
+            st = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=sw, flags=rtcore.FLAG_COUNT, **kw)
+            f = ctx.frame_read(w, h, 1)[0]
+            if ref is None:
+                ref = (st, f)
+                continue
+            assert np.array_equal(ref[1], f), (sw, rep)
+            for k in ("primary_rays", "shadow_rays", "bounce_rays", "box_tests", "tri_tests"):
+                assert ref[0][k] == st[k], (sw, rep, k)
+
+
 def test_whitted_default_material_and_flat(ctx, oracle, rtcore):
     """No rt_scene_shading: the default MaterialData (alpha 0: every hit
     refracts with index 1), flat normals."""
