@@ -745,9 +745,9 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
 // The next frame's tile order from this frame's tile costs, heaviest first:
 // a wave's time is the length of its longest lane's chain of dependent
 // iterations, and the frame ends with its last wave, so the longest tiles
-// should start first.  One block: a counting sort over 16 buckets per octave
-// of cost (descending; the order inside a bucket is whatever the atomics
-// give), then the costs are cleared for the next frame.
+// should start first.  Counting sorts over 16 buckets per octave of cost
+// (descending; the order inside a bucket is whatever the atomics give), then
+// the costs are cleared for the next frame.
 constexpr int kOrderBuckets = 33 * 16;
 
 __device__ __forceinline__ int order_bucket(uint32_t c)
@@ -758,28 +758,105 @@ __device__ __forceinline__ int order_bucket(uint32_t c)
     return kOrderBuckets - 1 - (e + 1) * 16 - (int)m;       // larger cost, smaller bucket
 }
 
+// Lanes of the wave whose bucket equals this lane's (ten ballots, one per
+// bit of the bucket index): one atomic per distinct bucket per wave instead
+// of one per lane, which matters because most tiles of a frame (rays that
+// miss) fall into the same few buckets.
+__device__ __forceinline__ uint64_t same_bucket_lanes(int key)
+{
+    uint64_t m = __ballot(1);
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+        const bool bit = (key >> b) & 1;
+        const uint64_t v = __ballot(bit);
+        m &= bit ? v : ~v;
+    }
+    return m;
+}
+
+// G blocks of 1024 threads (about 1024 tiles each).  Block g orders the
+// tiles i = g + r * G, an interleaved sample of the whole image, and writes
+// its r-th heaviest to position r * G + g: every block's heaviest come
+// first, so the order is close to a global one without any coordination
+// between blocks, and the positions of block g are exactly its tiles'
+// indices, so the result is a permutation.  Each pass takes the costs in
+// chunks of kPer per thread, loads issued before any is used.
 __global__ __launch_bounds__(1024) void k_tile_order(uint32_t *__restrict__ cost, uint32_t n,
                                                      uint32_t *__restrict__ order)
 {
+    static_assert(kOrderBuckets <= 1024, "ten-bit bucket index");
+    constexpr uint32_t kPer = 4, kChunk = 1024 * kPer;
     __shared__ uint32_t hist[kOrderBuckets];
-    for (int b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) hist[b] = 0;
+    const uint32_t t = threadIdx.x, lane = t & 63u;
+    const uint32_t g = blockIdx.x, G = gridDim.x;
+    const uint32_t ng = n > g ? (n - g + G - 1) / G : 0u;  // tiles of this block
+    for (int b = t; b < kOrderBuckets; b += blockDim.x) hist[b] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[order_bucket(cost[i])], 1u);
-    __syncthreads();
-    if (threadIdx.x == 0) {  // exclusive prefix sum, in place
-        uint32_t run = 0;
-        for (int b = 0; b < kOrderBuckets; ++b) {
-            const uint32_t h = hist[b];
-            hist[b] = run;
-            run += h;
+    for (uint32_t base = 0; base < ng; base += kChunk) {
+        uint32_t c[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t r = base + k * 1024u + t;
+            c[k] = r < ng ? cost[g + r * G] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            if (base + k * 1024u + t < ng) {
+                const int key = order_bucket(c[k]);
+                const uint64_t m = same_bucket_lanes(key);
+                if (lane == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+                    atomicAdd(&hist[key], (uint32_t)__popcll(m));
+            }
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const uint32_t pos = atomicAdd(&hist[order_bucket(cost[i])], 1u);
-        order[pos] = i;
-        cost[i] = 0u;
+    {  // exclusive prefix sum over the buckets, in place: wave scans, then the wave totals
+        __shared__ uint32_t wsum[16];
+        const uint32_t w = t >> 6;
+        const uint32_t v = t < (uint32_t)kOrderBuckets ? hist[t] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (lane == 63u) wsum[w] = x;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < w; ++k) before += wsum[k];
+        if (t < (uint32_t)kOrderBuckets) hist[t] = before + x - v;
     }
+    __syncthreads();
+    for (uint32_t base = 0; base < ng; base += kChunk) {
+        uint32_t c[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t r = base + k * 1024u + t;
+            c[k] = r < ng ? cost[g + r * G] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t r = base + k * 1024u + t;
+            if (r < ng) {
+                const int key = order_bucket(c[k]);
+                const uint64_t m = same_bucket_lanes(key);
+                const uint32_t leader = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+                uint32_t at = 0;
+                if (lane == leader) at = atomicAdd(&hist[key], (uint32_t)__popcll(m));
+                at = __shfl(at, (int)leader);
+                const uint32_t rank = at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                order[rank * G + g] = g + r * G;
+                cost[g + r * G] = 0u;
+            }
+        }
+    }
+}
+
+// Blocks for k_tile_order over n tiles: about 1024 tiles each, at most 256.
+static inline uint32_t tile_order_blocks(uint32_t n)
+{
+    uint32_t b = (n + 1023u) / 1024u;
+    return b < 1u ? 1u : (b > 256u ? 256u : b);
 }
 
 // ---------------------------------------------------------------------------
@@ -1527,7 +1604,8 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     }
     RT_TUNED(RT_L)
 #undef RT_L
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, fb.tile_cost, ntiles, fb.tile_order);
+    hipLaunchKernelGGL(k_tile_order, dim3(tile_order_blocks(ntiles)), dim3(1024), 0, st, fb.tile_cost, ntiles,
+                       fb.tile_order);
     fb.order_valid = true;
     return hipGetLastError();
 }
@@ -1600,7 +1678,8 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     }
     RT_TUNED(RT_L)
 #undef RT_L
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, fb.tile_cost, ntiles, fb.tile_order);
+    hipLaunchKernelGGL(k_tile_order, dim3(tile_order_blocks(ntiles)), dim3(1024), 0, st, fb.tile_cost, ntiles,
+                       fb.tile_order);
     fb.order_valid = true;
     return hipGetLastError();
 }
