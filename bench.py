@@ -331,12 +331,22 @@ def workload_whitted(args, rtcore):
         boxes += st["box_tests"]
         tris += st["tri_tests"]
         counted += st["primary_rays"] + st["shadow_rays"] + st["bounce_rays"]
-    # the other device form over the same frames (same images and rays)
-    alt = dict(kw, switches=kw["switches"] ^ rtcore.WHITTED_ONE_PASS)
-    ctx.profile_begin()
-    for f in range(args.steps):
-        ctx.render_whitted(s, cam, W, H, LIGHTS[0], frame=f, want_stats=False, **alt)
-    alt_prof = ctx.profile_end()
+    # the other device forms over the same frames (same images and rays)
+    forms = {"wavefront": 0, "one-pass": rtcore.WHITTED_ONE_PASS, "split": rtcore.WHITTED_SPLIT}
+    base_sw = kw["switches"] & ~(rtcore.WHITTED_ONE_PASS | rtcore.WHITTED_SPLIT)
+    form = next(k for k, bit in forms.items() if kw["switches"] == base_sw | bit)
+    others = []
+    for name, bit in forms.items():
+        if name == form:
+            continue
+        alt = dict(kw, switches=base_sw | bit)
+        ctx.profile_begin()
+        for f in range(args.steps):
+            ctx.render_whitted(s, cam, W, H, LIGHTS[0], frame=f, want_stats=False, **alt)
+        alt_prof = ctx.profile_end()
+        others.append({"form": name, "trace_ms_per_frame": round(alt_prof["trace_ms"] / alt_prof["launches"], 4),
+                       "mrays_s": round((alt_prof["primary_rays"] + alt_prof["shadow_rays"] + alt_prof["bounce_rays"])
+                                        / (alt_prof["trace_ms"] * 1e-3) / 1e6, 2)})
     rays = prof["primary_rays"] + prof["shadow_rays"] + prof["bounce_rays"]
     assert rays == counted, (rays, counted)
     mrays = rays / (prof["trace_ms"] * 1e-3) / 1e6
@@ -344,23 +354,21 @@ def workload_whitted(args, rtcore):
     alg = (RAY_BYTES * rays + BOX_BYTES * boxes + TRI_BYTES * tris) / prof["launches"]
     achieved = alg / (per_launch_ms * 1e-3) / 1e9
     ctx.close()
-    form = "one-pass" if kw["switches"] & rtcore.WHITTED_ONE_PASS else "wavefront"
-    other = "wavefront" if form == "one-pass" else "one-pass"
+    kernels = {"wavefront": "k_whitted_first + k_whitted_next per bounce", "one-pass": "k_render_whitted",
+               "split": "k_whitted_first + k_whitted_rest"}
     out = result(METRIC, mrays, "Mrays/s", args.steps, args.warmup, wall,
                  {"workload": f"{NTRI}-triangle soup (static), 5 materials (matte, mirror, glass 1.5, "
                               f"half mirror, default), {W}x{H} Whitted frames, {WHITTED_BOUNCES} bounces, "
                               f"shadow rays to 1 point light",
                   "tris": NTRI, "depth": DEPTH, "width": W, "height": H, "bounces": WHITTED_BOUNCES},
-                 form=form, trace_ms_per_frame=round(per_launch_ms, 4),
-                 other_form={"form": other, "trace_ms_per_frame": round(alt_prof["trace_ms"] / alt_prof["launches"], 4),
-                             "mrays_s": round(rays / (alt_prof["trace_ms"] * 1e-3) / 1e6, 2)},
+                 form=form, trace_ms_per_frame=round(per_launch_ms, 4), other_forms=others,
                  rays_per_frame=rays / args.steps,
                  shadow_rays_per_frame=prof["shadow_rays"] / args.steps,
                  bounce_rays_per_frame=prof["bounce_rays"] / args.steps,
                  tests_per_ray={"box": boxes / rays, "tri": tris / rays}, tuning=list(tuned),
                  roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                           "kernel": "k_render_whitted", "bytes_per_launch": alg},
+                           "kernel": kernels[form], "bytes_per_launch": alg},
                  cpu_baseline=None)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

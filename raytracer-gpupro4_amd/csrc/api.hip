@@ -859,8 +859,11 @@ int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint3
     if (desc->bounces > 16) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: %u bounces (0..16)", desc->bounces);
     if (desc->env_flag != -1 && desc->env_flag != 0)
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: env_flag %d (-1 or 0)", desc->env_flag);
-    if (desc->switches & ~(uint32_t)(RT_WHITTED_SHADOWS | RT_WHITTED_PHONG | RT_WHITTED_GLOSS | RT_WHITTED_ONE_PASS))
+    if (desc->switches & ~(uint32_t)(RT_WHITTED_SHADOWS | RT_WHITTED_PHONG | RT_WHITTED_GLOSS | RT_WHITTED_ONE_PASS |
+                                     RT_WHITTED_SPLIT))
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: unknown switches 0x%x", desc->switches);
+    if ((desc->switches & RT_WHITTED_ONE_PASS) && (desc->switches & RT_WHITTED_SPLIT))
+        return fail(ctx, RT_ERR_ARG, "rt_render_whitted: RT_WHITTED_ONE_PASS and RT_WHITTED_SPLIT together");
     const SceneDesc &d = b->desc[i];
     const bool have = i < b->shading.size();
     if (have && (b->shading[i].nv != d.nv || b->shading[i].nt != d.nt) &&
@@ -1209,6 +1212,16 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     return RT_OK;
 }
 
+// The Whitted pass forms the autotuner tries: 0 per-pass wavefront,
+// 1 RT_WHITTED_ONE_PASS, 2 RT_WHITTED_SPLIT.
+static uint32_t whitted_pass_form(uint32_t switches, int form)
+{
+    switches &= ~(uint32_t)(RT_WHITTED_ONE_PASS | RT_WHITTED_SPLIT);
+    if (form == 1) switches |= RT_WHITTED_ONE_PASS;
+    if (form == 2) switches |= RT_WHITTED_SPLIT;
+    return switches;
+}
+
 int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const char *cache_path, rt_tuning *best)
 {
     Batch *b;
@@ -1218,19 +1231,18 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     char key[192];
     snprintf(key, sizeof(key), "whitted:cu%d:%s:t%u:d%u:%ux%u:b%u:s%u", ctx->cu_count,
              uses_bvh(*b, i) ? "bvh" : "lbvh", b->desc[i].nt, trav_depth(*b, i), desc->width, desc->height,
-             desc->bounces, desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+             desc->bounces, whitted_pass_form(desc->switches, 0));
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "r")) {
             char k[256];
             rt_tuning t;
-            int one = 0;
+            int pf = 0;
             while (fscanf(f, "%255s %d %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal,
-                          &one) == 6) {
-                if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK) {
+                          &pf) == 6) {
+                if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK && pf >= 0 && pf <= 2) {
                     fclose(f);
                     ctx->tune = t;
-                    desc->switches = one ? (desc->switches | RT_WHITTED_ONE_PASS)
-                                         : (desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+                    desc->switches = whitted_pass_form(desc->switches, pf);
                     if (best) *best = t;
                     return RT_OK;
                 }
@@ -1240,12 +1252,12 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     }
     const rt_tuning saved = ctx->tune;
     rt_tuning bt = saved;
-    int bone = 0;
+    int bpf = 0;
     float bms = 1e30f;
     int rc = RT_OK;
     rt_whitted_desc d = *desc;
-    for (int one = 0; one < 2 && rc == RT_OK; ++one) {
-        d.switches = one ? (desc->switches | RT_WHITTED_ONE_PASS) : (desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+    for (int pf = 0; pf < 3 && rc == RT_OK; ++pf) {
+        d.switches = whitted_pass_form(desc->switches, pf);
         for (int form = 0; form < kForms && rc == RT_OK; ++form) {
             for (const rt_tuning &shape : kShapes) {
                 rt_tuning t = shape;
@@ -1262,18 +1274,18 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
                 if (ms < bms) {
                     bms = ms;
                     bt = t;
-                    bone = one;
+                    bpf = pf;
                 }
             }
         }
     }
     ctx->tune = rc ? saved : bt;
     if (rc) return rc;
-    desc->switches = bone ? (desc->switches | RT_WHITTED_ONE_PASS) : (desc->switches & ~(uint32_t)RT_WHITTED_ONE_PASS);
+    desc->switches = whitted_pass_form(desc->switches, bpf);
     if (best) *best = bt;
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "a")) {
-            fprintf(f, "%s %d %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal, bone);
+            fprintf(f, "%s %d %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal, bpf);
             fclose(f);
         }
     }

@@ -1334,6 +1334,49 @@ __global__ __launch_bounds__(BLOCK) void k_whitted_next(SceneView sv, ShadeView 
     count_add(stats, 3, nbounce);
 }
 
+// Split form, after k_whitted_first: each live path of queue qin runs all of
+// its remaining passes in one lane.  The bounce passes of the wavefront form
+// each hold few rays (a fraction of a GPU's waves), so each lasts as long as
+// its slowest ray; one launch for every bounce pays that tail once.  Same
+// passes in the same order per pixel as the other forms, so the same frame.
+template <int BLOCK, int KLDS, int FORM>
+__global__ __launch_bounds__(BLOCK) void k_whitted_rest(SceneView sv, ShadeView sh, WhittedParams wp,
+                                                        float4 *__restrict__ rgba, const RayRec *__restrict__ qin,
+                                                        const uint32_t *in_count, int count,
+                                                        unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    int32_t root_ref;
+    float root_box[6];
+    load_root(sv.hdr, root_ref, root_box);
+    WCount cnt;
+    uint32_t nbounce = 0;
+    const uint32_t n = *in_count;
+    for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        if (i < n) {
+            WRay r = get_ray(qin, i);
+            const float4 a = rgba[r.pix];
+            float acc[4] = {a.x, a.y, a.z, a.w};
+            bool white = false;
+            for (uint32_t pass = 1; pass <= wp.bounces; ++pass) {
+                nbounce += 1;
+                if (!whitted_pass<FORM>(sv, sh, wp, root_ref, root_box, r, acc, white, nullptr, nullptr, stk, cnt))
+                    break;
+            }
+            rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+    }
+    if (count) {
+        count_add(stats, 0, cnt.boxes);
+        count_add(stats, 1, cnt.ntris);
+    }
+    count_add(stats, 2, cnt.nshadow);
+    count_add(stats, 3, nbounce);
+}
+
 __global__ void k_zero_words(uint32_t *p, uint32_t n)
 {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
@@ -1647,6 +1690,7 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     const SceneView sv = view_of(b, scene);
     const int keep = (flags & RT_FLAG_KEEP_HITS) ? 1 : 0, cnt = (flags & RT_FLAG_COUNT) ? 1 : 0;
     const bool one_pass = (d.switches & RT_WHITTED_ONE_PASS) != 0 || d.bounces == 0;
+    const bool split = (d.switches & RT_WHITTED_SPLIT) != 0;
     const uint64_t px = (uint64_t)d.width * d.height;
     RayRec *q[2] = {static_cast<RayRec *>(queue), static_cast<RayRec *>(queue) + px};
     if (!one_pass) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, qcount, d.bounces + 1);
@@ -1670,10 +1714,14 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
             uint32_t blocks = (uint32_t)((px + B - 1) / B);                                               \
             const uint32_t cap = (uint32_t)cu_count * 8u;                                                 \
             if (blocks > cap) blocks = cap;                                                               \
-            for (uint32_t p = 1; p <= d.bounces; ++p)                                                     \
-                hipLaunchKernelGGL((k_whitted_next<B, K, F>), dim3(blocks), dim3(B), 0, st, sv, sh, wp,   \
-                                   fb.rgba, q[(p - 1) & 1], qcount + (p - 1), q[p & 1], qcount + p,       \
-                                   p == d.bounces ? 1 : 0, cnt, stats);                                   \
+            if (split)                                                                                    \
+                hipLaunchKernelGGL((k_whitted_rest<B, K, F>), dim3(blocks), dim3(B), 0, st, sv, sh, wp,   \
+                                   fb.rgba, q[0], qcount, cnt, stats);                                    \
+            else                                                                                          \
+                for (uint32_t p = 1; p <= d.bounces; ++p)                                                 \
+                    hipLaunchKernelGGL((k_whitted_next<B, K, F>), dim3(blocks), dim3(B), 0, st, sv, sh,   \
+                                       wp, fb.rgba, q[(p - 1) & 1], qcount + (p - 1), q[p & 1],           \
+                                       qcount + p, p == d.bounces ? 1 : 0, cnt, stats);                   \
         }                                                                                                 \
     }
     RT_TUNED(RT_L)

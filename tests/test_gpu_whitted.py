@@ -11,7 +11,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 F = np.float32
-SHADOWS, PHONG, GLOSS, ONE_PASS = 1, 2, 4, 8
+SHADOWS, PHONG, GLOSS, ONE_PASS, SPLIT = 1, 2, 4, 8, 16
+FORM_BITS = {"wavefront": 0, "one-pass": ONE_PASS, "split": SPLIT}
 
 
 def psnr(a, b):
@@ -95,15 +96,15 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("form", ["wavefront", "one-pass"])
+@pytest.mark.parametrize("form", ["wavefront", "one-pass", "split"])
 @pytest.mark.parametrize("tag,kw", CASES, ids=[c[0] for c in CASES])
 def test_whitted_room_matches_oracle(ctx, oracle, rtcore, room, tag, kw, form):
-    """Both forms of the device frame (a compacted queue per bounce pass, or
-    one lane per pixel through every pass) against the oracle."""
+    """The three forms of the device frame (a compacted queue per bounce
+    pass; one lane per pixel through every pass; pass 0 per pixel, then one
+    lane per live path through the rest) against the oracle."""
     s, xyz, idx, normals, mat_ids, mats, ref = room
     kw = dict(kw)
-    if form == "one-pass":
-        kw["switches"] = kw["switches"] | ONE_PASS
+    kw["switches"] = kw["switches"] | FORM_BITS[form]
     out = run_pair(ctx, oracle, rtcore, s, xyz, idx, ref, 160, 120, (0.4, -1.7, 0.9), (0.8, -0.6, 1.5),
                    normals=normals, mat_ids=mat_ids, mats=mats, **kw)
     check(f"{tag} [{form}]", *out)
@@ -112,23 +113,25 @@ def test_whitted_room_matches_oracle(ctx, oracle, rtcore, room, tag, kw, form):
 
 
 def test_whitted_forms_identical_at_1080p(ctx, rtcore, room):
-    """The two device forms give identical frames, counters included, on a
+    """The three device forms give identical frames, counters included, on a
     full-size image; then a smaller image reuses the larger queues."""
     s = room[0]
     for w, h in ((1920, 1080), (333, 197)):
         cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
         kw = dict(bounces=4, env=(0.2, 0.3, 0.4), seed=3)
-        a = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | PHONG | GLOSS,
-                               flags=rtcore.FLAG_COUNT, **kw)
-        fa = ctx.frame_read(w, h, 1)[0]
-        b = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | PHONG | GLOSS | ONE_PASS,
-                               flags=rtcore.FLAG_COUNT, **kw)
-        fb = ctx.frame_read(w, h, 1)[0]
-        print(f"{w}x{h}: wavefront {a['trace_ms']:.3f} ms, one-pass {b['trace_ms']:.3f} ms, "
-              f"rays {a['primary_rays'] + a['shadow_rays'] + a['bounce_rays']}", flush=True)
-        assert np.array_equal(fa, fb)
-        for k in ("primary_rays", "shadow_rays", "bounce_rays", "box_tests", "tri_tests"):
-            assert a[k] == b[k], k
+        out = {}
+        for form, bit in FORM_BITS.items():
+            st = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | PHONG | GLOSS | bit,
+                                    flags=rtcore.FLAG_COUNT, **kw)
+            out[form] = (st, ctx.frame_read(w, h, 1)[0])
+        a, fa = out["wavefront"]
+        print(f"{w}x{h}: " + ", ".join(f"{k} {v[0]['trace_ms']:.3f} ms" for k, v in out.items()) +
+              f", rays {a['primary_rays'] + a['shadow_rays'] + a['bounce_rays']}", flush=True)
+        for form in ("one-pass", "split"):
+            b, fb = out[form]
+            assert np.array_equal(fa, fb), form
+            for k in ("primary_rays", "shadow_rays", "bounce_rays", "box_tests", "tri_tests"):
+                assert a[k] == b[k], (form, k)
 
 
 def test_whitted_repeated_frames_identical(ctx, rtcore, room):
@@ -138,7 +141,7 @@ def test_whitted_repeated_frames_identical(ctx, rtcore, room):
     w, h = 640, 360
     cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
     kw = dict(bounces=3, env=(0.2, 0.3, 0.4), seed=5)
-    for sw in (SHADOWS | PHONG, SHADOWS | PHONG | ONE_PASS):
+    for sw in (SHADOWS | PHONG, SHADOWS | PHONG | ONE_PASS, SHADOWS | PHONG | SPLIT):
         ref = None
         for rep in range(3):
             st = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=sw, flags=rtcore.FLAG_COUNT, **kw)
@@ -235,14 +238,16 @@ def test_autotune_whitted(ctx, rtcore, room, tmp_path):
     kw = dict(bounces=3, switches=SHADOWS | PHONG, env=(0.2, 0.3, 0.4))
     cache = tmp_path / "tune.txt"
     t1, sw1 = ctx.autotune_whitted(s, cam, w, h, (0.8, -0.6, 1.5), cache_path=str(cache), **kw)
-    assert sw1 & ~ONE_PASS == kw["switches"] and len(cache.read_text().splitlines()) == 1
+    assert sw1 & ~(ONE_PASS | SPLIT) == kw["switches"] and len(cache.read_text().splitlines()) == 1
     t2, sw2 = ctx.autotune_whitted(s, cam, w, h, (0.8, -0.6, 1.5), cache_path=str(cache), **kw)
     assert (t1, sw1) == (t2, sw2) and len(cache.read_text().splitlines()) == 1
     ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), **dict(kw, switches=sw1))
     a = ctx.frame_read(w, h, 1)[0]
-    ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), **dict(kw, switches=sw1 ^ ONE_PASS))
-    b = ctx.frame_read(w, h, 1)[0]
-    assert np.array_equal(a, b)
+    for bit in FORM_BITS.values():
+        ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), **dict(kw, switches=kw["switches"] | bit))
+        assert np.array_equal(a, ctx.frame_read(w, h, 1)[0]), bit
+    with pytest.raises(rtcore.RtError):  # the two forms exclude each other
+        ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), **dict(kw, switches=kw["switches"] | ONE_PASS | SPLIT))
     ctx.set_tuning(128, 26, 8, 0)
 
 
