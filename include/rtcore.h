@@ -112,7 +112,12 @@ enum {
     RT_FLAG_COUNT = 1u << 1,     /* count box / triangle tests into stats */
     RT_FLAG_KEEP_HITS = 1u << 2, /* keep per-pixel primary hits + shadow results */
     RT_FLAG_TIMING = 1u << 3,    /* time the call with HIP events (synchronises) */
-    RT_FLAG_PERSISTENT = 1u << 4 /* rt_render_jobs: one persistent launch over all tiles */
+    RT_FLAG_PERSISTENT = 1u << 4, /* rt_render_jobs: one persistent launch over all tiles */
+    RT_FLAG_START_AT_IROOT = 1u << 5 /* rt_trace: ray i starts at heap node hits[i].root, the
+                                        previous intersection's iRoot, as CSComputeIntersections
+                                        passes it to LBVH_IntersectP (CSIntersections.hlsl:32);
+                                        hits are read, then overwritten; 0 and 1 are the root, a
+                                        node below a leaf starts at that leaf; LBVH only */
 };
 
 /* Launch shape of the traversal kernels (rt_autotune_jobs picks one). */

@@ -777,6 +777,8 @@ int rt_trace(rt_ctx *ctx, int scene, const rt_ray48 *rays, uint32_t n, rt_hit24 
     if (!get_scene(ctx, scene, &b, &i)) return fail(ctx, RT_ERR_ARG, "rt_trace: unknown scene %d", scene);
     if (!traversable(*b, i)) return fail(ctx, RT_ERR_STATE, "rt_trace: scene %d not built", scene);
     if (n && (!rays || !hits)) return fail(ctx, RT_ERR_ARG, "rt_trace: null array");
+    if ((flags & RT_FLAG_START_AT_IROOT) && uses_bvh(*b, i))
+        return fail(ctx, RT_ERR_STATE, "rt_trace: RT_FLAG_START_AT_IROOT needs the LBVH's heap node numbers");
     hipSetDevice(ctx->device);
     const uint64_t rb = (sizeof(rt_ray48) * (uint64_t)n + 255) & ~255ull;
     int rc = ensure_staging(ctx, rb + sizeof(rt_hit24) * (uint64_t)n + 256);
@@ -786,6 +788,9 @@ int rt_trace(rt_ctx *ctx, int scene, const rt_ray48 *rays, uint32_t n, rt_hit24 
     if (n)
         RT_HIP(ctx, hipMemcpyAsync(d_rays, rays, sizeof(rt_ray48) * (uint64_t)n,
                                    hipMemcpyHostToDevice, ctx->stream));
+    if (n && (flags & RT_FLAG_START_AT_IROOT))  // the start nodes travel in the hits
+        RT_HIP(ctx, hipMemcpyAsync(d_hits, hits, sizeof(rt_hit24) * (uint64_t)n, hipMemcpyHostToDevice,
+                                   ctx->stream));
     rc = trace_common(ctx, b, i, d_rays, n, d_hits, flags, stats);
     if (rc) return rc;
     if (n)
@@ -805,6 +810,8 @@ int rt_trace_device(rt_ctx *ctx, int scene, const rt_ray48 *d_rays, uint32_t n, 
     if (n && (!d_rays || !d_hits)) return fail(ctx, RT_ERR_ARG, "rt_trace_device: null array");
     if (((uintptr_t)d_rays & 15) || ((uintptr_t)d_hits & 7))
         return fail(ctx, RT_ERR_ARG, "rt_trace_device: rays need 16-byte, hits 8-byte alignment");
+    if ((flags & RT_FLAG_START_AT_IROOT) && uses_bvh(*b, i))
+        return fail(ctx, RT_ERR_STATE, "rt_trace_device: RT_FLAG_START_AT_IROOT needs the LBVH's heap node numbers");
     hipSetDevice(ctx->device);
     return trace_common(ctx, b, i, d_rays, n, d_hits, flags, stats);
 }

@@ -357,6 +357,130 @@ __device__ __forceinline__ TraceOut traverse_shared(const TravNode *__restrict__
     return r;
 }
 
+// The start of a traversal at heap node `start` (LBVH_IntersectP's start
+// argument, LBVH.hlsl:363-370: 0 and 1 are the root).  The trail algorithm
+// tests the start node's box, then, on its way back up, the box of the
+// sibling at each level from the start's own up to the root's children.  Here
+// the walk goes down from the root along the start's heap bits: each
+// off-path sibling's box is tested and counted now and pushed when live (so
+// the pops meet them deepest first, culled against the then-current best t,
+// as the trail algorithm meets them), then the start node's box is tested.
+// A start below a leaf stops at that leaf.  Returns the node to visit first,
+// or -1 when nothing is left.
+template <class Stack>
+__device__ __forceinline__ int32_t trav_enter(const TravNode *__restrict__ nodes, int32_t root_ref,
+                                              const float *root_box, uint32_t start, float ox, float oy,
+                                              float oz, float ix, float iy, float iz, TraceOut &r, Stack &stk)
+{
+    stk.sp = 0;
+    int32_t ref = root_ref;
+    float box[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) box[k] = root_box[k];
+    r.boxes = 0;
+    if (start > 1u) {
+        const int level = 31 - __clz(start);
+        for (int k = level - 1; k >= 0 && ref >= 0; --k) {
+            const TravNode *n = &nodes[ref];
+            const float4 q0 = *reinterpret_cast<const float4 *>(&n->box[0]);
+            const float4 q1 = *reinterpret_cast<const float4 *>(&n->box[4]);
+            const float4 q2 = *reinterpret_cast<const float4 *>(&n->box[8]);
+            const int4 q3 = *reinterpret_cast<const int4 *>(&n->ref[0]);
+            const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};  // child 2n
+            const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};  // child 2n + 1
+            const bool right = (start >> k) & 1u;
+            float sb[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) sb[j] = right ? ba[j] : bb[j];  // the sibling's box
+            float t0, t1;
+            box_test(sb, ox, oy, oz, ix, iy, iz, t0, t1);
+            r.boxes += 1;
+            if (!(t1 < 0.0f) && !(t0 > r.t)) stk.push(make_uint2((uint32_t)(right ? q3.x : q3.y), __float_as_uint(t0)));
+            ref = right ? q3.y : q3.x;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) box[j] = right ? bb[j] : ba[j];
+        }
+    }
+    float t0, t1;
+    box_test(box, ox, oy, oz, ix, iy, iz, t0, t1);
+    r.boxes += 1;
+    if (!(t0 > r.t || t1 < 0.0f)) return ref;
+    while (stk.sp > 0) {
+        const uint2 e = stk.pop();
+        if (!(__uint_as_float(e.y) > r.t)) return (int32_t)e.x;
+    }
+    return -1;
+}
+
+// traverse_shared from heap node `start` (rt_trace with RT_FLAG_START_AT_IROOT):
+// the same loop after trav_enter.  With start 1 it visits what traverse()
+// visits; from a leaf it visits what the reference's trail loop visits from
+// that leaf, in the same order, so hits and counters equal the oracle's.
+template <bool ANYHIT, class Stack>
+__device__ __forceinline__ TraceOut traverse_from(const TravNode *__restrict__ nodes,
+                                                  const TravTri *__restrict__ tris, int32_t root_ref,
+                                                  const float *root_box, uint32_t start, float ox, float oy,
+                                                  float oz, float dx, float dy, float dz, float tmax, int32_t self,
+                                                  Stack &stk)
+{
+    TraceOut r;
+    r.tri = -1;
+    r.u = -1.0f;
+    r.v = -1.0f;
+    r.t = tmax;
+    r.boxes = 0;
+    r.tris = 0;
+    r.leaf_heap = 1;
+    r.pos = -1;
+    if (root_ref == -1) return r;
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const uint32_t neg = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
+    int32_t cur = trav_enter(nodes, root_ref, root_box, start, ox, oy, oz, ix, iy, iz, r, stk);
+    if (cur == -1) return r;
+    while (true) {
+        float4 q0, q1, q2, q3f;
+        fetch_step(nodes, tris, cur, q0, q1, q2, q3f);
+        if (cur >= 0) {
+            const int4 q3 = as_int4(q3f);
+            const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+            const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            float a0, a1, b0, b1;
+            box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
+            box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+            r.boxes += 2;
+            const int first_is_b = (neg >> q3.z) & 1u;
+            const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
+            const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+            const int32_t fref = first_is_b ? q3.y : q3.x;
+            const int32_t sref = first_is_b ? q3.x : q3.y;
+            const bool f_ok = !(f0 > r.t || f1 < 0.0f);
+            const bool s_live = !(s1 < 0.0f);
+            if (f_ok) {
+                if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+                cur = fref;
+                continue;
+            }
+            if (s_live && !(s0 > r.t)) {
+                cur = sref;
+                continue;
+            }
+        } else {
+            if (leaf_test_first(r, tris, (uint32_t)cur, q0, q1, q2, ox, oy, oz, dx, dy, dz, self, ANYHIT)) return r;
+        }
+        bool found = false;
+        while (stk.sp > 0) {
+            const uint2 e = stk.pop();
+            if (!(__uint_as_float(e.y) > r.t)) {
+                cur = (int32_t)e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return r;
+}
+
 // The same traversal written as a step function: trav_begin tests the root
 // box; each trav_step fetches with fetch_step, handles one internal node (both
 // child boxes) or one leaf and pops the stack; a ray is finished when either
@@ -520,7 +644,7 @@ __device__ __forceinline__ void store_hit(rt_hit24 *dst, const TraceOut &o)
 // rt_trace: one lane per ray.
 template <bool ANYHIT, bool COUNT, int BLOCK, int KLDS, int FORM>
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const rt_ray48 *__restrict__ rays,
-                                                 uint32_t n, rt_hit24 *__restrict__ hits,
+                                                 uint32_t n, rt_hit24 *hits, int from_root_field,
                                                  unsigned long long *stats)
 {
     __shared__ uint2 lds[KLDS * BLOCK];
@@ -548,8 +672,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const rt_ray48 *_
             d.leaf_heap = 1;
             store_hit(&hits[i], d);
         } else {
-            const TraceOut o = traverse_form<FORM, ANYHIT>(sv.nodes, sv.tris, root_ref, root_box, r0.x, r0.y,
-                                                r0.z, r0.w, r1.x, r1.y, r2.z, self, stk);
+            TraceOut o;
+            if (from_root_field) {  // the previous intersection's iRoot, read before it is overwritten
+                const uint32_t start = (uint32_t)hits[i].root;
+                o = traverse_from<ANYHIT>(sv.nodes, sv.tris, root_ref, root_box, start, r0.x, r0.y, r0.z, r0.w,
+                                          r1.x, r1.y, r2.z, self, stk);
+            } else {
+                o = traverse_form<FORM, ANYHIT>(sv.nodes, sv.tris, root_ref, root_box, r0.x, r0.y, r0.z, r0.w,
+                                                r1.x, r1.y, r2.z, self, stk);
+            }
             store_hit(&hits[i], o);
             boxes = o.boxes;
             ntris = o.tris;
@@ -1586,13 +1717,14 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
     const SceneView sv = view_of(b, scene);
     const uint32_t max_depth = trav_depth(b, scene);
     const bool any = flags & RT_FLAG_ANYHIT, cnt = flags & RT_FLAG_COUNT;
+    const int from_root = (flags & RT_FLAG_START_AT_IROOT) ? 1 : 0;
 #define RT_L(B, K, F)                                                                               \
     {                                                                                              \
         const dim3 grid((n + B - 1) / B), block(B);                                                \
-        if (any && cnt) hipLaunchKernelGGL((k_trace<true, true, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
-        else if (any) hipLaunchKernelGGL((k_trace<true, false, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
-        else if (cnt) hipLaunchKernelGGL((k_trace<false, true, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
-        else hipLaunchKernelGGL((k_trace<false, false, B, K, F>), grid, block, 0, st, sv, rays, n, hits, stats); \
+        if (any && cnt) hipLaunchKernelGGL((k_trace<true, true, B, K, F>), grid, block, 0, st, sv, rays, n, hits, from_root, stats); \
+        else if (any) hipLaunchKernelGGL((k_trace<true, false, B, K, F>), grid, block, 0, st, sv, rays, n, hits, from_root, stats); \
+        else if (cnt) hipLaunchKernelGGL((k_trace<false, true, B, K, F>), grid, block, 0, st, sv, rays, n, hits, from_root, stats); \
+        else hipLaunchKernelGGL((k_trace<false, false, B, K, F>), grid, block, 0, st, sv, rays, n, hits, from_root, stats); \
     }
     RT_TUNED(RT_L)
 #undef RT_L

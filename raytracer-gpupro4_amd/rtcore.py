@@ -30,6 +30,7 @@ FLAG_COUNT = 1 << 1
 FLAG_KEEP_HITS = 1 << 2
 FLAG_TIMING = 1 << 3
 FLAG_PERSISTENT = 1 << 4
+FLAG_START_AT_IROOT = 1 << 5
 MODE_DIRECT = 0
 MODE_GI = 1
 WHITTED_SHADOWS = 1 << 0
@@ -590,9 +591,15 @@ class Context:
         return nodes, codes, ids, box
 
     # trace
-    def trace(self, scene: int, rays: np.ndarray, flags: int = 0, want_stats: bool = False):
+    def trace(self, scene: int, rays: np.ndarray, flags: int = 0, want_stats: bool = False, start=None):
+        """Closest (or any, FLAG_ANYHIT) hit per ray.  start: per-ray heap node
+        to begin at (the previous hits' "root", LBVH_IntersectP's iRoot); sets
+        FLAG_START_AT_IROOT."""
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.empty(len(rays), HIT_DTYPE)
+        if start is not None:
+            hits["root"] = np.broadcast_to(np.asarray(start, np.int32), len(rays))
+            flags |= FLAG_START_AT_IROOT
         st = FrameStats()
         self.check(self.L.rt_trace(self.h, scene, _ptr(rays), len(rays), _ptr(hits), flags,
                                    C.byref(st) if want_stats else None), "rt_trace")

@@ -270,3 +270,82 @@ def test_deep_heap_under_every_shape(ctx, oracle, rtcore):
     finally:
         ctx.set_tuning(*saved)
     ctx.scene_destroy(s)
+
+
+def test_trace_from_iroot_matches_oracle(ctx, oracle, rtcore):
+    """RT_FLAG_START_AT_IROOT: each ray starts at the heap node in its hit
+    record (the previous intersection's iRoot), as CSComputeIntersections
+    starts LBVH_IntersectP.  Bounce-like and shadow rays from the hit points,
+    started at the hit leaves, equal the oracle's trail loop from the same
+    nodes: ids, u, v, t, the visit counter and the new iRoot, closest and any
+    hit, under a spilling and a non-spilling launch shape.  Starts 0 and 1 are
+    the root; a start below a leaf starts at that leaf.  Any hit: the same
+    hits, and box counts at least the trail loop's, as from the root."""
+    xyz, idx = rtcore.gen_triangle_soup(60000, 0.008, seed=12)
+    s = ctx.add_scene(xyz, idx, 18)
+    ctx.build(s)
+    ref = oracle.build_scene(xyz, idx, 18)
+    nodes = ref["nodes"]
+    cam = rtcore.camera_look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0, 1, 0), 60.0, 1.0)
+    prim = oracle.raygen_pinhole(cam.as_array(), 128, 128)
+    h0 = ctx.trace(s, prim)
+    hit = h0["tri"] >= 0
+    assert hit.mean() > 0.2 and (h0["root"][hit] > 1).all()
+    p = prim["o"][hit] + h0["t"][hit][:, None] * prim["d"][hit]
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(len(p), 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    bounce = np.zeros(len(p), oracle.RAY_DTYPE)
+    bounce["o"], bounce["d"], bounce["refl"], bounce["tmax"], bounce["tri"] = p, d, 1.0, 1e16, h0["tri"][hit]
+    L = np.array([0.5, 2.0, -1.0], np.float32)
+    ld = (L - p).astype(np.float32)
+    dist = np.linalg.norm(ld, axis=1).astype(np.float32)
+    shadow = bounce.copy()
+    shadow["d"] = (ld / dist[:, None]).astype(np.float32)
+    shadow["tmax"] = dist
+    start = h0["root"][hit]
+    saved = ctx.tuning()
+    try:
+        for shape in [(128, 20, 4, 2), (64, 16, 4, 0)]:
+            ctx.set_tuning(*shape)
+            for rays, anyhit in ((bounce, False), (shadow, False), (shadow, True)):
+                fl = rtcore.FLAG_ANYHIT if anyhit else 0
+                g = ctx.trace(s, rays, flags=fl, start=start)
+                r = oracle.trace(nodes, ref["ids"], xyz, idx, rays, start=start, anyhit=anyhit, nthreads=8)
+                if not anyhit:
+                    assert np.array_equal(g, r), (shape, anyhit)
+                else:  # an early stop: boxes the device tested up front may be more (as from the root)
+                    for f in ("tri", "u", "v", "t", "root"):
+                        assert np.array_equal(g[f], r[f]), (shape, f)
+                    assert np.array_equal(g["visited"] >> 16, r["visited"] >> 16)
+                    assert np.all((g["visited"] & 0xFFFF) >= (r["visited"] & 0xFFFF))
+                from_root = oracle.trace(nodes, ref["ids"], xyz, idx, rays, anyhit=anyhit, nthreads=8)
+                assert not np.array_equal(g["visited"], from_root["visited"])  # the start matters
+        # 0 and 1 are the root
+        plain = ctx.trace(s, bounce)
+        for root in (0, 1):
+            assert np.array_equal(ctx.trace(s, bounce, start=np.full(len(bounce), root, np.int32)), plain)
+        # below a leaf: that leaf (the first leaf on the path of the start's bits)
+        leaf = (nodes["prim_count"] & 4) != 0
+        odd = np.concatenate([2 * start, 4 * start + 3, np.full(8, (1 << 30) + 12345, np.int64)]).astype(np.int64)
+        clamped = []
+        for v in odd:
+            lv = int(v).bit_length() - 1
+            node = 1
+            for k in range(lv - 1, -1, -1):
+                if leaf[node]:
+                    break
+                node = 2 * node + ((int(v) >> k) & 1)
+            clamped.append(node)
+        many = np.concatenate([bounce, bounce, bounce[:8]])
+        g = ctx.trace(s, many, start=odd.astype(np.int32))
+        r = oracle.trace(nodes, ref["ids"], xyz, idx, many, start=np.array(clamped, np.int32), nthreads=8)
+        assert np.array_equal(g, r)
+    finally:
+        ctx.set_tuning(*saved)
+    bvh = ctx.add_scene(xyz[:3000], idx[:1000] % 3000, 12)
+    ctx.bvh_build(bvh)
+    with pytest.raises(rtcore.RtError):
+        ctx.trace(bvh, bounce[:4], start=np.ones(4, np.int32))
+    ctx.scene_destroy(bvh)
+    ctx.scene_destroy(s)

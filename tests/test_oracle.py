@@ -327,6 +327,45 @@ def test_trace_shadow_start_leaf_same_occlusion(oracle, rtcore):
     assert np.all(anyhit["t"][occ_any] >= from_root["t"][occ_any])
 
 
+def test_trace_from_any_start_node_finds_the_closest_hit(oracle, rtcore):
+    """From any start node the trail loop visits every leaf (the start's
+    subtree, then the sibling subtree at each level up to the root's
+    children), so the closest t equals brute force from every start: the hit
+    leaf of a first trace, the root, an internal node and a leaf's sibling.
+    Counters: a start at node s tests level(s) boxes before any descent (s and
+    the sibling at each level below the root), so from the root they equal
+    the plain trace's and from elsewhere they differ."""
+    xyz, idx = rtcore.gen_triangle_soup(4000, 0.02, seed=11)
+    b = oracle.build_scene(xyz, idx, 16)
+    rng = np.random.default_rng(4)
+    n = 1500
+    o = rng.uniform(-0.1, 1.1, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays = _rays(o, d)
+    brute = oracle.trace_brute(xyz, idx, rays)
+    first = oracle.trace(b["nodes"], b["ids"], xyz, idx, rays)
+    assert (brute["tri"] >= 0).mean() > 0.15
+    leaf = (b["nodes"]["prim_count"] & 4) != 0
+    inner = np.flatnonzero(~leaf[2:64] & (b["nodes"]["prim_count"][2:64] > 0)) + 2
+    starts = {"hit leaf": np.where(first["root"] > 1, first["root"], 1),
+              "root": np.ones(n, np.int32),
+              "internal": np.full(n, inner[0], np.int32),
+              "sibling of hit leaf": np.where(first["root"] > 1, first["root"] ^ 1, 1)}
+    for name, st in starts.items():
+        h = oracle.trace(b["nodes"], b["ids"], xyz, idx, rays, start=st)
+        assert np.array_equal(h["t"], brute["t"]), name
+        same = h["tri"] == brute["tri"]
+        assert same.mean() > 0.999, name  # the rest are exact ties in t
+        if name == "root":
+            assert np.array_equal(h["visited"], first["visited"])
+    # boxes tested before any descent from a start at level L: L (s and L - 1 siblings)
+    tiny = _rays(np.array([[5.0, 5.0, 5.0]]), np.array([[1.0, 0.0, 0.0]]))  # misses every box
+    for s in (1, 2, 3, int(inner[0]), 37):
+        h = oracle.trace(b["nodes"], b["ids"], xyz, idx, tiny, start=np.array([s], np.int32))
+        assert (h["visited"][0] & 0xFFFF) == int(s).bit_length(), s
+
+
 # ----------------------------------------------------------------- rays/shade
 def test_raygen_numpy(oracle, rtcore):
     cam = rtcore.camera_look_at((0.3, 0.5, 3.0), (0.1, 0.0, -0.2), (0, 1, 0), 60.0, 640 / 360)
