@@ -243,6 +243,52 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(
 // LBVH.cpp:520-553), identity ids (batch-global triangle index), and the digit
 // histograms of the four code passes.
 // ---------------------------------------------------------------------------
+// A scene's box from its k_bounds block partials.  Scenes of at most
+// kBoxInline blocks let every k_morton block reduce the partials itself (a
+// few KB each, no extra launch); larger ones would make that O(blocks^2), so
+// one block of this kernel per scene reduces them once into the scene header.
+constexpr uint32_t kBoxInline = 512;
+
+__global__ __launch_bounds__(1024) void k_scene_box(const SceneDesc *__restrict__ desc,
+                                                    const float *__restrict__ partials,
+                                                    SceneHdr *__restrict__ hdr)
+{
+    __shared__ float red[16 * 6];
+    const SceneDesc D = desc[blockIdx.x];
+    if (D.nblocks <= kBoxInline) return;  // reduced inline by k_morton
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    constexpr uint32_t kUnroll = 8;
+    for (uint32_t j0 = 0; j0 < D.nblocks; j0 += 1024 * kUnroll) {
+        float v[kUnroll][6];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {  // all loads of the batch before any use
+            const uint32_t j = j0 + u * 1024 + threadIdx.x;
+            const float *p = partials + 6 * ((size_t)D.block_off + (j < D.nblocks ? j : 0));
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[u][k] = p[k];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            if (j0 + u * 1024 + threadIdx.x < D.nblocks) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = fminf(lo[k], v[u][k]);
+                    hi[k] = fmaxf(hi[k], v[u][3 + k]);
+                }
+            }
+        }
+    }
+    block_box_reduce<16>(lo, hi, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            hdr[blockIdx.x].scene_box[k] = lo[k];
+            hdr[blockIdx.x].scene_box[3 + k] = hi[k];
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBoundsThreads) void k_morton(
     const SceneDesc *__restrict__ desc, uint32_t S, const float *__restrict__ cent,
     const float *__restrict__ partials, uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
@@ -254,22 +300,31 @@ __global__ __launch_bounds__(kBoundsThreads) void k_morton(
     const SceneDesc D = desc[s];
     float lo[3] = {INFINITY, INFINITY, INFINITY};
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t j = threadIdx.x; j < D.nblocks; j += kBoundsThreads) {
-        const float *p = partials + 6 * ((size_t)D.block_off + j);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], p[k]);
-            hi[k] = fmaxf(hi[k], p[3 + k]);
-        }
-    }
-    for (int j = threadIdx.x; j < kCodePasses * kRadix; j += kBoundsThreads) lhist[j] = 0;
-    block_box_reduce<4>(lo, hi, red);
     const uint32_t lb = blockIdx.x - D.block_off;
-    if (lb == 0 && threadIdx.x == 0) {
+    for (int j = threadIdx.x; j < kCodePasses * kRadix; j += kBoundsThreads) lhist[j] = 0;
+    if (D.nblocks <= kBoxInline) {
+        for (uint32_t j = threadIdx.x; j < D.nblocks; j += kBoundsThreads) {
+            const float *p = partials + 6 * ((size_t)D.block_off + j);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fminf(lo[k], p[k]);
+                hi[k] = fmaxf(hi[k], p[3 + k]);
+            }
+        }
+        block_box_reduce<4>(lo, hi, red);
+        if (lb == 0 && threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                hdr[s].scene_box[k] = lo[k];
+                hdr[s].scene_box[3 + k] = hi[k];
+            }
+        }
+    } else {  // k_scene_box reduced them
+        __syncthreads();  // lhist cleared before the atomics below
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            hdr[s].scene_box[k] = lo[k];
-            hdr[s].scene_box[3 + k] = hi[k];
+            lo[k] = hdr[s].scene_box[k];
+            hi[k] = hdr[s].scene_box[3 + k];
         }
     }
     float m[3], f[3];
@@ -899,6 +954,9 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
     const uint32_t nb = b.total_blocks > 0 ? b.total_blocks : 1;
     hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(kBoundsThreads), 0, st, b.d_desc, S, b.total_blocks,
                        b.xyz, b.idx, b.trec, b.cent, b.partials, b.hist, nzero);
+    bool big = false;
+    for (uint32_t i = 0; i < S; ++i) big = big || b.desc[i].nblocks > kBoxInline;
+    if (big) hipLaunchKernelGGL(k_scene_box, dim3(S), dim3(1024), 0, st, b.d_desc, b.partials, b.hdr);
     if (ev) hipEventRecord(ev[1], st);
     const int start = (int)(passes & 1);  // the sorted result ends in buffer 0
     if (b.total_blocks > 0)
