@@ -16,9 +16,12 @@
 // Float arithmetic is IEEE single with no contraction (-ffp-contract=off):
 // IntersectBox and getIntersection (Intersection.hlsl:18-64) in source order.
 //
-// The stack keeps its top KLDS entries in LDS (one column per lane) and the
-// deeper ones in private memory; KLDS and the block size are launch-time
-// tunables (occupancy against spill traffic).
+// The stack keeps its first KLDS entries in LDS (one column per lane) and
+// any pushed beyond them in private memory; KLDS and the block size are
+// launch-time tunables (occupancy against spill traffic).  rt_trace can start
+// a ray at any heap node, as LBVH_IntersectP takes its iRoot argument
+// (trav_enter).  Frame kernels take their tiles heaviest first, from the
+// last frame's per-tile costs (k_tile_order).
 #include <cstring>
 
 #include "rt_internal.h"
