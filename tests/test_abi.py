@@ -90,3 +90,28 @@ def test_create_without_gpu_reports_status(rtcore):
         pytest.skip("a GPU is present")
     assert rc == rtcore.RT_ERR_NODEV and not h.value
     assert rtcore.lib().rt_last_error(None) == b"null context"
+
+
+def test_python_constants_match_the_header(rtcore):
+    """Every RT_* enum value of include/rtcore.h that rtcore.py mirrors (the
+    name without the RT_ prefix) has the header's value; the flags and
+    switches the tests and facades use are all mirrored."""
+    import os
+    import re
+
+    src = open(os.path.join(os.path.dirname(__file__), "..", "include", "rtcore.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    values = {}
+    for body in re.findall(r"enum\s*\w*\s*\{(.*?)\}", src, flags=re.S):
+        for name, expr in re.findall(r"(RT_[A-Z0-9_]+)\s*=\s*([^,]+)", body):
+            values[name] = eval(re.sub(r"(\d+)u\b", r"\1", expr.strip()), {}, {})
+    assert len(values) > 20
+    # status codes keep the prefix in Python (rtcore.RT_OK), the rest drop it
+    py = {n: (n if n.startswith(("RT_OK", "RT_ERR")) else n[3:]) for n in values}
+    mirrored = {n: v for n, v in values.items() if hasattr(rtcore, py[n])}
+    for n, v in mirrored.items():
+        assert getattr(rtcore, py[n]) == v, n
+    for n in ("RT_FLAG_ANYHIT", "RT_FLAG_COUNT", "RT_FLAG_KEEP_HITS", "RT_FLAG_PERSISTENT",
+              "RT_FLAG_START_AT_IROOT", "RT_WHITTED_ONE_PASS", "RT_WHITTED_SPLIT", "RT_BVH_SAH",
+              "RT_STRUCTURE_LBVH", "RT_MODE_GI", "RT_OK", "RT_ERR_BUDGET"):
+        assert n in mirrored, n
