@@ -128,6 +128,11 @@ typedef struct rt_tuning {
     int32_t traversal;     /* 0: loop form, 1: step form, 2: loop form fetching nodes and
                               leaves with one shared load (same visit order and results;
                               the fastest one depends on the scene) */
+    int32_t quad_tiles;    /* rt_render_direct of an LBVH scene without RT_FLAG_KEEP_HITS or
+                              RT_FLAG_COUNT: the quad_tiles heaviest tiles of the last frame's
+                              tile order get four lanes per pixel, each searching every fourth
+                              subtree of heap depth 8 along the ray (same image; shortens the
+                              longest pixels' chains).  0: off */
 } rt_tuning;
 
 /* One image of one scene for rt_render_jobs: spp samples per pixel are added
@@ -394,9 +399,11 @@ int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
 int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
                      const char *cache_path, rt_tuning *best);
 /* Times every launch shape and traversal form on rt_render_direct of this
- * scene and frame (renders into the context's frame buffer) and keeps the
- * fastest; the blocks_per_cu field is left as it is.  cache_path as for rt_autotune_jobs,
- * keyed by triangle count, depth, image size and light count. */
+ * scene and frame (renders into the context's frame buffer), then, for an
+ * LBVH scene, quad_tiles 64, 256 and 1024 with the fastest of them, and keeps
+ * the fastest; the blocks_per_cu field is left as it is.  cache_path as for
+ * rt_autotune_jobs, keyed by triangle count, depth, image size and light
+ * count. */
 int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc,
                        const char *cache_path, rt_tuning *best);
 /* Captures rebuild (optional) + one persistent rt_render_jobs launch of these

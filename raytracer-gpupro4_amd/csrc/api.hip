@@ -119,8 +119,7 @@ int ensure_frame(rt_ctx *ctx, uint32_t w, uint32_t h, uint32_t nl)
     hipFree(f.rgba);
     hipFree(f.hits);
     hipFree(f.occ);
-    hipFree(f.tile_order);
-    hipFree(f.tile_cost);
+    hipFree(f.order_mem);
     ctx->bytes -= f.bytes;
     f = rt::FrameBuffers();
     const uint32_t nlc = nl > 0 ? nl : 1;
@@ -128,12 +127,16 @@ int ensure_frame(rt_ctx *ctx, uint32_t w, uint32_t h, uint32_t nl)
     RT_HIP(ctx, hipMalloc(&f.rgba, sizeof(float4) * (px ? px : 1)));
     RT_HIP(ctx, hipMalloc(&f.hits, sizeof(rt_hit24) * (px ? px : 1)));
     RT_HIP(ctx, hipMalloc(&f.occ, (px ? px : 1) * nlc));
-    RT_HIP(ctx, hipMalloc(&f.tile_order, sizeof(uint32_t) * (tiles ? tiles : 1)));
-    RT_HIP(ctx, hipMalloc(&f.tile_cost, sizeof(uint32_t) * (tiles ? tiles : 1)));
+    const uint64_t cap = tiles ? tiles : 1, order_words = 2 * (uint64_t)rt::FrameBuffers::kOrderSlots * cap;
+    RT_HIP(ctx, hipMalloc(&f.order_mem, sizeof(uint32_t) * order_words));
+    for (int k = 0; k < rt::FrameBuffers::kOrderSlots; ++k) {
+        f.slots[k].order = f.order_mem + 2 * k * cap;
+        f.slots[k].cost = f.order_mem + (2 * k + 1) * cap;
+    }
     f.width = w;
     f.height = h;
     f.nlights = nlc;
-    f.bytes = (sizeof(float4) + sizeof(rt_hit24) + nlc) * (px ? px : 1) + 2 * sizeof(uint32_t) * (tiles ? tiles : 1);
+    f.bytes = (sizeof(float4) + sizeof(rt_hit24) + nlc) * (px ? px : 1) + sizeof(uint32_t) * order_words;
     ctx->bytes += f.bytes;
     return RT_OK;
 }
@@ -326,8 +329,7 @@ void rt_destroy(rt_ctx *ctx)
     hipFree(ctx->frame.rgba);
     hipFree(ctx->frame.hits);
     hipFree(ctx->frame.occ);
-    hipFree(ctx->frame.tile_order);
-    hipFree(ctx->frame.tile_cost);
+    hipFree(ctx->frame.order_mem);
     for (rt_ctx::Graph &g : ctx->graphs) {
         if (g.exec) hipGraphExecDestroy(g.exec);
         if (g.graph) hipGraphDestroy(g.graph);
@@ -933,6 +935,7 @@ static int check_tuning(rt_ctx *ctx, const rt_tuning &t)
     if (t.blocks_per_cu < 1 || t.blocks_per_cu > 32)
         return fail(ctx, RT_ERR_ARG, "tuning: blocks_per_cu in 1..32");
     if (t.traversal < 0 || t.traversal >= kForms) return fail(ctx, RT_ERR_ARG, "tuning: traversal 0, 1 or 2");
+    if (t.quad_tiles < 0 || t.quad_tiles > (1 << 24)) return fail(ctx, RT_ERR_ARG, "tuning: quad_tiles in 0..2^24");
     return RT_OK;
 }
 
@@ -956,6 +959,7 @@ int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t)
     rt_tuning s = kShapes[rt::tune_index(*t)];
     s.blocks_per_cu = t->blocks_per_cu;
     s.traversal = t->traversal;
+    s.quad_tiles = t->quad_tiles;
     ctx->tune = s;
     return RT_OK;
 }
@@ -1099,7 +1103,7 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "r")) {
             char k[256];
-            rt_tuning t;
+            rt_tuning t{};
             while (fscanf(f, "%255s %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal) == 5) {
                 if (key == k && check_tuning(ctx, t) == RT_OK) {
                     fclose(f);
@@ -1172,8 +1176,9 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "r")) {
             char k[256];
-            rt_tuning t;
-            while (fscanf(f, "%255s %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal) == 5) {
+            rt_tuning t{};
+            while (fscanf(f, "%255s %d %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal,
+                          &t.quad_tiles) == 6) {
                 if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK) {
                     fclose(f);
                     ctx->tune = t;
@@ -1207,12 +1212,36 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
             }
         }
     }
+    // quads for the heaviest tiles, with the fastest shape and form (the
+    // frames of the loop above leave a tile order for this view); kept only
+    // when at least 2 % faster, so that timing noise does not pick them
+    static const int32_t kQuadTiles[] = {64, 256, 1024};
+    const float one_lane_ms = bms;
+    if (rc == RT_OK && !uses_bvh(*b, i)) {
+        for (int32_t q : kQuadTiles) {
+            rt_tuning t = bt;
+            t.quad_tiles = q;
+            ctx->tune = t;
+            float ms = 1e30f;
+            for (int rep = 0; rep < 3 && rc == RT_OK; ++rep) {
+                rt_frame_stats st;
+                rc = rt_render_direct(ctx, scene, desc, 0, &st);
+                if (rep > 0 && st.trace_ms < ms) ms = st.trace_ms;
+            }
+            if (rc) break;
+            if (ms < bms && ms < 0.98f * one_lane_ms) {
+                bms = ms;
+                bt = t;
+            }
+        }
+    }
     ctx->tune = rc ? saved : bt;
     if (rc) return rc;
     if (best) *best = bt;
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "a")) {
-            fprintf(f, "%s %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal);
+            fprintf(f, "%s %d %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal,
+                    bt.quad_tiles);
             fclose(f);
         }
     }
@@ -1242,7 +1271,7 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     if (cache_path) {
         if (FILE *f = fopen(cache_path, "r")) {
             char k[256];
-            rt_tuning t;
+            rt_tuning t{};
             int pf = 0;
             while (fscanf(f, "%255s %d %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal,
                           &pf) == 6) {

@@ -210,14 +210,21 @@ struct FrameBuffers {
     float4 *rgba = nullptr;
     rt_hit24 *hits = nullptr;
     uint8_t *occ = nullptr;
-    // rt_render_direct's tile order (heaviest tiles of the last frame first)
-    // and the per-tile costs it is made from; sized for the smallest tile
-    // (16 x 4, block 64), valid for one (scene, block size, tile count).
-    uint32_t *tile_order = nullptr;
-    uint32_t *tile_cost = nullptr;
-    bool order_valid = false;
-    uint64_t order_key = 0;
-    uint32_t order_block = 0, order_tiles = 0;
+    // Tile orders of the frame kernels (the heaviest tiles of the last frame
+    // of a view first) and the per-tile costs they are made from, one slot
+    // per view: a slot is sized for the smallest tile (16 x 4, block 64) and
+    // holds one (scene, camera, kernel, block size, tile count); a new view
+    // takes the least recently used slot.
+    static constexpr int kOrderSlots = 8;
+    struct OrderSlot {
+        uint32_t *order = nullptr, *cost = nullptr;
+        bool used = false, valid = false;  // valid: order written by a frame of this view
+        uint64_t key = 0, tick = 0;
+        uint32_t block = 0, tiles = 0;
+    };
+    OrderSlot slots[kOrderSlots];
+    uint32_t *order_mem = nullptr;  // the slots' orders and costs
+    uint64_t order_tick = 0;
     uint64_t bytes = 0;
 };
 

@@ -21,7 +21,8 @@
 // launch-time tunables (occupancy against spill traffic).  rt_trace can start
 // a ray at any heap node, as LBVH_IntersectP takes its iRoot argument
 // (trav_enter).  Frame kernels take their tiles heaviest first, from the
-// last frame's per-tile costs (k_tile_order).
+// last frame's per-tile costs (k_tile_order), and direct frames may trace the
+// heaviest tiles with four lanes per pixel (quads, traverse_quad).
 #include <cstring>
 
 #include "rt_internal.h"
@@ -731,18 +732,170 @@ __device__ __forceinline__ void flat_normal(const TravTri &T, float &nx, float &
     nz = nz / nl;
 }
 
+// ---------------------------------------------------------------------------
+// Quads: four lanes per ray for the heaviest tiles of a frame.  The subtrees
+// rooted at heap depth D of the LBVH ("items"; a leaf above depth D is an
+// item too) are numbered in the ray's front-to-back visit order (the side
+// dirIsNeg[axis] first at each node), and lane s of the quad takes the items
+// numbered s, s + 4, s + 8, ...  Above depth D the four lanes walk the same
+// nodes: there a box is culled only against the ray's tmax and an entry
+// stacked there is never culled on pop, so the numbering is the same in all
+// four lanes whatever each lane's best t.  Inside its own items a lane culls
+// as the whole traversal does (a box it culls cannot hold a hit closer than
+// its own best, hence none closer than the quad's).  The quad's closest hit
+// is the smallest t over its lanes, on equal t the one of the lowest item
+// number: the hit the whole traversal keeps, since it takes a hit only with
+// t < best t.  A shadow ray is blocked when any lane finds a blocker.  A
+// pixel's chain of dependent iterations becomes the longest of its four
+// shares (the items along a ray's path alternate between the lanes); box
+// and triangle counts differ from a single traversal's, so quads render
+// frames and never count.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kQuadDepth = 8;  // heap depth of the subtrees a quad shares out
+
+template <bool ANYHIT, class Stack>
+__device__ __forceinline__ TraceOut traverse_quad(const TravNode *__restrict__ nodes,
+                                                  const TravTri *__restrict__ tris, int32_t root_ref,
+                                                  const float *root_box, uint32_t s, uint32_t D, float ox,
+                                                  float oy, float oz, float dx, float dy, float dz, float tmax,
+                                                  int32_t self, Stack &stk, uint32_t &rank)
+{
+    TraceOut r;
+    r.tri = -1;
+    r.u = -1.0f;
+    r.v = -1.0f;
+    r.t = tmax;
+    r.boxes = 0;
+    r.tris = 0;
+    r.leaf_heap = 1;
+    r.pos = -1;
+    rank = 0;
+    if (root_ref == -1) return r;
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const uint32_t neg = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
+    float t0, t1;
+    box_test(root_box, ox, oy, oz, ix, iy, iz, t0, t1);
+    r.boxes = 1;
+    if (t0 > r.t || t1 < 0.0f) return r;
+    const uint32_t kNeverCulled = 0xff800000u;  // -inf: no best t is below it
+    int32_t cur = root_ref;
+    uint32_t items = 0, item = 0;
+    stk.sp = 0;
+    while (true) {
+        float4 q0, q1, q2, q3f;
+        fetch_step(nodes, tris, cur, q0, q1, q2, q3f);
+        const bool at_node = cur >= 0;
+        const uint32_t depth = 31u - __clz((uint32_t)__float_as_int(at_node ? q3f.w : q2.w));
+        const bool top = depth < D;
+        bool mine = true;
+        if (depth == D || (top && !at_node)) {
+            item = items++;
+            mine = (item & 3u) == s;
+        }
+        if (mine) {
+            if (at_node) {
+                const int4 q3 = as_int4(q3f);
+                const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                float a0, a1, b0, b1;
+                box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
+                box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+                r.boxes += 2;
+                const int first_is_b = (neg >> q3.z) & 1u;
+                const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
+                const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+                const int32_t fref = first_is_b ? q3.y : q3.x;
+                const int32_t sref = first_is_b ? q3.x : q3.y;
+                const float lim = top ? tmax : r.t;
+                const bool f_ok = !(f0 > lim || f1 < 0.0f);
+                const bool s_ok = !(s1 < 0.0f) && !(s0 > lim);
+                if (f_ok) {
+                    if (s_ok) stk.push(make_uint2((uint32_t)sref, top ? kNeverCulled : __float_as_uint(s0)));
+                    cur = fref;
+                    continue;
+                }
+                if (s_ok) {
+                    cur = sref;
+                    continue;
+                }
+            } else {
+                const float before = r.t;
+                const bool stop = leaf_test_first(r, tris, (uint32_t)cur, q0, q1, q2, ox, oy, oz, dx, dy, dz,
+                                                  self, ANYHIT);
+                if (r.t < before) rank = item;
+                if (stop) return r;
+            }
+        }
+        bool found = false;
+        while (stk.sp > 0) {
+            const uint2 e = stk.pop();
+            if (!(__uint_as_float(e.y) > r.t)) {
+                cur = (int32_t)e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return r;
+}
+
+// The quad's closest hit in every lane of the quad: smallest t, on equal t
+// the lowest rank.  Lanes without a hit hold t = tmax, above every hit.
+__device__ __forceinline__ void quad_closest(TraceOut &h, uint32_t rank)
+{
+#pragma unroll
+    for (int o = 1; o <= 2; o <<= 1) {
+        const float t = __shfl_xor(h.t, o);
+        const int32_t tri = __shfl_xor(h.tri, o);
+        const int32_t pos = __shfl_xor(h.pos, o);
+        const float u = __shfl_xor(h.u, o);
+        const float v = __shfl_xor(h.v, o);
+        const int32_t heap = __shfl_xor(h.leaf_heap, o);
+        const uint32_t rk = __shfl_xor(rank, o);
+        if (t < h.t || (t == h.t && rk < rank)) {
+            h.t = t;
+            h.tri = tri;
+            h.pos = pos;
+            h.u = u;
+            h.v = v;
+            h.leaf_heap = heap;
+            rank = rk;
+        }
+    }
+}
+
+__device__ __forceinline__ bool quad_any(bool x)
+{
+    int b = x ? 1 : 0;
+    b |= __shfl_xor(b, 1);
+    b |= __shfl_xor(b, 2);
+    return b != 0;
+}
+
+__device__ __forceinline__ uint32_t quad_sum(uint32_t x)
+{
+    x += __shfl_xor(x, 1);
+    x += __shfl_xor(x, 2);
+    return x;
+}
+
 // Direct term of one primary hit (CSComputeColorRT.hlsl:125-198, white
 // material of alpha 1): shadow ray {o = hit, d = -Ldir, maxT = |hit - L|,
 // self = hit triangle} per light, cast when dot(n, Ldir) * dot(n, d) > 0,
 // traced any-hit from the root (the occlusion predicate of the reference's
 // closest-hit shadow ray).  Returns (ambient + sum max(0, -n'.Ldir)) * albedo.
-template <int FORM, class Stack>
+// With QUAD the four lanes of a quad hold the same hit and each traces its
+// share of every shadow ray (quad_lane: the lane's place in the quad;
+// quad_depth: the heap depth of the items shared out).
+template <int FORM, bool QUAD = false, class Stack>
 __device__ __forceinline__ float direct_term(const SceneView &sv, int32_t root_ref,
                                              const float *root_box, const TraceOut &h, float ox,
                                              float oy, float oz, float dx, float dy, float dz,
                                              const float *lights, uint32_t nlights, float ambient,
                                              float albedo, Stack &stk, uint32_t &boxes,
-                                             uint32_t &ntris, uint32_t &nshadow, uint8_t *occ_out)
+                                             uint32_t &ntris, uint32_t &nshadow, uint8_t *occ_out,
+                                             uint32_t quad_lane = 0, uint32_t quad_depth = 0)
 {
     float nx, ny, nz;
     flat_normal(sv.tris[h.pos], nx, ny, nz);
@@ -766,8 +919,16 @@ __device__ __forceinline__ float direct_term(const SceneView &sv, int32_t root_r
         const float rs = (nx * dx + ny * dy) + nz * dz;
         uint8_t o = 2;
         if ((ls * rs) > 0.0f) {
-            const TraceOut s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz,
-                                              -lx, -ly, -lz, dist, h.tri, stk);
+            TraceOut s;
+            if constexpr (QUAD) {
+                uint32_t rank;
+                s = traverse_quad<true>(sv.nodes, sv.tris, root_ref, root_box, quad_lane, quad_depth, hx, hy,
+                                        hz, -lx, -ly, -lz, dist, h.tri, stk, rank);
+                s.tri = quad_any(s.tri >= 0) ? 0 : -1;
+            } else {
+                s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz, -lx, -ly, -lz,
+                                              dist, h.tri, stk);
+            }
             boxes += s.boxes;
             ntris += s.tris;
             nshadow += 1;
@@ -789,6 +950,8 @@ struct RenderParams {
     float lights[24];           // up to 8 point lights
     const uint32_t *order;      // tile of each block, heaviest first; null: block index order
     uint32_t *cost;             // per tile: max over its lanes of box + triangle tests (atomicMax)
+    uint32_t nsplit;            // k_render_direct_quad: the first nsplit tiles of the order get quads
+    uint32_t quad_depth;        // the heap depth of the subtrees a quad shares out
 };
 
 // A block renders a tile of 16 x BLOCK/16 pixels.  With two or more waves
@@ -820,6 +983,45 @@ __device__ __forceinline__ void tile_cost_max(uint32_t *cost, uint32_t tile, uin
         c = y > c ? y : c;
     }
     if (__lane_id() == 0 && c) atomicMax(&cost[tile], c);
+}
+
+// k_render_direct's work for one tile, without kept hits or counts: the
+// tiles of k_render_direct_quad that have one lane per pixel.  (The kernel
+// keeps its own copy, so that its code stays as it was measured.)
+template <int BLOCK, int KLDS, int FORM>
+__device__ __forceinline__ void render_tile(const SceneView &sv, const RenderParams &rp, uint32_t tile,
+                                            float4 *__restrict__ rgba, unsigned long long *stats,
+                                            RayStack<BLOCK, KLDS> &stk)
+{
+    constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
+    const uint32_t tiles_x = (rp.width + TW - 1) / TW;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    uint32_t ox_, oy_;
+    Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
+    const uint32_t px = tx * TW + ox_;
+    const uint32_t py = ty * TH + oy_;
+    const bool valid = px < rp.width && py < rp.height;
+    int32_t root_ref;
+    float root_box[6];
+    load_root(sv.hdr, root_ref, root_box);
+    uint32_t boxes = 0, ntris = 0, nshadow = 0;
+    if (valid) {
+        const uint32_t pix = py * rp.width + px;
+        float dx, dy, dz;
+        primary_ray(rp.cam, px, py, (float)rp.width, (float)rp.height, 0.0f, 0.0f, dx, dy, dz);
+        const float ox = rp.cam.pos[0], oy = rp.cam.pos[1], oz = rp.cam.pos[2];
+        const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz,
+                                                      dx, dy, dz, 10000000000000000.0f, -1, stk);
+        boxes += h.boxes;
+        ntris += h.tris;
+        float color = rp.env;
+        if (h.tri >= 0)
+            color = direct_term<FORM>(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz, rp.lights, rp.nlights,
+                                      rp.ambient, rp.albedo, stk, boxes, ntris, nshadow, nullptr);
+        rgba[pix] = make_float4(color, color, color, 1.0f);
+    }
+    count_add(stats, 2, nshadow);
+    if (rp.cost) tile_cost_max(rp.cost, tile, boxes + ntris);
 }
 
 // Primary + shadow rays + direct term for one image, one tile per block.
@@ -874,6 +1076,71 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
     }
     count_add(stats, 2, nshadow);
     if (rp.cost) tile_cost_max(rp.cost, tile, boxes + ntris);
+}
+
+// Part `part` (0..3) of a tile with a quad per pixel: the tile's pixels
+// part * BLOCK/4 .. (part + 1) * BLOCK/4 - 1 in its lane order, pixel
+// threadIdx.x / 4 of the part on the lanes 4k .. 4k + 3.  The image equals
+// render_tile's; the tile's cost is the largest sum over a quad's lanes (at
+// least the single lane's count, so the tile keeps its place in the order).
+template <int BLOCK, int KLDS, int FORM>
+__device__ __forceinline__ void render_quad_part(const SceneView &sv, const RenderParams &rp, uint32_t tile,
+                                                 uint32_t part, float4 *__restrict__ rgba,
+                                                 unsigned long long *stats, RayStack<BLOCK, KLDS> &stk)
+{
+    constexpr int TW = Tile<BLOCK>::W, TH = Tile<BLOCK>::H;
+    const uint32_t tiles_x = (rp.width + TW - 1) / TW;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    const uint32_t s = threadIdx.x & 3u;
+    uint32_t ox_, oy_;
+    Tile<BLOCK>::offset(part * (BLOCK / 4) + (threadIdx.x >> 2), ox_, oy_);
+    const uint32_t px = tx * TW + ox_;
+    const uint32_t py = ty * TH + oy_;
+    const bool valid = px < rp.width && py < rp.height;  // the same for the four lanes of a quad
+    int32_t root_ref;
+    float root_box[6];
+    load_root(sv.hdr, root_ref, root_box);
+    uint32_t boxes = 0, ntris = 0, nshadow = 0;
+    if (valid) {
+        const uint32_t pix = py * rp.width + px;
+        float dx, dy, dz;
+        primary_ray(rp.cam, px, py, (float)rp.width, (float)rp.height, 0.0f, 0.0f, dx, dy, dz);
+        const float ox = rp.cam.pos[0], oy = rp.cam.pos[1], oz = rp.cam.pos[2];
+        uint32_t rank;
+        TraceOut h = traverse_quad<false>(sv.nodes, sv.tris, root_ref, root_box, s, rp.quad_depth, ox, oy, oz,
+                                          dx, dy, dz, 10000000000000000.0f, -1, stk, rank);
+        boxes += h.boxes;
+        ntris += h.tris;
+        quad_closest(h, rank);
+        float color;
+        if (h.tri < 0) {
+            color = rp.env;
+        } else {
+            color = direct_term<FORM, true>(sv, root_ref, root_box, h, ox, oy, oz, dx, dy, dz, rp.lights,
+                                            rp.nlights, rp.ambient, rp.albedo, stk, boxes, ntris, nshadow,
+                                            nullptr, s, rp.quad_depth);
+        }
+        if (s == 0) rgba[pix] = make_float4(color, color, color, 1.0f);
+    }
+    count_add(stats, 2, s == 0 ? nshadow : 0u);
+    if (rp.cost) tile_cost_max(rp.cost, tile, quad_sum(boxes + ntris));
+}
+
+// A frame whose rp.nsplit heaviest tiles (the head of rp.order) take four
+// blocks each, a quad per pixel, ahead of the other tiles, one block each.
+template <int BLOCK, int KLDS, int FORM>
+__global__ __launch_bounds__(BLOCK) void k_render_direct_quad(SceneView sv, RenderParams rp,
+                                                              float4 *__restrict__ rgba,
+                                                              unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    const uint32_t b = blockIdx.x, nq = 4u * rp.nsplit;
+    if (b < nq)
+        render_quad_part<BLOCK, KLDS, FORM>(sv, rp, rp.order[b >> 2], b & 3u, rgba, stats, stk);
+    else
+        render_tile<BLOCK, KLDS, FORM>(sv, rp, rp.order[b - nq + rp.nsplit], rgba, stats, stk);
 }
 
 // The next frame's tile order from this frame's tile costs, heaviest first:
@@ -1734,25 +2001,43 @@ hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, ui
     return hipGetLastError();
 }
 
-// The tile order kept with the frame buffers belongs to one (scene, block
-// size, tile count); any other launch starts from block order with cleared
-// costs.  Returns false if clearing failed.
-static bool order_ready(FrameBuffers &fb, uint64_t key, int block, uint32_t tiles, hipStream_t st)
+// The order slot of a view: the one holding (key, block, tiles), else the
+// least recently used one, taken over with its costs cleared and no order
+// yet (the view's first frame runs in block order).  Null if clearing failed.
+static FrameBuffers::OrderSlot *order_slot(FrameBuffers &fb, uint64_t key, int block, uint32_t tiles,
+                                           hipStream_t st)
 {
-    if (fb.order_valid && fb.order_key == key && fb.order_block == (uint32_t)block && fb.order_tiles == tiles)
-        return true;
-    fb.order_valid = false;
-    fb.order_key = key;
-    fb.order_block = (uint32_t)block;
-    fb.order_tiles = tiles;
-    return hipMemsetAsync(fb.tile_cost, 0, sizeof(uint32_t) * tiles, st) == hipSuccess;
+    FrameBuffers::OrderSlot *lru = &fb.slots[0];
+    for (FrameBuffers::OrderSlot &o : fb.slots) {
+        if (o.used && o.key == key && o.block == (uint32_t)block && o.tiles == tiles) {
+            o.tick = ++fb.order_tick;
+            return &o;
+        }
+        if (o.tick < lru->tick) lru = &o;
+    }
+    lru->used = true;
+    lru->valid = false;
+    lru->key = key;
+    lru->block = (uint32_t)block;
+    lru->tiles = tiles;
+    lru->tick = ++fb.order_tick;
+    return hipMemsetAsync(lru->cost, 0, sizeof(uint32_t) * tiles, st) == hipSuccess ? lru : nullptr;
+}
+
+// A view's key: the scene, the kernel (tag) and the camera's bytes (FNV-1a).
+static uint64_t view_key(const Batch &b, uint32_t scene, uint32_t tag, const rt_camera &cam)
+{
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(&cam);
+    for (size_t k = 0; k < sizeof(cam); ++k) h = (h ^ p[k]) * 1099511628211ull;
+    return ((((uint64_t)(uintptr_t)&b << 16) ^ scene) + tag) ^ h;
 }
 
 hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st)
 {
-    const uint64_t scene_key = (((uint64_t)(uintptr_t)&b << 16) ^ scene) + 1u;
+    const uint64_t key = view_key(b, scene, 1u, d.cam);
     const uint32_t max_depth = trav_depth(b, scene);
     RenderParams rp;
     rp.cam = d.cam;
@@ -1765,26 +2050,34 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     for (uint32_t i = 0; i < 24; ++i) rp.lights[i] = (i < 3 * d.nlights) ? d.lights[i] : 0.0f;
     const SceneView sv = view_of(b, scene);
     const bool keep = flags & RT_FLAG_KEEP_HITS, cnt = flags & RT_FLAG_COUNT;
+    // quads need heap indices in the nodes (the LBVH's), and frames that keep
+    // hits or count keep one lane per ray
+    const uint32_t quad_tiles = (keep || cnt || uses_bvh(b, scene)) ? 0u : (uint32_t)tune.quad_tiles;
+    rp.quad_depth = kQuadDepth;
     uint32_t ntiles = 0;
+    FrameBuffers::OrderSlot *os = nullptr;
 #define RT_L(B, K, F)                                                                               \
     {                                                                                              \
         constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                            \
         const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);             \
         const dim3 grid(tiles), block(B);                                                          \
         ntiles = tiles;                                                                            \
-        if (!order_ready(fb, scene_key, B, tiles, st)) return hipGetLastError();                   \
-        rp.order = fb.order_valid ? fb.tile_order : nullptr;                                       \
-        rp.cost = fb.tile_cost;                                                                    \
-        if (keep && cnt) hipLaunchKernelGGL((k_render_direct<true, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
+        os = order_slot(fb, key, B, tiles, st);                                                    \
+        if (!os) return hipGetLastError();                                                         \
+        rp.order = os->valid ? os->order : nullptr;                                                \
+        rp.cost = os->cost;                                                                        \
+        rp.nsplit = rp.order ? (quad_tiles < tiles ? quad_tiles : tiles) : 0u;                     \
+        if (rp.nsplit) hipLaunchKernelGGL((k_render_direct_quad<B, K, F>), dim3(tiles + 3u * rp.nsplit), block, 0, st, sv, rp, fb.rgba, stats); \
+        else if (keep && cnt) hipLaunchKernelGGL((k_render_direct<true, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
         else if (keep) hipLaunchKernelGGL((k_render_direct<true, false, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
         else if (cnt) hipLaunchKernelGGL((k_render_direct<false, true, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
         else hipLaunchKernelGGL((k_render_direct<false, false, B, K, F>), grid, block, 0, st, sv, rp, fb.rgba, fb.hits, fb.occ, stats); \
     }
     RT_TUNED(RT_L)
 #undef RT_L
-    hipLaunchKernelGGL(k_tile_order, dim3(tile_order_blocks(ntiles)), dim3(1024), 0, st, fb.tile_cost, ntiles,
-                       fb.tile_order);
-    fb.order_valid = true;
+    hipLaunchKernelGGL(k_tile_order, dim3(tile_order_blocks(ntiles)), dim3(1024), 0, st, os->cost, ntiles,
+                       os->order);
+    os->valid = true;
     return hipGetLastError();
 }
 
@@ -1830,16 +2123,18 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     RayRec *q[2] = {static_cast<RayRec *>(queue), static_cast<RayRec *>(queue) + px};
     if (!one_pass) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, qcount, d.bounces + 1);
     // the tile order of the one-pass kernel or of the wavefront's first pass
-    const uint64_t order_key = (((uint64_t)(uintptr_t)&b << 16) ^ scene) + (one_pass ? 2u : 3u);
+    const uint64_t order_key = view_key(b, scene, one_pass ? 2u : 3u, d.cam);
     uint32_t ntiles = 0;
+    FrameBuffers::OrderSlot *os = nullptr;
 #define RT_L(B, K, F)                                                                                      \
     {                                                                                                     \
         constexpr int TW = Tile<B>::W, TH = Tile<B>::H;                                                   \
         const uint32_t tiles = ((d.width + TW - 1) / TW) * ((d.height + TH - 1) / TH);                    \
         ntiles = tiles;                                                                                   \
-        if (!order_ready(fb, order_key, B, tiles, st)) return hipGetLastError();                          \
-        wp.order = fb.order_valid ? fb.tile_order : nullptr;                                              \
-        wp.cost = fb.tile_cost;                                                                           \
+        os = order_slot(fb, order_key, B, tiles, st);                                                     \
+        if (!os) return hipGetLastError();                                                                \
+        wp.order = os->valid ? os->order : nullptr;                                                       \
+        wp.cost = os->cost;                                                                               \
         if (one_pass) {                                                                                   \
             hipLaunchKernelGGL((k_render_whitted<B, K, F>), dim3(tiles), dim3(B), 0, st, sv, sh, wp,      \
                                fb.rgba, fb.hits, fb.occ, keep, cnt, stats);                               \
@@ -1861,9 +2156,9 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     }
     RT_TUNED(RT_L)
 #undef RT_L
-    hipLaunchKernelGGL(k_tile_order, dim3(tile_order_blocks(ntiles)), dim3(1024), 0, st, fb.tile_cost, ntiles,
-                       fb.tile_order);
-    fb.order_valid = true;
+    hipLaunchKernelGGL(k_tile_order, dim3(tile_order_blocks(ntiles)), dim3(1024), 0, st, os->cost, ntiles,
+                       os->order);
+    os->valid = true;
     return hipGetLastError();
 }
 

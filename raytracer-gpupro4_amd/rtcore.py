@@ -110,10 +110,10 @@ class BvhInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("lds_stack", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("traversal", C.c_int32)]
+                ("traversal", C.c_int32), ("quad_tiles", C.c_int32)]
 
     def as_tuple(self):
-        return (self.block, self.lds_stack, self.blocks_per_cu, self.traversal)
+        return (self.block, self.lds_stack, self.blocks_per_cu, self.traversal, self.quad_tiles)
 
 
 class JobDesc(C.Structure):
@@ -449,8 +449,8 @@ class Context:
         self.check(self.L.rt_tuning_get(self.h, C.byref(t)), "rt_tuning_get")
         return t.as_tuple()
 
-    def set_tuning(self, block, lds_stack, blocks_per_cu=4, traversal=0):
-        t = Tuning(block, lds_stack, blocks_per_cu, traversal)
+    def set_tuning(self, block, lds_stack, blocks_per_cu=4, traversal=0, quad_tiles=0):
+        t = Tuning(block, lds_stack, blocks_per_cu, traversal, quad_tiles)
         self.check(self.L.rt_tuning_set(self.h, C.byref(t)), "rt_tuning_set")
 
     def alloc(self, nbytes: int) -> int:

@@ -186,7 +186,8 @@ def test_autotuner_picks_and_caches(ctx, rtcore, tmp_path):
     best = ctx.autotune(jobs, cache)
     assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] in (2, 4, 8) and best[3] in (0, 1, 2)
     assert ctx.tuning() == best
-    assert open(cache).read().strip().endswith(" ".join(str(v) for v in best))
+    assert best[4] == 0  # quads are for direct frames only
+    assert open(cache).read().strip().endswith(" ".join(str(v) for v in best[:4]))
     ctx.set_tuning(128, 26, 4)
     assert ctx.autotune(jobs, cache) == best  # cache hit
     # the search rendered into scratch buffers, not into the caller's
@@ -225,8 +226,9 @@ def test_autotune_direct_picks_and_caches(ctx, rtcore, mix, tmp_path):
     try:
         best = ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache)
         assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] == saved[2] and best[3] in (0, 1, 2)
+        assert best[4] in (0, 64, 256, 1024)  # quads on the heaviest tiles, or none
         assert ctx.tuning() == best
-        assert open(cache).read().split()[-4:] == [str(v) for v in best]
+        assert open(cache).read().split()[-5:] == [str(v) for v in best]
         ctx.set_tuning(64, 8, saved[2])
         assert ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache) == best  # cache hit
     finally:

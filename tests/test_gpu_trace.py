@@ -218,6 +218,112 @@ def test_render_direct_tile_order_keeps_the_frame(ctx, rtcore, soup262k, block):
         ctx.set_tuning(*saved)
 
 
+def quad_frames(ctx, rtcore, s, cams, W, H, lights, block, form, quads):
+    """rgba and shadow-ray counts of each camera's frame with one lane per
+    pixel and with quads on the `quads` heaviest tiles; every frame is drawn
+    three times (the first in block order, the others in cost order, which
+    is when quads run)."""
+    lds = 26 if block == 128 else 12
+    out = []
+    for q in (0, quads):
+        ctx.set_tuning(block, lds, 4, form, q)
+        frames = []
+        for cam in cams:
+            for rep in range(3):
+                st = ctx.render_direct(s, cam, W, H, lights)
+                frames.append((ctx.frame_read(W, H, len(lights))[0], st["shadow_rays"], st["primary_rays"]))
+        out.append(frames)
+    return out
+
+
+@pytest.mark.parametrize("block,form", [(64, 2), (64, 0), (128, 1), (256, 2)])
+def test_render_direct_quads_keep_the_frame(ctx, rtcore, soup262k, torus, block, form):
+    """Quads (four lanes per pixel on the heaviest tiles, each lane searching
+    every fourth subtree of heap depth 8 along the ray) give the frame of one
+    lane per pixel bit for bit: on the soup and on the torus mesh (whose rays
+    meet shared edges, where the tie rule decides), in a ragged image, with
+    one, a few or all tiles in quads, and the same shadow-ray count."""
+    W, H = 401, 233
+    cams = [rtcore.camera_look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0, 1, 0), 60.0, W / H),
+            rtcore.camera_look_at((1.8, 0.9, -0.6), (0.5, 0.5, 0.5), (0, 1, 0), 50.0, W / H)]
+    tcams = [rtcore.camera_look_at((0.0, 0.5, 3.0), (0, 0, 0), (0, 1, 0), 60.0, W / H),
+             rtcore.camera_look_at((2.0, 1.5, 0.5), (0, 0, 0), (0, 1, 0), 45.0, W / H)]
+    lights = [(0.5, 2.0, -1.0), (1.5, 0.5, 0.5)]
+    saved = ctx.tuning()
+    try:
+        for scene, cs in ((soup262k[0], cams), (torus[0], tcams)):
+            for quads in (1, 7, 1 << 20):
+                one, quad = quad_frames(ctx, rtcore, scene, cs, W, H, lights, block, form, quads)
+                for k, ((a, sa, pa), (b, sb, pb)) in enumerate(zip(one, quad)):
+                    assert np.array_equal(a, b), (quads, k, int(np.any(a != b, axis=1).sum()))
+                    assert (sa, pa) == (sb, pb), (quads, k)
+    finally:
+        ctx.set_tuning(*saved)
+
+
+def test_render_direct_quads_small_and_shallow_trees(ctx, rtcore):
+    """Trees whose leaves sit above heap depth 8 (one triangle: the root is a
+    leaf; a handful: every leaf is an item of its own; a chain as deep as the
+    heap allows, with a leaf at every level) and an SAH BVH scene (which
+    keeps one lane per pixel): quads give the one-lane frame."""
+    W, H = 160, 90
+    lights = [(0.3, 3.0, -2.0)]
+    saved = ctx.tuning()
+    try:
+        scenes = []
+        for n in (1, 2, 5, 40):
+            xyz, idx = rtcore.gen_triangle_soup(n, 0.2, 7 + n)
+            scenes.append((ctx.add_scene(xyz, idx, 20), False))
+        cx, ci = chain_scene()
+        scenes.append((ctx.add_scene(cx, ci, 26), False))
+        xyz, idx = rtcore.gen_triangle_soup(3000, 0.01, 3)
+        scenes.append((ctx.add_scene(xyz, idx, 12), True))
+        cams = [rtcore.camera_look_at((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), (0, 1, 0), 70.0, W / H),
+                rtcore.camera_look_at((-0.01, -0.01, -0.01), (0.01, 0.01, 0.01), (0, 1, 0), 60.0, W / H)]
+        for s, sah in scenes:
+            if sah:
+                ctx.bvh_build(s)
+            else:
+                ctx.build(s)
+            for block, form in ((64, 2), (128, 0)):
+                one, quad = quad_frames(ctx, rtcore, s, cams, W, H, lights, block, form, 1 << 20)
+                for k, ((a, sa, pa), (b, sb, pb)) in enumerate(zip(one, quad)):
+                    assert np.array_equal(a, b), (s, block, form, k)
+                    assert (sa, pa) == (sb, pb)
+        for s, _ in scenes:
+            ctx.scene_destroy(s)
+    finally:
+        ctx.set_tuning(*saved)
+
+
+def test_render_direct_views_keep_their_orders(ctx, rtcore, soup262k):
+    """Each view (scene, camera, shape) keeps its own tile order; ten cameras
+    taken in turn, more than the eight order slots, with quads on their
+    heaviest tiles: every frame equals the camera's first frame (block
+    order, one lane per pixel)."""
+    import math
+
+    s = soup262k[0]
+    W, H = 200, 120
+    lights = [(0.5, 2.0, -1.0)]
+    cams = [rtcore.camera_look_at((0.5 + 1.8 * math.cos(a), 0.7, 0.5 + 1.8 * math.sin(a)), (0.5, 0.5, 0.5),
+                                  (0, 1, 0), 55.0, W / H) for a in np.linspace(0.0, 6.0, 10)]
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(64, 20, 4, 2, 0)
+        ref = []
+        for c in cams:
+            ctx.render_direct(s, c, W, H, lights)
+            ref.append(ctx.frame_read(W, H, 1)[0])
+        ctx.set_tuning(64, 20, 4, 2, 64)
+        for rnd in range(3):
+            for k in (list(range(10)) if rnd != 1 else [0, 1, 0, 1, 2, 9, 2]):
+                ctx.render_direct(s, cams[k], W, H, lights)
+                assert np.array_equal(ctx.frame_read(W, H, 1)[0], ref[k]), (rnd, k)
+    finally:
+        ctx.set_tuning(*saved)
+
+
 def chain_scene():
     """Triangles whose Morton codes are 2^29, 2^28, ..., 1, 0: each split
     peels one off, so the tree is a chain as deep as the heap allows."""
@@ -358,7 +464,8 @@ def test_render_direct_full_size_parity(ctx, oracle, rtcore, config):
     2,097,152-triangle sphere's orbit cameras at 1920x1080 with 4 lights
     (depth 23).  Primary hits bit-exact, >= 99.99 % occlusion agreement,
     PSNR >= 50 dB, shadow-ray count equal; every frame is rendered twice (the
-    second in the heaviest-tiles-first order) and both are checked."""
+    second in the heaviest-tiles-first order) and both are checked, then
+    twice more with quads on the 1024 heaviest tiles (the same image)."""
     import math
 
     W, H = 1920, 1080
@@ -391,4 +498,14 @@ def test_render_direct_full_size_parity(ctx, oracle, rtcore, config):
               f"occlusion match {occ_match:.6f}, PSNR {p:.2f} dB", flush=True)
         assert occ_match >= 0.9999 and p >= 50.0
         assert st["shadow_rays"] == int((rocc[:, :nl] != 2).sum())
+    # the same frame with quads on its 1024 heaviest tiles: the image above
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(*saved[:4], 1024)
+        for rep in range(2):
+            st = ctx.render_direct(s, cam, W, H, lights)
+            assert np.array_equal(ctx.frame_read(W, H, nl)[0], rgba), rep
+            assert st["shadow_rays"] == int((rocc[:, :nl] != 2).sum())
+    finally:
+        ctx.set_tuning(*saved)
     ctx.scene_destroy(s)
