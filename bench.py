@@ -103,6 +103,13 @@ def traffic_from_profile():
 
 
 # ------------------------------------------------------------------ frame
+def direct_kernel(tuning):
+    """The kernel that renders direct frames under this tuning: with quads
+    (tuning[4] > 0) the heaviest tiles and the rest share one launch of
+    k_render_direct_quad."""
+    return "k_render_direct_quad" if tuning[4] > 0 else "k_render_direct"
+
+
 def frame(ctx, rtcore, s, cam, f, flags=0, stats=False):
     ctx.scene_transform(s, rtcore.rotation_about(AXIS, float(f), CENTER))
     ctx.build(s)
@@ -226,7 +233,7 @@ def workload_frame(args, rtcore):
                  tuning=list(tuned),
                  roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                           "traffic": traffic, "kernel": "k_render_direct",
+                           "traffic": traffic, "kernel": direct_kernel(tuned),
                            "bytes_per_launch": per_launch_bytes},
                  cpu_baseline=None)
     if not args.no_cpu:
@@ -283,7 +290,7 @@ def workload_sphere(args, rtcore):
                   tests_per_ray={"box": boxes / per_step_rays, "tri": tris / per_step_rays},
                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                            "kernel": "k_render_direct", "bytes_per_launch": alg / 8},
+                            "kernel": direct_kernel(tuned), "bytes_per_launch": alg / 8},
                   cpu_baseline=None)
 
 
