@@ -401,9 +401,9 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
 /* Times every launch shape and traversal form on rt_render_direct of this
  * scene and frame (renders into the context's frame buffer), then, for an
  * LBVH scene, quad_tiles 64, 256 and 1024 with the fastest of them, and keeps
- * the fastest; the blocks_per_cu field is left as it is.  cache_path as for
- * rt_autotune_jobs, keyed by triangle count, depth, image size and light
- * count. */
+ * the fastest (quads only when at least 3 % faster); the blocks_per_cu field
+ * is left as it is.  cache_path as for rt_autotune_jobs, keyed by triangle
+ * count, depth, image size and light count. */
 int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc,
                        const char *cache_path, rt_tuning *best);
 /* Captures rebuild (optional) + one persistent rt_render_jobs launch of these

@@ -1214,7 +1214,7 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     }
     // quads for the heaviest tiles, with the fastest shape and form (the
     // frames of the loop above leave a tile order for this view); kept only
-    // when at least 2 % faster, so that timing noise does not pick them
+    // when at least 3 % faster, so that timing noise does not pick them
     static const int32_t kQuadTiles[] = {64, 256, 1024};
     const float one_lane_ms = bms;
     if (rc == RT_OK && !uses_bvh(*b, i)) {
@@ -1229,7 +1229,7 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
                 if (rep > 0 && st.trace_ms < ms) ms = st.trace_ms;
             }
             if (rc) break;
-            if (ms < bms && ms < 0.98f * one_lane_ms) {
+            if (ms < bms && ms < 0.97f * one_lane_ms) {
                 bms = ms;
                 bt = t;
             }

@@ -775,7 +775,7 @@ __device__ __forceinline__ TraceOut traverse_quad(const TravNode *__restrict__ n
     const uint32_t neg = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
     float t0, t1;
     box_test(root_box, ox, oy, oz, ix, iy, iz, t0, t1);
-    r.boxes = 1;
+    r.boxes = s == 0 ? 1u : 0u;  // the walk above depth D is counted once per quad
     if (t0 > r.t || t1 < 0.0f) return r;
     const uint32_t kNeverCulled = 0xff800000u;  // -inf: no best t is below it
     int32_t cur = root_ref;
@@ -800,7 +800,7 @@ __device__ __forceinline__ TraceOut traverse_quad(const TravNode *__restrict__ n
                 float a0, a1, b0, b1;
                 box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
                 box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
-                r.boxes += 2;
+                r.boxes += (!top || s == 0) ? 2u : 0u;
                 const int first_is_b = (neg >> q3.z) & 1u;
                 const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
                 const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
@@ -1081,8 +1081,10 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
 // Part `part` (0..3) of a tile with a quad per pixel: the tile's pixels
 // part * BLOCK/4 .. (part + 1) * BLOCK/4 - 1 in its lane order, pixel
 // threadIdx.x / 4 of the part on the lanes 4k .. 4k + 3.  The image equals
-// render_tile's; the tile's cost is the largest sum over a quad's lanes (at
-// least the single lane's count, so the tile keeps its place in the order).
+// render_tile's.  The tile's cost is the largest sum over a quad's lanes,
+// with the walk above the items counted once: about the single lane's
+// count (at least it: each lane culls less), so the tile keeps its place in
+// the order as the scene or view moves.
 template <int BLOCK, int KLDS, int FORM>
 __device__ __forceinline__ void render_quad_part(const SceneView &sv, const RenderParams &rp, uint32_t tile,
                                                  uint32_t part, float4 *__restrict__ rgba,
