@@ -115,3 +115,44 @@ def test_python_constants_match_the_header(rtcore):
               "RT_FLAG_START_AT_IROOT", "RT_WHITTED_ONE_PASS", "RT_WHITTED_SPLIT", "RT_BVH_SAH",
               "RT_STRUCTURE_LBVH", "RT_MODE_GI", "RT_OK", "RT_ERR_BUDGET"):
         assert n in mirrored, n
+
+
+STRUCTS = {"BuildTimes": "rt_build_times", "Profile": "rt_profile", "Camera": "rt_camera",
+           "RenderDesc": "rt_render_desc", "WhittedDesc": "rt_whitted_desc", "Model": "rt_model",
+           "BvhInfo": "rt_bvh_info", "Tuning": "rt_tuning", "JobDesc": "rt_job_desc",
+           "ServerConfig": "rt_server_config", "Request": "rt_request", "RequestResult": "rt_request_result",
+           "ServerStats": "rt_server_stats", "FrameStats": "rt_frame_stats"}
+DTYPES = {"NODE_DTYPE": "rt_node32", "RAY_DTYPE": "rt_ray48", "HIT_DTYPE": "rt_hit24",
+          "MATERIAL_DTYPE": "rt_material"}
+
+
+def test_python_structures_match_the_header(rtcore, tmp_path):
+    """Every ctypes structure and record dtype of the Python facade has the
+    size and field offsets of its C struct in include/rtcore.h (compiled
+    with the host C compiler), so a field added to one side alone fails."""
+    want, lines = {}, []
+    for py, c in STRUCTS.items():
+        cls = getattr(rtcore, py)
+        want[f"{c} sizeof"] = ctypes.sizeof(cls)
+        lines.append(f'printf("{c} sizeof %zu\\n", sizeof({c}));')
+        for name, _ in cls._fields_:
+            want[f"{c} {name}"] = getattr(cls, name).offset
+            lines.append(f'printf("{c} {name} %zu\\n", offsetof({c}, {name}));')
+    for py, c in DTYPES.items():
+        dt = getattr(rtcore, py)
+        want[f"{c} sizeof"] = dt.itemsize
+        lines.append(f'printf("{c} sizeof %zu\\n", sizeof({c}));')
+        for name in dt.names:
+            want[f"{c} {name}"] = dt.fields[name][1]
+            lines.append(f'printf("{c} {name} %zu\\n", offsetof({c}, {name}));')
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include "rtcore.h"\nint main(void) {\n' + "\n".join(lines) +
+                   "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True, text=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        key, v = line.rsplit(" ", 1)
+        got[key] = int(v)
+    assert got == want
