@@ -60,6 +60,7 @@ AXIS, CENTER = (1.0, 2.0, 3.0), (0.5, 0.5, 0.5)
 # triangle test)
 RAY_BYTES, BOX_BYTES, TRI_BYTES = 48 + 24, 32, 52
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "render_direct_soup262k_1080p_pmc.md")
+SPHERE_PMC_SUMMARY = os.path.join(ROOT, "profiles", "render_direct_quad_sphere2m_1080p_pmc.md")
 
 
 def log(msg):
@@ -89,13 +90,15 @@ def result(metric, value, unit, steps, warmup, wall, config, dtype="f32", **extr
     return out
 
 
-def traffic_from_profile():
-    """HBM bytes per k_render_direct launch measured by rocprofv3 --pmc
+def traffic_from_profile(path=None, kernel=None):
+    """HBM bytes per launch of the frame kernel measured by rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by their copy-kernel calibration),
-    read from the committed summary; None when it is absent."""
+    read from a committed summary (the line for `kernel` where the summary
+    covers several); None when it is absent."""
+    key = "traffic_bytes_per_launch" + (f" {kernel}" if kernel else "") + ":"
     try:
-        for line in open(PMC_SUMMARY):
-            if line.startswith("traffic_bytes_per_launch:"):
+        for line in open(path or PMC_SUMMARY):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1].split()[0])
     except OSError:
         pass
@@ -289,7 +292,8 @@ def workload_sphere(args, rtcore):
                   rays_per_frame=per_step_rays / 8,
                   tests_per_ray={"box": boxes / per_step_rays, "tri": tris / per_step_rays},
                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": traffic_from_profile(SPHERE_PMC_SUMMARY, direct_kernel(tuned)),
                             "kernel": direct_kernel(tuned), "bytes_per_launch": alg / 8},
                   cpu_baseline=None)
 
