@@ -296,6 +296,21 @@ def test_render_direct_quads_small_and_shallow_trees(ctx, rtcore):
         ctx.set_tuning(*saved)
 
 
+def test_tuning_quad_tiles_is_checked(ctx, rtcore):
+    """rt_tuning_set takes quad_tiles in 0..2^24 and keeps it; other values
+    are rejected and leave the tuning as it was."""
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(64, 20, 4, 2, 256)
+        assert ctx.tuning() == (64, 20, 4, 2, 256)
+        for bad in (-1, (1 << 24) + 1):
+            with pytest.raises(rtcore.RtError):
+                ctx.set_tuning(64, 20, 4, 2, bad)
+            assert ctx.tuning() == (64, 20, 4, 2, 256)
+    finally:
+        ctx.set_tuning(*saved)
+
+
 def test_render_direct_views_keep_their_orders(ctx, rtcore, soup262k):
     """Each view (scene, camera, shape) keeps its own tile order; ten cameras
     taken in turn, more than the eight order slots, with quads on their
