@@ -20,37 +20,9 @@
 #include <string.h>
 
 #include "lbvh_oracle.h"
+#include "pcg32.h"
 
-typedef struct {
-    uint64_t state, inc;
-} pcg32;
-
-static uint32_t pcg_next(pcg32 *r)
-{
-    uint64_t old = r->state;
-    r->state = old * 6364136223846793005ULL + r->inc;
-    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
-    uint32_t rot = (uint32_t)(old >> 59u);
-    return (xs >> rot) | (xs << ((0u - rot) & 31u));
-}
-
-static pcg32 pcg_seed(uint64_t seed, uint64_t stream)
-{
-    pcg32 r;
-    r.state = 0u;
-    r.inc = (stream << 1u) | 1u;
-    pcg_next(&r);
-    r.state += seed;
-    pcg_next(&r);
-    return r;
-}
-
-static double pcg_unit(pcg32 *r) { return (double)(pcg_next(r) >> 8) * (1.0 / 16777216.0); }
-
-static uint64_t sample_seed(uint64_t seed, uint32_t frame, uint32_t sample)
-{
-    return seed + 0x9E3779B97F4A7C15ull * ((uint64_t)frame * 65536ull + sample + 1ull);
-}
+static double pcg_unit(orc_pcg32 *r) { return (double)(orc_pcg_next(r) >> 8) * (1.0 / 16777216.0); }
 
 static int fbh(uint32_t x) { return 31 - __builtin_clz(x); }
 static int fbl(uint32_t x) { return __builtin_ctz(x); }
@@ -155,7 +127,7 @@ static void *gi_worker(void *arg)
             const uint32_t pix = py * J->w + px;
             double sum = 0.0;
             for (uint32_t s = J->first_sample; s < J->first_sample + J->spp; ++s) {
-                pcg32 rng = pcg_seed(sample_seed(J->seed, J->frame, s), pix);
+                orc_pcg32 rng = orc_pcg_seed(orc_sample_seed(J->seed, J->frame, s), pix);
                 double jx = 2.0 * pcg_unit(&rng) - 1.0, jy = 2.0 * pcg_unit(&rng) - 1.0;
                 double x = ((2.0 * px + 1.0 + jx) - fw) / fw;
                 double y = -(((2.0 * py + 1.0 + jy) - fh) / fh);

@@ -4,6 +4,7 @@
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no -ffast-math).
  */
 #include "lbvh_oracle.h"
+#include "pcg32.h"
 
 #include <math.h>
 #include <pthread.h>
@@ -527,15 +528,27 @@ void orc_trace_brute(const float *xyz, const uint32_t *idx, uint32_t nt, const o
 /* Pinhole ray generation: the pixel-centre rule of CSPrimaryRays.hlsl:37-40 */
 /* with an explicit camera basis; maxT and the self id as at :49-53.         */
 /* ------------------------------------------------------------------------ */
-void orc_raygen_pinhole(const float cam[14], uint32_t w, uint32_t h, orc_ray *rays)
+static void ray_defaults(orc_ray *r)
+{
+    for (int k = 0; k < 3; ++k) r->refl[k] = 1.0f;
+    r->tmin = 0.0f;
+    r->tmax = 10000000000000000.0f;
+    r->tri = -1;
+}
+
+void orc_raygen_pinhole_jitter(const float cam[14], uint32_t w, uint32_t h, const float *jxy,
+                               orc_ray *rays)
 {
     const float *pos = cam, *fwd = cam + 3, *right = cam + 6, *up = cam + 9;
     const float tx = cam[12], ty = cam[13];
     const float fw = (float)w, fh = (float)h;
     for (uint32_t py = 0; py < h; ++py) {
         for (uint32_t px = 0; px < w; ++px) {
-            float x = ((float)(2u * px + 1u) - fw) / fw;
-            float y = -(((float)(2u * py + 1u) - fh) / fh);
+            const size_t pix = (size_t)py * w + px;
+            const float jx = jxy ? jxy[2 * pix + 0] : 0.0f;
+            const float jy = jxy ? jxy[2 * pix + 1] : 0.0f;
+            float x = (((float)(2u * px + 1u) + jx) - fw) / fw;
+            float y = -((((float)(2u * py + 1u) + jy) - fh) / fh);
             float sx = x * tx, sy = y * ty;
             float d[3];
             for (int k = 0; k < 3; ++k) {
@@ -543,15 +556,74 @@ void orc_raygen_pinhole(const float cam[14], uint32_t w, uint32_t h, orc_ray *ra
                 d[k] = a + sy * up[k];
             }
             float len = sqrtf(dot3(d, d));
-            orc_ray *r = &rays[(size_t)py * w + px];
+            orc_ray *r = &rays[pix];
             for (int k = 0; k < 3; ++k) {
                 r->o[k] = pos[k];
                 r->d[k] = d[k] / len;
-                r->refl[k] = 1.0f;
             }
-            r->tmin = 0.0f;
-            r->tmax = 10000000000000000.0f;
-            r->tri = -1;
+            ray_defaults(r);
+        }
+    }
+}
+
+void orc_raygen_pinhole(const float cam[14], uint32_t w, uint32_t h, orc_ray *rays)
+{
+    orc_raygen_pinhole_jitter(cam, w, h, NULL, rays);
+}
+
+void orc_pcg32_draws(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t *out)
+{
+    orc_pcg32 rng = orc_pcg_seed(initstate, initseq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = orc_pcg_next(&rng);
+}
+
+void orc_pcg_jitter(uint64_t seed, uint32_t frame, uint32_t sample, uint32_t w, uint32_t h, float *jxy)
+{
+    for (uint32_t pix = 0; pix < w * h; ++pix) {
+        orc_pcg32 rng = orc_pcg_seed(orc_sample_seed(seed, frame, sample), pix);
+        const float u0 = (float)(orc_pcg_next(&rng) >> 8) * (1.0f / 16777216.0f);
+        jxy[2 * (size_t)pix + 0] = 2.0f * u0 - 1.0f;
+        const float u1 = (float)(orc_pcg_next(&rng) >> 8) * (1.0f / 16777216.0f);
+        jxy[2 * (size_t)pix + 1] = 2.0f * u1 - 1.0f;
+    }
+}
+
+/* Row r of the reference's Matrix4 (row-major, translation in column 3)
+ * applied to (v, 1): the HLSL mul(float4(v, 1), g_mfWorld) of a column-major
+ * cbuffer matrix, summed left to right. */
+static inline float mat_row(const float *m, int r, const float v[3])
+{
+    return ((m[4 * r + 0] * v[0] + m[4 * r + 1] * v[1]) + m[4 * r + 2] * v[2]) + m[4 * r + 3] * 1.0f;
+}
+
+void orc_raygen_world(const float world[16], uint32_t w, uint32_t h, const float *jxy, orc_ray *rays)
+{
+    const float fw = (float)w, fh = (float)h;
+    const float inverse = 1.0f / fh;
+    const float z = 2.0f;
+    const float scale = 0.008f / fabsf(0.035f * z);
+    const float zero[3] = {0.0f, 0.0f, 0.0f};
+    const float ow = mat_row(world, 3, zero);
+    float org[3];
+    for (int k = 0; k < 3; ++k) org[k] = mat_row(world, k, zero) / ow;
+    for (uint32_t py = 0; py < h; ++py) {
+        for (uint32_t px = 0; px < w; ++px) {
+            const size_t pix = (size_t)py * w + px;
+            const float jx = jxy ? jxy[2 * pix + 0] : 0.0f;
+            const float jy = jxy ? jxy[2 * pix + 1] : 0.0f;
+            const float y = -((((2.0f * (float)py + jy) + 1.0f) - fh) * inverse);
+            const float x = (((2.0f * (float)px + jx) + 1.0f) - fw) * inverse;
+            const float p[3] = {scale * x, scale * y, scale * z};
+            const float pw = mat_row(world, 3, p);
+            float d[3];
+            for (int k = 0; k < 3; ++k) d[k] = mat_row(world, k, p) / pw - org[k];
+            const float len = sqrtf(dot3(d, d));
+            orc_ray *r = &rays[pix];
+            for (int k = 0; k < 3; ++k) {
+                r->o[k] = org[k];
+                r->d[k] = d[k] / len;
+            }
+            ray_defaults(r);
         }
     }
 }

@@ -16,6 +16,9 @@
  *                                          + Shaders/Core/Intersection.hlsl:18-78
  *   - orc_trace_bvh                      : BVH.hlsl:124-217 (stack traversal of the SAH BVH)
  *   - orc_raygen_pinhole                 : pixel-centre rule of CSPrimaryRays.hlsl:37-40
+ *   - orc_raygen_pinhole_jitter,         : jittered primary rays (CSPrimaryRays.hlsl:25-36)
+ *     orc_pcg_jitter                       with PCG32 streams (pcg32.h)
+ *   - orc_raygen_world                   : the camera-matrix rule of CSPrimaryRays.hlsl:33-56
  *   - orc_render_direct                  : shadow-ray construction + direct term of
  *                                          CSComputeColorRT.hlsl:54-177
  *   - orc_render_whitted                 : the ray-tracing frame, RayTracerCS.cpp:286-295 +
@@ -111,6 +114,33 @@ void orc_trace_brute(const float *xyz, const uint32_t *idx, uint32_t nt, const o
  * Pixel (px, py): sx = ((2px+1) - W)/W * tan_x, sy = -((2py+1) - H)/H * tan_y,
  * d = normalize((fwd + sx*right) + sy*up). */
 void orc_raygen_pinhole(const float cam[14], uint32_t w, uint32_t h, orc_ray *rays);
+
+/* The same with a sub-pixel offset per pixel (the jittered primary rays of
+ * CSPrimaryRays.hlsl:25-36): x = (((2px+1) + jx) - W)/W,
+ * y = -((((2py+1) + jy) - H)/H), jxy = {jx, jy} per pixel (NULL: 0). */
+void orc_raygen_pinhole_jitter(const float cam[14], uint32_t w, uint32_t h, const float *jxy,
+                               orc_ray *rays);
+
+/* n outputs of the PCG32 stream pcg32_srandom_r(initstate, initseq) (pcg32.h). */
+void orc_pcg32_draws(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t *out);
+
+/* The render jobs' pixel jitter: per pixel p, the PCG32 stream
+ * (initstate = seed + 0x9E3779B97F4A7C15 * (frame * 65536 + sample + 1),
+ * initseq = p) draws u0, u1 = (next >> 8) / 2^24 and jx = 2 u0 - 1,
+ * jy = 2 u1 - 1 (the device's replacement of the random texture of
+ * CSPrimaryRays.hlsl:26-32). */
+void orc_pcg_jitter(uint64_t seed, uint32_t frame, uint32_t sample, uint32_t w, uint32_t h, float *jxy);
+
+/* The reference's camera-matrix rule (CSPrimaryRays.hlsl:39-56, GI jitter
+ * :33-37): world = cbCamera.g_mfWorld as the host's Matrix4 holds it
+ * (Camera::GetInverseMatrix, row-major, translation in column 3;
+ * RayTracerCS/ConstantBuffers.h:23-31), inverse = 1/H, x = (((2px + jx) + 1)
+ * - W) * inverse, y = -((((2py + jy) + 1) - H) * inverse), z = 2,
+ * scale = 0.008 / |0.035 z|, origin = (M (0,0,0,1)).xyz / w,
+ * d = normalize((M (scale x, scale y, scale z, 1)).xyz / w - origin).  Each
+ * matrix row is summed left to right.  For a square image (the reference's
+ * only case) inverse = 1/N; wider images keep the pixel pitch. */
+void orc_raygen_world(const float world[16], uint32_t w, uint32_t h, const float *jxy, orc_ray *rays);
 
 /* Direct lighting of one primary-hit buffer: per pixel, the flat normal,
  * and for each light a shadow ray {o = hit, d = -Ldir, maxT = |hit - L|,

@@ -54,6 +54,10 @@ def lib():
         L.orc_sort_pairs.argtypes = [P, P, U32]
         L.orc_trace.argtypes = [P, P, P, P, P, P, U32, I32, P, I32]
         L.orc_raygen_pinhole.argtypes = [P, U32, U32, P]
+        L.orc_raygen_pinhole_jitter.argtypes = [P, U32, U32, P, P]
+        L.orc_pcg_jitter.argtypes = [C.c_uint64, U32, U32, U32, U32, P]
+        L.orc_pcg32_draws.argtypes = [C.c_uint64, C.c_uint64, U32, P]
+        L.orc_raygen_world.argtypes = [P, U32, U32, P, P]
         L.orc_render_direct.argtypes = [P, P, P, P, P, P, U32, P, U32, P, P, P, I32]
         L.orc_render_gi.argtypes = [P, P, P, P, P, U32, U32, U32, U32, C.c_double, C.c_double,
                                     C.c_uint64, U32, U32, P, I32]
@@ -156,9 +160,36 @@ def trace_brute(xyz, idx, rays):
     return hits
 
 
-def raygen_pinhole(cam14: np.ndarray, w: int, h: int) -> np.ndarray:
+def raygen_pinhole(cam14: np.ndarray, w: int, h: int, jitter: np.ndarray | None = None) -> np.ndarray:
+    """Pinhole primary rays; jitter: float32 [w*h, 2] sub-pixel offsets or None."""
     rays = np.empty(w * h, RAY_DTYPE)
-    lib().orc_raygen_pinhole(_p(np.ascontiguousarray(cam14, np.float32)), w, h, _p(rays))
+    cam = _p(np.ascontiguousarray(cam14, np.float32))
+    if jitter is None:
+        lib().orc_raygen_pinhole(cam, w, h, _p(rays))
+    else:
+        lib().orc_raygen_pinhole_jitter(cam, w, h, _p(np.ascontiguousarray(jitter, np.float32)), _p(rays))
+    return rays
+
+
+def pcg32_draws(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.uint32)
+    lib().orc_pcg32_draws(initstate, initseq, n, _p(out))
+    return out
+
+
+def pcg_jitter(seed: int, frame: int, sample: int, w: int, h: int) -> np.ndarray:
+    """The render jobs' per-pixel jitter (jx, jy) of one sample: float32 [w*h, 2]."""
+    out = np.empty((w * h, 2), np.float32)
+    lib().orc_pcg_jitter(seed, frame, sample, w, h, _p(out))
+    return out
+
+
+def raygen_world(world16: np.ndarray, w: int, h: int, jitter: np.ndarray | None = None) -> np.ndarray:
+    """Primary rays of the reference's camera matrix (g_mfWorld, row-major Matrix4)."""
+    rays = np.empty(w * h, RAY_DTYPE)
+    lib().orc_raygen_world(_p(np.ascontiguousarray(world16, np.float32).reshape(16)), w, h,
+                           None if jitter is None else _p(np.ascontiguousarray(jitter, np.float32)),
+                           _p(rays))
     return rays
 
 
@@ -178,6 +209,31 @@ def render_direct(nodes, ids, xyz, idx, rays, hits, lights, ambient=0.1, albedo=
                             _p(np.ascontiguousarray(hits, HIT_DTYPE)), n, _p(lights), nl, _p(shade),
                             _p(rgba), _p(occ), nthreads)
     return rgba, occ
+
+
+def render_direct_job(nodes, ids, xyz, idx, w, h, spp, lights, ambient=0.1, albedo=0.8, sky=1.0, seed=0,
+                      frame=0, cam14=None, world16=None, nthreads=1):
+    """What one direct-mode render job adds to a zeroed accumulator: per
+    sample s, the jittered primary ray (pcg_jitter), the closest hit, the
+    direct term with the reference's shadow rays (render_direct; a miss adds
+    sky), summed in float32 in sample order.  Returns (accum [h, w, 4],
+    shadow rays cast, primary hits)."""
+    total = np.zeros(w * h, np.float32)
+    shadow = 0
+    nhit = 0
+    for s in range(spp):
+        jit = pcg_jitter(seed, frame, s, w, h)
+        rays = raygen_world(world16, w, h, jit) if world16 is not None else raygen_pinhole(cam14, w, h, jit)
+        hits = trace(nodes, ids, xyz, idx, rays, nthreads=nthreads)
+        rgba, occ = render_direct(nodes, ids, xyz, idx, rays, hits, lights, ambient, albedo, sky,
+                                  nthreads=nthreads)
+        total = (total + rgba[:, 0]).astype(np.float32)
+        shadow += int((occ[:, :max(len(np.asarray(lights).reshape(-1)) // 3, 0)] != 2).sum())
+        nhit += int((hits["tri"] >= 0).sum())
+    acc = np.empty((h, w, 4), np.float32)
+    acc[..., 0] = acc[..., 1] = acc[..., 2] = total.reshape(h, w)
+    acc[..., 3] = np.float32(spp)
+    return acc, shadow, nhit
 
 
 def render_gi(nodes, ids, xyz, idx, cam14, w, h, spp, max_bounces, albedo, sky, seed, frame=0,

@@ -50,37 +50,10 @@
 #include <string.h>
 
 #include "lbvh_oracle.h"
+#include "pcg32.h"
 
-typedef struct {
-    uint64_t state, inc;
-} wpcg;
-
-static uint32_t wpcg_next(wpcg *r)
-{
-    uint64_t old = r->state;
-    r->state = old * 6364136223846793005ULL + r->inc;
-    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
-    uint32_t rot = (uint32_t)(old >> 59u);
-    return (xs >> rot) | (xs << ((0u - rot) & 31u));
-}
-
-static wpcg wpcg_seed(uint64_t seed, uint64_t stream)
-{
-    wpcg r;
-    r.state = 0u;
-    r.inc = (stream << 1u) | 1u;
-    wpcg_next(&r);
-    r.state += seed;
-    wpcg_next(&r);
-    return r;
-}
-
-static float wpcg_unit(wpcg *r) { return (float)(wpcg_next(r) >> 8) * (1.0f / 16777216.0f); }
-
-static uint64_t wsample_seed(uint64_t seed, uint32_t frame, uint32_t sample)
-{
-    return seed + 0x9E3779B97F4A7C15ull * ((uint64_t)frame * 65536ull + sample + 1ull);
-}
+/* The float draw of the device's pcg_unit: the top 24 bits, exact. */
+static float pcg_unit_f(orc_pcg32 *r) { return (float)(orc_pcg_next(r) >> 8) * (1.0f / 16777216.0f); }
 
 static inline float d3(const float *a, const float *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 
@@ -120,7 +93,7 @@ static void *whitted_worker(void *arg)
     for (uint32_t pix = j->begin; pix < j->end; ++pix) {
         orc_ray ray = j->prim[pix];
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, res[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        wpcg rng = wpcg_seed(wsample_seed(p->seed, p->frame, 0u), pix);
+        orc_pcg32 rng = orc_pcg_seed(orc_sample_seed(p->seed, p->frame, 0u), pix);
         int32_t start = 1;
         if (j->occ0) j->occ0[pix] = 2;
         for (uint32_t pass = 0; pass <= p->bounces; ++pass) {
@@ -209,7 +182,7 @@ static void *whitted_worker(void *arg)
                 acc[c] = acc[c] + (dcol + scol) * fc;
                 res[c] = acc[c];
             }
-            const float u = wpcg_unit(&rng);
+            const float u = pcg_unit_f(&rng);
             if (M->alpha >= u) {
                 for (int c = 0; c < 3; ++c) {
                     ray.refl[c] = ray.refl[c] * M->specular[c];
