@@ -74,15 +74,32 @@ typedef struct rt_build_times {
     float total_ms;  /* first launch to last, on the device */
 } rt_build_times;
 
-/* Pinhole camera: unit basis and half-angle tangents (see rt_camera_look_at).
- * Pixel (px, py) of a W x H image: x = ((2px+1) - W)/W, y = -((2py+1) - H)/H
- * (CSPrimaryRays.hlsl:37-40), d = normalize((fwd + x*tan_x*right) + y*tan_y*up). */
+/* Primary-ray camera (CSPrimaryRays.hlsl:22-69), one of two kinds:
+ *  - RT_CAMERA_BASIS (0): a pinhole with a unit basis and half-angle tangents
+ *    (rt_camera_look_at).  Pixel (px, py) of a W x H image:
+ *    x = ((2px+1) - W)/W, y = -((2py+1) - H)/H,
+ *    d = normalize((fwd + x*tan_x*right) + y*tan_y*up), origin pos.
+ *  - RT_CAMERA_WORLD (1): the reference's own camera, the cbCamera matrix
+ *    g_mfWorld (Shaders/ConstantBuffers.hlsl:18-21) as the host's Matrix4
+ *    holds it (Camera::GetInverseMatrix, Common/Camera.h:64; row-major, the
+ *    translation in column 3; RayTracerCS/ConstantBuffers.h:23-31), with the
+ *    rays of CSPrimaryRays.hlsl:39-52: x = ((2px+1) - W)/H,
+ *    y = -((2py+1) - H)/H, z = 2, scale = 0.008/|0.035 z|,
+ *    origin = (M (0,0,0,1)).xyz / w,
+ *    d = normalize((M (scale x, scale y, scale z, 1)).xyz / w - origin).
+ *    For the reference's square images this is its rule exactly; wider
+ *    images keep its pixel pitch.  Set it with rt_camera_from_world; the
+ *    library derives pos from the matrix.
+ * Each matrix row is summed left to right; normalize is v / sqrt(v.v). */
+enum { RT_CAMERA_BASIS = 0, RT_CAMERA_WORLD = 1 };
 typedef struct rt_camera {
     float pos[3];
     float fwd[3];
     float right[3];
     float up[3];
     float tan_x, tan_y;
+    uint32_t kind;   /* RT_CAMERA_BASIS or RT_CAMERA_WORLD */
+    float world[16]; /* RT_CAMERA_WORLD: the Matrix4's 16 floats, xx xy xz xt yx ... tt */
 } rt_camera;
 
 /* Direct-lighting frame: primary rays, one shadow ray per light per hit
@@ -486,8 +503,13 @@ int rt_device_zero(rt_ctx *ctx, void *ptr, uint64_t bytes);
 int rt_device_read(rt_ctx *ctx, void *host, const void *ptr, uint64_t bytes);
 
 /* ---- camera helper (Camera::SetCamera role, Common/Camera.cpp:154-211) */
-/* Fills a pinhole camera at pos looking at target, vertical field of view
- * fov_y_deg, aspect = width / height (basis computed in double, rounded). */
+/* The reference's camera: kind RT_CAMERA_WORLD, world copied, pos set to
+ * the matrix's origin (M (0,0,0,1)).xyz / w, the basis fields zeroed.
+ * Replaces the upload of cbCamera (RayTracerCS.cpp:253, ConstantBuffers.h:28-31). */
+void rt_camera_from_world(rt_camera *cam, const float world[16]);
+/* Fills a pinhole camera (RT_CAMERA_BASIS) at pos looking at target, vertical
+ * field of view fov_y_deg, aspect = width / height (basis computed in double,
+ * rounded). */
 void rt_camera_look_at(rt_camera *cam, const float pos[3], const float target[3],
                        const float up[3], float fov_y_deg, float aspect);
 

@@ -829,6 +829,7 @@ int rt_render_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, uint32_
         return fail(ctx, RT_ERR_ARG, "rt_render_direct: 0..8 lights with a light array");
     if (desc->width == 0 || desc->height == 0 || desc->width > 16384 || desc->height > 16384)
         return fail(ctx, RT_ERR_ARG, "rt_render_direct: image %ux%u", desc->width, desc->height);
+    if (!rt::camera_ok(desc->cam)) return fail(ctx, RT_ERR_ARG, "rt_render_direct: camera kind %u", desc->cam.kind);
     hipSetDevice(ctx->device);
     int rc = ensure_frame(ctx, desc->width, desc->height, desc->nlights);
     if (rc) return rc;
@@ -865,6 +866,7 @@ int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint3
     if (!traversable(*b, i)) return fail(ctx, RT_ERR_STATE, "rt_render_whitted: scene %d not built", scene);
     if (desc->width == 0 || desc->height == 0 || desc->width > 16384 || desc->height > 16384)
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: image %ux%u", desc->width, desc->height);
+    if (!rt::camera_ok(desc->cam)) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: camera kind %u", desc->cam.kind);
     if (desc->bounces > 16) return fail(ctx, RT_ERR_ARG, "rt_render_whitted: %u bounces (0..16)", desc->bounces);
     if (desc->env_flag != -1 && desc->env_flag != 0)
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: env_flag %d (-1 or 0)", desc->env_flag);
@@ -983,8 +985,8 @@ static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, cons
             return fail(ctx, RT_ERR_ARG, "job %u: accum must be a 16-byte aligned device pointer", k);
         if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || d.spp == 0 ||
             d.spp > 65536 || d.max_bounces > 64 || d.nlights > 8 || (d.nlights && !d.lights) ||
-            (d.mode != RT_MODE_DIRECT && d.mode != RT_MODE_GI))
-            return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights or mode", k);
+            (d.mode != RT_MODE_DIRECT && d.mode != RT_MODE_GI) || !rt::camera_ok(d.cam))
+            return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights, mode or camera", k);
         uint32_t tiles = 0;
         rt::fill_render_job(host.data() + jb * k, *b, i, d, t, off, &tiles);
         if (trav_depth(*b, i) > *max_depth) *max_depth = trav_depth(*b, i);
@@ -1508,9 +1510,18 @@ int rt_profile_end(rt_ctx *ctx, rt_profile *out)
     return RT_OK;
 }
 
+void rt_camera_from_world(rt_camera *cam, const float world[16])
+{
+    memset(cam, 0, sizeof(*cam));
+    cam->kind = RT_CAMERA_WORLD;
+    memcpy(cam->world, world, sizeof(cam->world));
+    *cam = rt::device_camera(*cam);
+}
+
 void rt_camera_look_at(rt_camera *cam, const float pos[3], const float target[3], const float up[3],
                        float fov_y_deg, float aspect)
 {
+    memset(cam, 0, sizeof(*cam));
     double f[3], r[3], u[3];
     for (int k = 0; k < 3; ++k) f[k] = (double)target[k] - (double)pos[k];
     double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);

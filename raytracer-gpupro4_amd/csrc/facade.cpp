@@ -137,6 +137,12 @@ void RayTracerHip::SetCamera(const float pos[3], const float target[3], const fl
     wdesc_.cam = desc_.cam;
 }
 
+void RayTracerHip::SetCamera(const float world[16])
+{
+    rt_camera_from_world(&desc_.cam, world);
+    wdesc_.cam = desc_.cam;
+}
+
 void RayTracerHip::SetMaterials(const float *normals, const uint32_t *mat_ids, const rt_material *materials,
                                 uint32_t nmat)
 {

@@ -126,6 +126,9 @@ class RayTracerHip {
     RayTracerHip(Device &dev, LbvhGpu &lbvh, unsigned int width, unsigned int height,
                  bool dynamic);
     void SetCamera(const float pos[3], const float target[3], const float up[3], float fov_y_deg);
+    // the reference's own camera: the Matrix4 that cbCamera::Update uploads
+    // (Camera::GetInverseMatrix, RayTracerCS/ConstantBuffers.h:28-31)
+    void SetCamera(const float world[16]);
     void SetLights(const std::vector<float> &xyz);
     void SetShading(float ambient, float albedo, float env);
     // ray-tracing colour stage and its toggles

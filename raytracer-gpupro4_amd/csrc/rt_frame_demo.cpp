@@ -1,7 +1,9 @@
 // Drives the C++ facades end to end: torus scene (16,384 triangles), LBVH built
 // on the device through LbvhGpu / StructureConstructionHip, a few frames of
-// RayTracerHip::Render, and one batch of LbvhGpu::IntersectP.  Prints one JSON
+// RayTracerHip::Render, one batch of LbvhGpu::IntersectP, and a frame from the
+// reference's camera matrix (RayTracerHip::SetCamera(world)).  Prints one JSON
 // line with what it measured; exits non-zero on any error.
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -55,14 +57,51 @@ int main(int argc, char **argv)
         r.TriangleID = -1;
         rtgpu::TIntersection h = lbvh.IntersectP(r);
         const rtgpu::BBox wb = lbvh.WorldBound();
+        // the reference's own camera: a Matrix4 (rotation columns right, up,
+        // forward; translation in column 3) at (0.3, 0.8, -2.6) looking at the
+        // origin; one direct frame, written raw (argv[3], optional)
+        const float eye[3] = {0.3f, 0.8f, -2.6f};
+        float fw[3] = {-eye[0], -eye[1], -eye[2]}, rt3[3], up3[3];
+        float fl = std::sqrt(fw[0] * fw[0] + fw[1] * fw[1] + fw[2] * fw[2]);
+        for (float &c : fw) c /= fl;
+        rt3[0] = fw[2];  // (0, 1, 0) x fw
+        rt3[1] = 0.0f;
+        rt3[2] = -fw[0];
+        float rl = std::sqrt(rt3[0] * rt3[0] + rt3[2] * rt3[2]);
+        rt3[0] /= rl;
+        rt3[2] /= rl;
+        up3[0] = fw[1] * rt3[2] - fw[2] * rt3[1];
+        up3[1] = fw[2] * rt3[0] - fw[0] * rt3[2];
+        up3[2] = fw[0] * rt3[1] - fw[1] * rt3[0];
+        float world[16] = {};
+        for (int k = 0; k < 3; ++k) {
+            world[4 * k + 0] = rt3[k];
+            world[4 * k + 1] = up3[k];
+            world[4 * k + 2] = fw[k];
+            world[4 * k + 3] = eye[k];
+        }
+        world[15] = 1.0f;
+        rt.SetRayTracing(false);
+        rt.SetCamera(world);
+        rt.Render();
+        std::vector<float> wframe;
+        rt.ReadFrame(wframe);
+        if (argc > 3) {
+            FILE *f = fopen(argv[3], "wb");
+            if (!f || fwrite(wframe.data(), sizeof(float), wframe.size(), f) != wframe.size()) return 3;
+            fclose(f);
+        }
+        char wtxt[512];
+        int at = 0;
+        for (int k = 0; k < 16; ++k) at += snprintf(wtxt + at, sizeof(wtxt) - at, "%s%.9g", k ? ", " : "", world[k]);
         printf("{\"frames\": %u, \"samples\": %u, \"rays\": %llu, \"trace_ms\": %.4f, "
                "\"build_ms\": %.4f, \"image_mean\": %.6f, \"probe_tri\": %d, \"probe_t\": %.6f, "
                "\"root_box\": [%.4f, %.4f, %.4f, %.4f, %.4f, %.4f], \"keyboard_id\": %u, "
-               "\"name\": \"%s\", \"whitted_rays\": %llu}\n",
+               "\"name\": \"%s\", \"whitted_rays\": %llu, \"world\": [%s]}\n",
                frames, rt.Samples(), (unsigned long long)rays, trace_ms,
                rt.LastBuildTimes().total_ms, sum / (rgba.size() / 4), h.IDTr, h.t, wb.pMin[0],
                wb.pMin[1], wb.pMin[2], wb.pMax[0], wb.pMax[1], wb.pMax[2], lbvh.GetKeyboardId(),
-               lbvh.GetName(), (unsigned long long)whitted_rays);
+               lbvh.GetName(), (unsigned long long)whitted_rays, wtxt);
         return 0;
     } catch (const rtgpu::Error &e) {
         fprintf(stderr, "rt_frame_demo: %s (code %d)\n", e.what(), e.code());

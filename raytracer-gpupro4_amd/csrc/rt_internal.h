@@ -281,6 +281,11 @@ hipError_t launch_transform(Batch &b, uint32_t scene, const Xform &x, hipStream_
 hipError_t launch_trace(const Batch &b, uint32_t scene, const rt_ray48 *rays, uint32_t n,
                         rt_hit24 *hits, uint32_t flags, unsigned long long *stats,
                         const rt_tuning &tune, hipStream_t st);
+// The camera as the kernels read it: kind checked by camera_ok; for
+// RT_CAMERA_WORLD, pos becomes the matrix's origin (M (0,0,0,1)).xyz / w,
+// each row summed left to right (the kernels' primary_ray subtracts it).
+bool camera_ok(const rt_camera &c);
+rt_camera device_camera(const rt_camera &c);
 hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_desc &d,
                                 FrameBuffers &fb, uint32_t flags, unsigned long long *stats,
                                 const rt_tuning &tune, hipStream_t st);

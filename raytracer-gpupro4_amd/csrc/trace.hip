@@ -696,22 +696,48 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const rt_ray48 *_
     }
 }
 
-// Pinhole ray through (px + 0.5 + jx/2, py + 0.5 + jy/2): the pixel-centre
-// rule of CSPrimaryRays.hlsl:37-40 (and its GI jitter, :27-32) with an
-// explicit camera basis.  jx = jy = 0 gives the pixel centre.
+// Row r of the reference's camera Matrix4 applied to (v, 1), summed left to
+// right: HLSL mul(float4(v, 1), g_mfWorld) of the column-major cbuffer.
+__device__ __forceinline__ float world_row(const float *m, int r, float v0, float v1, float v2)
+{
+    return ((m[4 * r + 0] * v0 + m[4 * r + 1] * v1) + m[4 * r + 2] * v2) + m[4 * r + 3];
+}
+
+// The primary ray's direction (its origin is c.pos).  jx = jy = 0 gives the
+// pixel centre; render jobs pass a jitter in [-1, 1) (CSPrimaryRays.hlsl:25-36).
+//  - RT_CAMERA_BASIS: through (px + 0.5 + jx/2, py + 0.5 + jy/2) of the image
+//    plane at distance 1 along fwd, half extents tan_x, tan_y;
+//  - RT_CAMERA_WORLD: the reference's matrix rule (CSPrimaryRays.hlsl:39-52):
+//    pixel pitch 2/H, z = 2, scale = 0.008 / |0.035 z|, d = normalize(
+//    (M (scale x, scale y, scale z, 1)).xyz / w - origin); the host has put
+//    the origin (M (0, 0, 0, 1)).xyz / w into c.pos.
 __device__ __forceinline__ void primary_ray(const rt_camera &c, uint32_t px, uint32_t py, float fw,
                                             float fh, float jx, float jy, float &dx, float &dy,
                                             float &dz)
 {
-    const float x = (((float)(2u * px + 1u) + jx) - fw) / fw;
-    const float y = -((((float)(2u * py + 1u) + jy) - fh) / fh);
-    const float sx = x * c.tan_x, sy = y * c.tan_y;
-    const float ax = c.fwd[0] + sx * c.right[0];
-    const float ay = c.fwd[1] + sx * c.right[1];
-    const float az = c.fwd[2] + sx * c.right[2];
-    const float ddx = ax + sy * c.up[0];
-    const float ddy = ay + sy * c.up[1];
-    const float ddz = az + sy * c.up[2];
+    float ddx, ddy, ddz;
+    if (c.kind == RT_CAMERA_WORLD) {
+        const float inverse = 1.0f / fh;
+        const float y = -((((2.0f * (float)py + jy) + 1.0f) - fh) * inverse);
+        const float x = (((2.0f * (float)px + jx) + 1.0f) - fw) * inverse;
+        const float z = 2.0f;
+        const float scale = 0.008f / fabsf(0.035f * z);
+        const float p0 = scale * x, p1 = scale * y, p2 = scale * z;
+        const float w = world_row(c.world, 3, p0, p1, p2);
+        ddx = world_row(c.world, 0, p0, p1, p2) / w - c.pos[0];
+        ddy = world_row(c.world, 1, p0, p1, p2) / w - c.pos[1];
+        ddz = world_row(c.world, 2, p0, p1, p2) / w - c.pos[2];
+    } else {
+        const float x = (((float)(2u * px + 1u) + jx) - fw) / fw;
+        const float y = -((((float)(2u * py + 1u) + jy) - fh) / fh);
+        const float sx = x * c.tan_x, sy = y * c.tan_y;
+        const float ax = c.fwd[0] + sx * c.right[0];
+        const float ay = c.fwd[1] + sx * c.right[1];
+        const float az = c.fwd[2] + sx * c.right[2];
+        ddx = ax + sy * c.up[0];
+        ddy = ay + sy * c.up[1];
+        ddz = az + sy * c.up[2];
+    }
     const float len = sqrtf((ddx * ddx + ddy * ddy) + ddz * ddz);
     dx = ddx / len;
     dy = ddy / len;
@@ -2026,6 +2052,19 @@ static FrameBuffers::OrderSlot *order_slot(FrameBuffers &fb, uint64_t key, int b
     return hipMemsetAsync(lru->cost, 0, sizeof(uint32_t) * tiles, st) == hipSuccess ? lru : nullptr;
 }
 
+bool camera_ok(const rt_camera &c) { return c.kind == RT_CAMERA_BASIS || c.kind == RT_CAMERA_WORLD; }
+
+rt_camera device_camera(const rt_camera &c)
+{
+    rt_camera d = c;
+    if (c.kind == RT_CAMERA_WORLD) {
+        const float *m = c.world;
+        const float w = ((m[12] * 0.0f + m[13] * 0.0f) + m[14] * 0.0f) + m[15];
+        for (int k = 0; k < 3; ++k) d.pos[k] = (((m[4 * k] * 0.0f + m[4 * k + 1] * 0.0f) + m[4 * k + 2] * 0.0f) + m[4 * k + 3]) / w;
+    }
+    return d;
+}
+
 // A view's key: the scene, the kernel (tag) and the camera's bytes (FNV-1a).
 static uint64_t view_key(const Batch &b, uint32_t scene, uint32_t tag, const rt_camera &cam)
 {
@@ -2042,7 +2081,7 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     const uint64_t key = view_key(b, scene, 1u, d.cam);
     const uint32_t max_depth = trav_depth(b, scene);
     RenderParams rp;
-    rp.cam = d.cam;
+    rp.cam = device_camera(d.cam);
     rp.width = d.width;
     rp.height = d.height;
     rp.nlights = d.nlights;
@@ -2092,7 +2131,7 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     const SceneDesc &sd = b.desc[scene];
     WhittedParams wp;
     memset(&wp, 0, sizeof(wp));
-    wp.cam = d.cam;
+    wp.cam = device_camera(d.cam);
     wp.width = d.width;
     wp.height = d.height;
     wp.bounces = d.bounces;
@@ -2179,7 +2218,7 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
     j->tris = sv.tris;
     j->hdr = sv.hdr;
     j->accum = reinterpret_cast<float4 *>(d.accum);
-    j->cam = d.cam;
+    j->cam = device_camera(d.cam);
     j->width = d.width;
     j->height = d.height;
     j->tiles_x = (d.width + TW - 1) / TW;

@@ -33,6 +33,8 @@ FLAG_PERSISTENT = 1 << 4
 FLAG_START_AT_IROOT = 1 << 5
 MODE_DIRECT = 0
 MODE_GI = 1
+CAMERA_BASIS = 0
+CAMERA_WORLD = 1
 WHITTED_SHADOWS = 1 << 0
 WHITTED_PHONG = 1 << 1
 WHITTED_GLOSS = 1 << 2
@@ -76,11 +78,16 @@ class Profile(C.Structure):
 
 class Camera(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3),
-                ("up", C.c_float * 3), ("tan_x", C.c_float), ("tan_y", C.c_float)]
+                ("up", C.c_float * 3), ("tan_x", C.c_float), ("tan_y", C.c_float), ("kind", C.c_uint32),
+                ("world", C.c_float * 16)]
 
     def as_array(self) -> np.ndarray:
-        """The 14 floats {pos, fwd, right, up, tan_x, tan_y}."""
-        return np.frombuffer(bytes(self), dtype=np.float32).copy()
+        """The 14 floats {pos, fwd, right, up, tan_x, tan_y} of a basis camera."""
+        return np.frombuffer(bytes(self), dtype=np.float32, count=14).copy()
+
+    def world_matrix(self) -> np.ndarray:
+        """The 16 floats of a CAMERA_WORLD camera's matrix (row-major Matrix4)."""
+        return np.array(self.world[:], np.float32)
 
 
 class RenderDesc(C.Structure):
@@ -240,6 +247,7 @@ def lib() -> C.CDLL:
         "rt_load_model": (I32, [C.c_char_p, U32, P, C.POINTER(U32), P, C.POINTER(U32)]),
         "rt_load_last_error": (C.c_char_p, []),
         "rt_camera_look_at": (None, [C.POINTER(Camera), P, P, P, C.c_float, C.c_float]),
+        "rt_camera_from_world": (None, [C.POINTER(Camera), P]),
         "rt_gen_torus": (I32, [C.c_float, C.c_float, U32, U32, C.c_float, U64, P, C.POINTER(U32), P,
                                C.POINTER(U32)]),
         "rt_gen_triangle_soup": (I32, [U32, C.c_float, U64, P, C.POINTER(U32), P, C.POINTER(U32)]),
@@ -371,6 +379,14 @@ def camera_look_at(pos, target, up=(0.0, 1.0, 0.0), fov_y_deg=60.0, aspect=1.0) 
     t = np.asarray(target, np.float32)
     u = np.asarray(up, np.float32)
     lib().rt_camera_look_at(C.byref(cam), _ptr(p), _ptr(t), _ptr(u), fov_y_deg, aspect)
+    return cam
+
+
+def camera_from_world(world) -> Camera:
+    """The reference's camera: its Matrix4 (cbCamera g_mfWorld), 16 floats row-major."""
+    cam = Camera()
+    m = np.ascontiguousarray(np.asarray(world, np.float32).reshape(16))
+    lib().rt_camera_from_world(C.byref(cam), _ptr(m))
     return cam
 
 

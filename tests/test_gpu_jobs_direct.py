@@ -1,0 +1,195 @@
+"""Direct-mode render jobs (rt_render_jobs, RT_MODE_DIRECT) and the render
+server at the served stream's shape, against the oracle.
+
+A direct job adds, per sample, the jittered primary ray's direct term
+(CSPrimaryRays.hlsl:25-36 jitter, CSComputeColorRT.hlsl:126-177 shadow rays)
+into its accumulator.  The oracle composes the same from its own pieces
+(oracle.render_direct_job: PCG32 jitter, ray generation, closest-hit trace,
+reference-semantics shadow rays from the hit leaf).  Bars, as for direct
+frames: >= 99.99 % of pixels identical (the rest only where a shadow ray's
+any-hit from the root and the reference's closest hit from the leaf differ
+at an exact tie), PSNR >= 50 dB of the mean radiance, equal primary and
+shadow ray counts.  GI streams are compared with the FP64 path tracer on the
+same PCG streams: RMSE of the mean radiance < 0.02 (stated per stream) and
+>= 98 % of pixels within 1e-3, as in test_gpu_jobs.py."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def psnr(a, b):
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10 * np.log10(1.0 / mse)
+
+
+def check_direct(tag, got, want, spp_total):
+    assert np.all(got[..., 3] == spp_total), tag
+    same = float(np.mean(got[..., 0] == want[..., 0]))
+    p = psnr(got[..., 0] / spp_total, want[..., 0] / spp_total)
+    print(f"{tag}: identical pixels {same:.6f}, PSNR {p:.2f} dB", flush=True)
+    assert same >= 0.9999 and p >= 50.0, tag
+    assert np.array_equal(got[..., 0], got[..., 1]) and np.array_equal(got[..., 0], got[..., 2])
+
+
+@pytest.fixture(scope="module")
+def torus(ctx, oracle, rtcore):
+    xyz, idx = rtcore.gen_torus()
+    s = ctx.add_scene(xyz, idx, 20)
+    ctx.build(s)
+    ref = oracle.build_scene(xyz, idx, 20)
+    yield s, xyz, idx, ref
+    ctx.scene_destroy(s)
+
+
+@pytest.fixture(scope="module")
+def soup262k(ctx, oracle, rtcore):
+    xyz, idx = rtcore.gen_triangle_soup(262144, 0.004, 2)
+    s = ctx.add_scene(xyz, idx, 20)
+    ctx.build(s)
+    ref = oracle.build_scene(xyz, idx, 20)
+    yield s, xyz, idx, ref
+    ctx.scene_destroy(s)
+
+
+def run_direct_job(ctx, rtcore, scene, cam, W, H, spp, lights, seed, frame, persistent, ambient=0.1,
+                   albedo=0.8, sky=0.25):
+    acc = ctx.alloc(16 * W * H)
+    ctx.zero(acc, 16 * W * H)
+    st = ctx.render_jobs([ctx.job(scene, cam, W, H, acc, rtcore.MODE_DIRECT, spp, 0, albedo, sky, ambient,
+                                  lights=lights, seed=seed, frame=frame)], persistent=persistent)
+    img = ctx.read_accum(acc, W, H)
+    ctx.free(acc)
+    return img, st
+
+
+@pytest.mark.parametrize("case", ["torus512_spp2", "soup262k_1080p", "torus512_world"])
+def test_direct_job_matches_oracle(ctx, oracle, rtcore, torus, soup262k, case):
+    if case.startswith("torus"):
+        s, xyz, idx, ref = torus
+        W = H = 512
+        spp = 2
+        lights = [(0.0, 0.0, 0.0), (0.0, 2.0, 1.0)]
+        if case.endswith("world"):  # the reference's camera matrix (RT_CAMERA_WORLD)
+            R = np.array([[1, 0, 0], [0, math.cos(0.3), -math.sin(0.3)], [0, math.sin(0.3), math.cos(0.3)]])
+            world = np.eye(4, dtype=np.float32)
+            world[:3, :3] = R
+            world[:3, 3] = (0.0, 1.2, -2.5)
+            cam = rtcore.camera_from_world(world)
+            assert cam.kind == rtcore.CAMERA_WORLD and list(cam.pos) == [0.0, np.float32(1.2), -2.5]
+            kw = dict(world16=world)
+        else:
+            cam = rtcore.camera_look_at((0, 0.5, 3), (0, 0, 0), (0, 1, 0), 60.0, 1.0)
+            kw = dict(cam14=cam.as_array())
+    else:
+        s, xyz, idx, ref = soup262k
+        W, H, spp = 1920, 1080, 1
+        cam = rtcore.camera_look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0, 1, 0), 60.0, W / H)
+        lights = [(0.5, 2.5, -1.0)]
+        kw = dict(cam14=cam.as_array())
+    want, shadow, nhit = oracle.render_direct_job(ref["nodes"], ref["ids"], xyz, idx, W, H, spp, lights, 0.1, 0.8,
+                                                  0.25, seed=13, frame=4, nthreads=16, **kw)
+    assert nhit > 0.05 * W * H * spp
+    for persistent in (False, True):
+        got, st = run_direct_job(ctx, rtcore, s, cam, W, H, spp, lights, 13, 4, persistent)
+        check_direct(f"{case} persistent={persistent}", got, want, spp)
+        assert st["primary_rays"] == W * H * spp and st["shadow_rays"] == shadow and st["bounce_rays"] == 0
+
+
+def test_direct_job_jitter_is_not_the_pixel_centre(ctx, oracle, rtcore, torus):
+    """A planted wrong jitter (the pixel centre, or jx and jy swapped) in the
+    oracle gives a different image, so the comparison above sees the jitter."""
+    s, xyz, idx, ref = torus
+    W = H = 128
+    cam = rtcore.camera_look_at((0, 0.5, 3), (0, 0, 0), (0, 1, 0), 60.0, 1.0)
+    got, _ = run_direct_job(ctx, rtcore, s, cam, W, H, 1, [(0, 0, 0)], 3, 0, True)
+    for bad in ("centre", "swapped"):
+        jit = oracle.pcg_jitter(3, 0, 0, W, H)
+        jit = np.zeros_like(jit) if bad == "centre" else jit[:, ::-1].copy()
+        rays = oracle.raygen_pinhole(cam.as_array(), W, H, jit)
+        hits = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays, nthreads=8)
+        rgba, _ = oracle.render_direct(ref["nodes"], ref["ids"], xyz, idx, rays, hits, [(0, 0, 0)], 0.1, 0.8, 0.25,
+                                       nthreads=8)
+        assert np.mean(rgba[:, 0] == got[..., 0].reshape(-1)) < 0.99, bad
+
+
+def test_served_stream_shape_matches_oracle(ctx, oracle, rtcore):
+    """The served render stream's shape: scenes from a pool of log-uniform
+    sizes (1k-500k triangles), 512^2 and 1080p, direct and 2-bounce GI at
+    1 spp, three progressive passes per stream, requests arriving while
+    others are on the device, a 200 GB budget.  Each stream's accumulated
+    image equals the oracle's (direct) or the FP64 render's (GI), and the
+    same passes rendered alone (one whole-image job per launch)."""
+    pool = {k: rtcore.gen_object_mix(n, seed=100 + k) for k, n in ((1, 1500), (2, 60000), (3, 420000))}
+    depth = {1: 14, 2: 19, 3: 22}
+    refs = {k: oracle.build_scene(xyz, idx, depth[k]) for k, (xyz, idx) in pool.items()}
+    srv = rtcore.Server(ctx, 200 << 30, max_inflight=8, tiles_per_step=4096)
+    for k, (xyz, idx) in pool.items():
+        srv.add_scene(k, xyz, idx, depth[k])
+    # (scene, view, w, h, mode) per stream; every stream gets three passes
+    streams = [(1, 0, 512, 512, rtcore.MODE_DIRECT), (2, 1, 1920, 1080, rtcore.MODE_DIRECT),
+               (3, 2, 512, 512, rtcore.MODE_GI), (2, 3, 1920, 1080, rtcore.MODE_GI)]
+    cams, lights = {}, {}
+    for k, view, w, h, mode in streams:
+        xyz = pool[k][0]
+        lo, hi = xyz.min(0), xyz.max(0)
+        c = (lo + hi) / 2
+        ext = float(np.linalg.norm(hi - lo)) / 2
+        ang = 0.7 * view
+        eye = c + ext * np.array([math.sin(ang), 0.4, -math.cos(ang)], np.float32)
+        cams[view] = rtcore.camera_look_at(tuple(eye), tuple(c), (0, 1, 0), 55.0, w / h)
+        lights[view] = [tuple(c + ext * np.array([0.5, 2.0, -0.5], np.float32))]
+    arrivals = []
+    t = 0.0
+    for p in range(3):
+        for k, view, w, h, mode in streams:
+            srv.submit(k, view, cams[view], w, h, mode, 1, 2 if mode == rtcore.MODE_GI else 0, 0.7, 1.0, 0.1,
+                       lights=lights[view], seed=21, arrival=t)
+            arrivals.append(t)
+            t += 0.0004  # arrivals overlap the requests on the device
+    res, _ = srv.run(arrivals)
+    assert len(res) == 12 and all(r["status"] == rtcore.RT_OK for r in res)
+    st = srv.status()
+    assert st["builds"] == 3 and st["steps"] >= 3
+    for k, view, w, h, mode in streams:
+        got, n = srv.read_stream(k, view, w, h, mode)
+        assert n == 3
+        xyz, idx = pool[k]
+        ref = refs[k]
+        # the same passes alone, one whole-image job per launch
+        sc = ctx.add_scene(xyz, idx, depth[k])
+        ctx.build(sc)
+        acc = ctx.alloc(16 * w * h)
+        ctx.zero(acc, 16 * w * h)
+        for f in range(3):
+            ctx.render_jobs([ctx.job(sc, cams[view], w, h, acc, mode, 1, 2 if mode == rtcore.MODE_GI else 0, 0.7,
+                                     1.0, 0.1, lights=lights[view], seed=21, frame=f)], persistent=False,
+                            want_stats=False)
+        alone = ctx.read_accum(acc, w, h)
+        ctx.free(acc)
+        ctx.scene_destroy(sc)
+        assert np.array_equal(got, alone), (k, view)
+        tag = f"stream scene={k} ({len(idx)} tris) {w}x{h} {'GI' if mode else 'direct'}"
+        if mode == rtcore.MODE_DIRECT:
+            want = np.zeros((h, w, 4), np.float32)
+            for f in range(3):
+                a, _, _ = oracle.render_direct_job(ref["nodes"], ref["ids"], xyz, idx, w, h, 1, lights[view], 0.1,
+                                                   0.7, 1.0, seed=21, frame=f, cam14=cams[view].as_array(),
+                                                   nthreads=16)
+                want[..., :3] = (want[..., :3] + a[..., :3]).astype(np.float32)
+            want[..., 3] = 3
+            check_direct(tag, got, want, 3)
+        else:
+            r = np.zeros((h, w))
+            for f in range(3):
+                r += oracle.render_gi(ref["nodes"], ref["ids"], xyz, idx, cams[view].as_array(), w, h, 1, 2, 0.7,
+                                      1.0, 21, frame=f, nthreads=16)[0]
+            g = got[..., 0] / 3
+            r = r / 3
+            rmse = float(np.sqrt(np.mean((g - r) ** 2)))
+            close = float(np.mean(np.abs(g - r) <= 1e-3))
+            print(f"{tag}: GI vs FP64 same streams RMSE {rmse:.5f}, within 1e-3 {close:.4f}", flush=True)
+            assert np.all(got[..., 3] == 3) and rmse < 0.02 and close >= 0.98
+    srv.close()

@@ -146,14 +146,31 @@ def test_facade_demo_binary(rtcore, tmp_path):
 
     exe = os.path.join(os.path.dirname(rtcore.LIB_PATH), "rt_frame_demo")
     shot = tmp_path / "shot.png"
-    out = subprocess.run([exe, "3", str(shot)], capture_output=True, text=True, timeout=120)
+    raw = tmp_path / "world.f32"
+    out = subprocess.run([exe, "3", str(shot), str(raw)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     rec = json.loads(out.stdout.strip().splitlines()[-1])
-    assert rec["samples"] == 4 and rec["keyboard_id"] == 9 and rec["name"] == "LBVH"
+    assert rec["samples"] == 5 and rec["keyboard_id"] == 9 and rec["name"] == "LBVH"
     assert rec["probe_tri"] >= 0 and 2.0 < rec["probe_t"] < 3.0
     assert rec["rays"] >= 3 * 512 * 512
     assert rec["whitted_rays"] > 512 * 512
     assert shot.read_bytes()[:8] == b"\x89PNG\r\n\x1a\n"
+    # the frame of RayTracerHip::SetCamera(world): the reference's camera-matrix
+    # rays (CSPrimaryRays.hlsl:39-52) against the oracle's, light at the origin
+    import oracle
+
+    world = np.array(rec["world"], np.float32)
+    got = np.fromfile(raw, np.float32).reshape(-1, 4)
+    xyz, idx = rtcore.gen_torus(seed=7)
+    ref = oracle.build_scene(xyz, idx, 20)
+    rays = oracle.raygen_world(world, 512, 512)
+    hits = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays, nthreads=8)
+    want, _ = oracle.render_direct(ref["nodes"], ref["ids"], xyz, idx, rays, hits, [(0, 0, 0)], nthreads=8)
+    same = float(np.mean(np.all(got == want, axis=1)))
+    print(f"facade world camera: hits {(hits['tri'] >= 0).mean():.3f}, identical pixels {same:.6f}, "
+          f"PSNR {psnr(got[:, :3], want[:, :3]):.2f} dB", flush=True)
+    assert (hits["tri"] >= 0).mean() > 0.05
+    assert same >= 0.9999 and psnr(got[:, :3], want[:, :3]) >= 50.0
 
 
 @pytest.mark.parametrize("form", [1, 2])
