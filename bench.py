@@ -1,6 +1,10 @@
 """Benchmarks of the LBVH build + traversal core on one GPU.
 
-Default workload "frame" (one step = one frame): 262,144 triangles
+Default workload "all": the "frame" line below, with the "batched" and
+"serve" workloads' key figures added under "batched" and "serve" (the
+metric's batched-vs-plain and p50/p99 parts).
+
+Workload "frame" (one step = one frame): 262,144 triangles
 (centroids U[0,1]^3, edges N(0, 0.004^2)), rigidly rotated 1 degree per frame
 about the axis (1, 2, 3) through (0.5, 0.5, 0.5); the LBVH is rebuilt every
 frame (bounds, Morton, onesweep sort, emit, refit), then a 1920x1080 frame
@@ -59,6 +63,12 @@ AXIS, CENTER = (1.0, 2.0, 3.0), (0.5, 0.5, 0.5)
 # 32 B node box per box test, 4 B id + 12 B indices + 36 B vertices per
 # triangle test)
 RAY_BYTES, BOX_BYTES, TRI_BYTES = 48 + 24, 32, 52
+# launch shape of the headline frame (block, LDS stack entries, blocks per CU,
+# traversal form, quad tiles): held fixed so that the profile of the same
+# command times the same kernel.  rt_autotune_direct (on the first, static
+# frame) picks this shape or (64, 20, ., 2, 64), 1-2 % apart on the rotating
+# frames, so its choice flipped from run to run; --autotune searches again.
+FRAME_TUNING = (128, 20, 8, 2, 0)
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "render_direct_soup262k_1080p_pmc.md")
 SPHERE_PMC_SUMMARY = os.path.join(ROOT, "profiles", "render_direct_quad_sphere2m_1080p_pmc.md")
 
@@ -113,6 +123,14 @@ def direct_kernel(tuning):
     return "k_render_direct_quad" if tuning[4] > 0 else "k_render_direct"
 
 
+def kernel_name(tuning):
+    """The frame kernel's instance as rocprofv3 names it, e.g.
+    k_render_direct<false, false, 128, 20, 2> (block, LDS stack, form)."""
+    if tuning[4] > 0:
+        return f"k_render_direct_quad<{tuning[0]}, {tuning[1]}, {tuning[3]}>"
+    return f"k_render_direct<false, false, {tuning[0]}, {tuning[1]}, {tuning[3]}>"
+
+
 def frame(ctx, rtcore, s, cam, f, flags=0, stats=False):
     ctx.scene_transform(s, rtcore.rotation_about(AXIS, float(f), CENTER))
     ctx.build(s)
@@ -127,7 +145,8 @@ def cpu_baseline(rtcore, xyz, idx, cam, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    cores = min(CPU_SHARE, len(os.sched_getaffinity(0)))
+    affinity = len(os.sched_getaffinity(0))
+    cores = min(CPU_SHARE, affinity)
     f32 = np.float32
     rays0 = oracle.raygen_pinhole(cam.as_array(), W, H)
     build_s = trace_s = 0.0
@@ -158,21 +177,32 @@ def cpu_baseline(rtcore, xyz, idx, cam, budget_s=12.0):
     _, occ1 = oracle.render_direct(ref0["nodes"], ref0["ids"], rot0, idx, sub, hits1, LIGHTS, nthreads=1)
     one_s = time.perf_counter() - t0
     one_rays = len(sub) + int((occ1 != 2).sum())
-    model = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {"value": round(nrays / trace_s / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
             "sample": f"{frames} full frames of the same workload ({W}x{H} primary + shadow rays, "
                       f"closest hit, shadow rays from the hit leaf): {nrays} rays in {trace_s:.1f} s "
                       f"on {cores} threads; oracle rebuild of {NTRI} tris per frame on 1 thread",
             "value_1thread": round(one_rays / one_s / 1e6, 3),
             "sample_1thread": f"every 16th pixel of frame 0: {one_rays} rays in {one_s:.1f} s",
-            "build_ms_1thread": round(build_s * 1e3 / frames, 2), "cpu_model": model}
+            "build_ms_1thread": round(build_s * 1e3 / frames, 2), "cpu_model": cpu_model(),
+            "affinity_cores": affinity,
+            "threads_note": f"traversal threads = min({CPU_SHARE}, affinity mask): a one-GPU box grants a "
+                            f"process {CPU_SHARE} host threads whatever the mask shows"}
+
+
+def cpu_model():
+    """The host CPU as /proc/cpuinfo names it: model name, family/model/
+    stepping, vendor and the logical CPUs online."""
+    info = {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info.setdefault(k.strip(), v.strip())
+    except OSError:
+        return "unknown"
+    return (f"{info.get('model name', 'unknown')} ({info.get('vendor_id', '?')} family {info.get('cpu family', '?')} "
+            f"model {info.get('model', '?')} stepping {info.get('stepping', '?')}, {os.cpu_count()} logical CPUs, "
+            f"{info.get('cpu MHz', '?')} MHz)")
 
 
 def device_sync(ctx):
@@ -189,7 +219,11 @@ def workload_frame(args, rtcore):
     ctx = rtcore.Context(0)
     s = ctx.add_scene(xyz, idx, DEPTH)
     ctx.build(s)
-    tuned = ctx.autotune_direct(s, cam, W, H, LIGHTS)  # launch shape for this frame, before timing
+    if args.autotune:
+        tuned = ctx.autotune_direct(s, cam, W, H, LIGHTS)  # launch shape for this frame, before timing
+    else:
+        ctx.set_tuning(*FRAME_TUNING)
+        tuned = ctx.tuning()
     for f in range(args.warmup):
         frame(ctx, rtcore, s, cam, f)
     device_sync(ctx)
@@ -221,7 +255,7 @@ def workload_frame(args, rtcore):
     per_launch_ms = trace_ms / prof["launches"]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
     build = {k: v / prof["builds"] for k, v in prof["build_ms"].items()}
-    traffic = traffic_from_profile()
+    traffic = traffic_from_profile(PMC_SUMMARY, kernel_name(tuned))
     out = result(METRIC, mrays, "Mrays/s", args.steps, args.warmup, wall,
                  {"workload": f"{NTRI}-triangle soup rebuilt every frame (1 deg/frame rotation), "
                               f"{W}x{H} primary + shadow rays, 1 point light",
@@ -233,11 +267,14 @@ def workload_frame(args, rtcore):
                  shadow_rays_per_frame=prof["shadow_rays"] / args.steps,
                  frame_mrays_s_wall=round(rays / wall / 1e6, 2),
                  tests_per_ray={"box": boxes / rays, "tri": tris / rays},
-                 tuning=list(tuned),
+                 tuning=list(tuned), tuning_source="autotuned" if args.autotune else "fixed (FRAME_TUNING)",
                  roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                           "traffic": traffic, "kernel": direct_kernel(tuned),
-                           "bytes_per_launch": per_launch_bytes},
+                           "traffic": traffic, "kernel": kernel_name(tuned),
+                           "bytes_per_launch": per_launch_bytes,
+                           "note": "achieved = algorithmic bytes (per-ray box and triangle tests x their record "
+                                   "sizes) / launch time; most of them are served by L2 and the Infinity Cache, "
+                                   "traffic is what reaches HBM"},
                  cpu_baseline=None)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(rtcore, xyz, idx, cam)
@@ -778,13 +815,49 @@ def workload_serve(args, rtcore):
                   cpu_baseline=None)
 
 
+def workload_all(args, rtcore):
+    """The default line: the frame workload (value, roofline, CPU baseline),
+    with the metric's other two parts beside it: the batched GI workload
+    (64 scenes, batched against per-scene launches, images compared bit for
+    bit, GI RMSE against the FP64 oracle) and the served stream (Poisson
+    200/s for 60 s against one request per launch, p50/p99)."""
+    import types
+
+    out = workload_frame(args, rtcore)
+    t0 = time.perf_counter()
+    b = workload_batched(types.SimpleNamespace(steps=3, warmup=1, no_cpu=args.no_cpu), rtcore)
+    out["batched"] = {"workload": b["config"]["workload"], "steps": 3, "mrays_s": b["value"],
+                      "plain_mrays_s": b["plain"]["mrays_s"], "speedup_trace": b["speedup_trace"],
+                      "speedup_step": b["speedup_step"], "ms_per_step": b["batched"]["ms_per_step"],
+                      "plain_ms_per_step": b["plain"]["ms_per_step"],
+                      "graph_ms_per_step": b["batched"]["graph_ms_per_step"],
+                      "identical_images": b["identical_images"], "tuning": b["tuning"],
+                      "gi_rmse_vs_fp64": b["gi_rmse_vs_fp64"], "wall_s": round(time.perf_counter() - t0, 1)}
+    t0 = time.perf_counter()
+    sv = workload_serve(types.SimpleNamespace(duration=60.0, plain_seconds=10.0, burst=2000), rtcore)
+    out["serve"] = {"workload": sv["config"]["workload"], "requests": sv["config"]["requests"],
+                    "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"],
+                    "served_mrays_s_device": sv["served"]["mrays_s_device"],
+                    "plain_mrays_s_device": sv["plain"]["mrays_s_device"],
+                    "served_vs_plain_device": round(sv["served"]["mrays_s_device"] /
+                                                    sv["plain"]["mrays_s_device"], 3),
+                    "plain_p50_ms": sv["plain"]["p50_ms"], "plain_p99_ms": sv["plain"]["p99_ms"],
+                    "device_busy_frac": sv["served"]["device_busy_frac"],
+                    "burst_speedup_device": sv["burst"]["speedup_device"],
+                    "burst_speedup_clock": sv["burst"]["speedup_clock"], "clock": sv["clock"],
+                    "wall_s": round(time.perf_counter() - t0, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["frame", "sphere", "whitted", "bvh", "build", "batched", "serve"],
-                    default="frame")
+    ap.add_argument("--workload", choices=["all", "frame", "sphere", "whitted", "bvh", "build", "batched", "serve"],
+                    default="all")
+    ap.add_argument("--autotune", action="store_true",
+                    help="frame: pick the launch shape with rt_autotune_direct instead of FRAME_TUNING")
     ap.add_argument("--tris", type=int, default=0, help="build: one soup size instead of the list")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--plain-seconds", type=float, default=10.0,
@@ -800,7 +873,7 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py is single-process; WORLD_SIZE > 1 is not supported", file=sys.stderr)
         sys.exit(2)
-    defaults = {"frame": (100, 5), "sphere": (10, 1), "whitted": (20, 2), "bvh": (50, 3), "build": (20, 2),
+    defaults = {"all": (100, 5), "frame": (100, 5), "sphere": (10, 1), "whitted": (20, 2), "bvh": (50, 3), "build": (20, 2),
                 "batched": (3, 1), "serve": (0, 0)}
     ds, dw = defaults[args.workload]
     args.steps = ds if args.steps is None else args.steps
@@ -813,7 +886,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     rtcore.lib()
-    fn = {"frame": workload_frame, "sphere": workload_sphere, "whitted": workload_whitted, "bvh": workload_bvh,
+    fn = {"all": workload_all, "frame": workload_frame, "sphere": workload_sphere, "whitted": workload_whitted, "bvh": workload_bvh,
           "build": workload_build, "batched": workload_batched, "serve": workload_serve}[args.workload]
     print(json.dumps(fn(args, rtcore)), flush=True)
 

@@ -425,7 +425,13 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc,
                        const char *cache_path, rt_tuning *best);
 /* Captures rebuild (optional) + one persistent rt_render_jobs launch of these
  * jobs as a HIP graph; each rt_graph_launch replays it with frame + n for the
- * n-th replay.  Jobs must name scenes of the batch of `scene`. */
+ * n-th replay.  Jobs must name scenes of the batch of `scene`.  The graph
+ * holds that batch (its memory stays allocated, and counted by
+ * rt_memory_in_use, until rt_graph_destroy); the accumulators it names are
+ * the caller's and must outlive it.  rt_graph_launch returns RT_ERR_STATE,
+ * and runs nothing, once a scene handle of the batch has been destroyed, a
+ * BVH of it built or dropped (rt_bvh_build, an upload), or its structure
+ * switched (rt_scene_select) since the capture: capture again. */
 int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_desc *jobs,
                            uint32_t njobs, int *graph);
 int rt_graph_launch(rt_ctx *ctx, int graph);

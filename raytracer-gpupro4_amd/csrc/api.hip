@@ -335,6 +335,7 @@ void rt_destroy(rt_ctx *ctx)
         if (g.graph) hipGraphDestroy(g.graph);
         hipFree(g.jobs);
         hipFree(g.counter);
+        if (g.batch && --g.batch->refs == 0) batch_free(g.batch);
     }
     hipFree(ctx->stats);
     hipFree(ctx->staging);
@@ -371,15 +372,15 @@ int rt_memory_in_use(rt_ctx *ctx, uint64_t *bytes)
     if (!ctx || !bytes) return RT_ERR_ARG;
     uint64_t b = ctx->bytes;
     std::vector<Batch *> seen;
-    for (const rt::SceneRef &r : ctx->scenes) {
-        if (!r.batch) continue;
-        bool dup = false;
-        for (Batch *x : seen) dup |= (x == r.batch);
-        if (!dup) {
-            seen.push_back(r.batch);
-            b += r.batch->bytes;
-        }
-    }
+    auto count = [&](Batch *x) {
+        if (!x) return;
+        for (Batch *y : seen)
+            if (y == x) return;
+        seen.push_back(x);
+        b += x->bytes;
+    };
+    for (const rt::SceneRef &r : ctx->scenes) count(r.batch);
+    for (const rt_ctx::Graph &g : ctx->graphs) count(g.batch);  // batches only a graph still holds
     *bytes = b;
     return RT_OK;
 }
@@ -403,6 +404,7 @@ int rt_scene_destroy(rt_ctx *ctx, int scene)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     ctx->scenes[scene].batch = nullptr;
+    b->generation++;
     if (--b->refs == 0) batch_free(b);
     return RT_OK;
 }
@@ -419,6 +421,7 @@ static int check_upload(rt_ctx *ctx, Batch *b, uint32_t i, int scene, uint32_t n
 // vertices drop it and the scene goes back to the LBVH.
 static void drop_bvh(Batch *b, uint32_t i)
 {
+    if ((i < b->use_bvh.size() && b->use_bvh[i]) || (i < b->bvh.size() && b->bvh[i].valid)) b->generation++;
     if (i < b->use_bvh.size()) b->use_bvh[i] = 0;
     if (i < b->bvh.size()) b->bvh[i].valid = false;
 }
@@ -613,6 +616,7 @@ int rt_bvh_build(rt_ctx *ctx, int scene, uint32_t max_prims, uint32_t split, rt_
     Batch::Bvh &v = b->bvh[i];
     v.valid = false;
     b->use_bvh[i] = 0;
+    b->generation++;  // the arrays below are replaced
     rt::bvh_build_host(xyz.data(), idx.data(), d.nt, max_prims, split, v.host, info);
     std::vector<rt::TravNode> trav;
     std::vector<rt::TravTri> tris;
@@ -650,11 +654,15 @@ int rt_scene_select(rt_ctx *ctx, int scene, int structure)
     if (structure == RT_STRUCTURE_BVH) {
         if (i >= b->bvh.size() || !b->bvh[i].valid)
             return fail(ctx, RT_ERR_STATE, "rt_scene_select: scene %d has no BVH (rt_bvh_build)", scene);
+        if (!b->use_bvh[i]) b->generation++;
         b->use_bvh[i] = 1;
         return RT_OK;
     }
     if (structure == RT_STRUCTURE_LBVH) {
-        if (i < b->use_bvh.size()) b->use_bvh[i] = 0;
+        if (i < b->use_bvh.size() && b->use_bvh[i]) {
+            b->use_bvh[i] = 0;
+            b->generation++;
+        }
         return RT_OK;
     }
     return fail(ctx, RT_ERR_ARG, "rt_scene_select: structure %d (0 BVH, 9 LBVH)", structure);
@@ -1096,25 +1104,55 @@ static std::string tune_key(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs
     return buf;
 }
 
+// Autotune cache: a text file, one record per line,
+//   <key> block lds_stack blocks_per_cu traversal quad_tiles extra
+// (extra: the Whitted pass form, 0 for the other searches).  Lines of any
+// other shape are skipped; the last valid record of a key wins.
+static bool cache_find(rt_ctx *ctx, const char *path, const char *key, rt_tuning *t, int *extra)
+{
+    FILE *f = path ? fopen(path, "r") : nullptr;
+    if (!f) return false;
+    bool found = false;
+    char line[512];
+    while (fgets(line, sizeof(line), f)) {
+        char k[256];
+        rt_tuning x{};
+        int e = 0, n = 0;
+        if (sscanf(line, "%255s %d %d %d %d %d %d %n", k, &x.block, &x.lds_stack, &x.blocks_per_cu, &x.traversal,
+                   &x.quad_tiles, &e, &n) == 7 &&
+            line[n] == '\0' && strcmp(k, key) == 0 && check_tuning(ctx, x) == RT_OK) {
+            *t = x;
+            *extra = e;
+            found = true;
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+static void cache_add(const char *path, const char *key, const rt_tuning &t, int extra)
+{
+    if (FILE *f = path ? fopen(path, "a") : nullptr) {
+        fprintf(f, "%s %d %d %d %d %d %d\n", key, t.block, t.lds_stack, t.blocks_per_cu, t.traversal, t.quad_tiles,
+                extra);
+        fclose(f);
+    }
+}
+
 int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const char *cache_path,
                      rt_tuning *best)
 {
     if (!ctx || !jobs || njobs == 0) return fail(ctx, RT_ERR_ARG, "rt_autotune_jobs: bad argument");
     hipSetDevice(ctx->device);
     const std::string key = tune_key(ctx, jobs, njobs);
-    if (cache_path) {
-        if (FILE *f = fopen(cache_path, "r")) {
-            char k[256];
-            rt_tuning t{};
-            while (fscanf(f, "%255s %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal) == 5) {
-                if (key == k && check_tuning(ctx, t) == RT_OK) {
-                    fclose(f);
-                    ctx->tune = t;
-                    if (best) *best = t;
-                    return RT_OK;
-                }
-            }
-            fclose(f);
+    {
+        rt_tuning t{};
+        int extra = 0;
+        if (cache_find(ctx, cache_path, key.c_str(), &t, &extra)) {
+            t.quad_tiles = ctx->tune.quad_tiles;  // quads are a direct-frame choice: kept as set
+            ctx->tune = t;
+            if (best) *best = t;
+            return RT_OK;
         }
     }
     // scratch accumulators so the search leaves the callers' buffers alone
@@ -1138,6 +1176,7 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
                 rt_tuning t = shape;
                 t.blocks_per_cu = bpc;
                 t.traversal = form;
+                t.quad_tiles = ctx->tune.quad_tiles;  // not used by jobs; kept for later direct frames
                 float ms = 0.0f;
                 rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);  // warm
                 if (rc) break;
@@ -1156,12 +1195,7 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
     if (rc) return rc;
     ctx->tune = bt;
     if (best) *best = bt;
-    if (cache_path) {
-        if (FILE *f = fopen(cache_path, "a")) {
-            fprintf(f, "%s %d %d %d %d\n", key.c_str(), bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal);
-            fclose(f);
-        }
-    }
+    cache_add(cache_path, key.c_str(), bt, 0);
     return RT_OK;
 }
 
@@ -1175,20 +1209,13 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     char key[160];
     snprintf(key, sizeof(key), "direct:cu%d:%s:t%u:d%u:%ux%u:l%u", ctx->cu_count, uses_bvh(*b, i) ? "bvh" : "lbvh",
              b->desc[i].nt, trav_depth(*b, i), desc->width, desc->height, desc->nlights);
-    if (cache_path) {
-        if (FILE *f = fopen(cache_path, "r")) {
-            char k[256];
-            rt_tuning t{};
-            while (fscanf(f, "%255s %d %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal,
-                          &t.quad_tiles) == 6) {
-                if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK) {
-                    fclose(f);
-                    ctx->tune = t;
-                    if (best) *best = t;
-                    return RT_OK;
-                }
-            }
-            fclose(f);
+    {
+        rt_tuning t{};
+        int extra = 0;
+        if (cache_find(ctx, cache_path, key, &t, &extra)) {
+            ctx->tune = t;
+            if (best) *best = t;
+            return RT_OK;
         }
     }
     const rt_tuning saved = ctx->tune;
@@ -1240,13 +1267,7 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
     ctx->tune = rc ? saved : bt;
     if (rc) return rc;
     if (best) *best = bt;
-    if (cache_path) {
-        if (FILE *f = fopen(cache_path, "a")) {
-            fprintf(f, "%s %d %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal,
-                    bt.quad_tiles);
-            fclose(f);
-        }
-    }
+    cache_add(cache_path, key, bt, 0);
     return RT_OK;
 }
 
@@ -1270,22 +1291,15 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     snprintf(key, sizeof(key), "whitted:cu%d:%s:t%u:d%u:%ux%u:b%u:s%u", ctx->cu_count,
              uses_bvh(*b, i) ? "bvh" : "lbvh", b->desc[i].nt, trav_depth(*b, i), desc->width, desc->height,
              desc->bounces, whitted_pass_form(desc->switches, 0));
-    if (cache_path) {
-        if (FILE *f = fopen(cache_path, "r")) {
-            char k[256];
-            rt_tuning t{};
-            int pf = 0;
-            while (fscanf(f, "%255s %d %d %d %d %d", k, &t.block, &t.lds_stack, &t.blocks_per_cu, &t.traversal,
-                          &pf) == 6) {
-                if (strcmp(key, k) == 0 && check_tuning(ctx, t) == RT_OK && pf >= 0 && pf <= 2) {
-                    fclose(f);
-                    ctx->tune = t;
-                    desc->switches = whitted_pass_form(desc->switches, pf);
-                    if (best) *best = t;
-                    return RT_OK;
-                }
-            }
-            fclose(f);
+    {
+        rt_tuning t{};
+        int pf = 0;
+        if (cache_find(ctx, cache_path, key, &t, &pf) && pf >= 0 && pf <= 2) {
+            t.quad_tiles = ctx->tune.quad_tiles;  // a direct-frame choice: kept as set
+            ctx->tune = t;
+            desc->switches = whitted_pass_form(desc->switches, pf);
+            if (best) *best = t;
+            return RT_OK;
         }
     }
     const rt_tuning saved = ctx->tune;
@@ -1301,6 +1315,7 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
                 rt_tuning t = shape;
                 t.blocks_per_cu = saved.blocks_per_cu;
                 t.traversal = form;
+                t.quad_tiles = saved.quad_tiles;  // not used by Whitted frames; kept for direct ones
                 ctx->tune = t;
                 float ms = 1e30f;
                 for (int rep = 0; rep < 3 && rc == RT_OK; ++rep) {  // first run warms caches
@@ -1321,12 +1336,7 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     if (rc) return rc;
     desc->switches = whitted_pass_form(desc->switches, bpf);
     if (best) *best = bt;
-    if (cache_path) {
-        if (FILE *f = fopen(cache_path, "a")) {
-            fprintf(f, "%s %d %d %d %d %d\n", key, bt.block, bt.lds_stack, bt.blocks_per_cu, bt.traversal, bpf);
-            fclose(f);
-        }
-    }
+    cache_add(cache_path, key, bt, bpf);
     return RT_OK;
 }
 
@@ -1379,6 +1389,9 @@ int rt_graph_capture_frame(rt_ctx *ctx, int scene, int rebuild, const rt_job_des
         hipFree(g.counter);
         return hip_fail(ctx, e, "rt_graph_capture_frame");
     }
+    g.batch = b;
+    g.generation = b->generation;
+    b->refs++;
     *graph = (int)ctx->graphs.size();
     ctx->graphs.push_back(g);
     return RT_OK;
@@ -1388,6 +1401,11 @@ int rt_graph_launch(rt_ctx *ctx, int graph)
 {
     if (!ctx || graph < 0 || graph >= (int)ctx->graphs.size() || !ctx->graphs[graph].exec)
         return fail(ctx, RT_ERR_ARG, "rt_graph_launch: unknown graph %d", graph);
+    const rt_ctx::Graph &gr = ctx->graphs[graph];
+    if (gr.batch->generation != gr.generation)
+        return fail(ctx, RT_ERR_STATE,
+                    "rt_graph_launch: graph %d is stale (a scene of its batch was destroyed, a BVH built or "
+                    "dropped, or the structure switched since capture); capture it again", graph);
     hipSetDevice(ctx->device);
     hipEvent_t *ev = ctx->profiling ? prof_events(ctx, 1, 2) : nullptr;
     if (ev) hipEventRecord(ev[0], ctx->stream);
@@ -1407,6 +1425,7 @@ int rt_graph_destroy(rt_ctx *ctx, int graph)
     hipGraphDestroy(g.graph);
     hipFree(g.jobs);
     hipFree(g.counter);
+    if (g.batch && --g.batch->refs == 0) batch_free(g.batch);
     g = rt_ctx::Graph();
     return RT_OK;
 }

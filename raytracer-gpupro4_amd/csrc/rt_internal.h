@@ -164,7 +164,11 @@ struct Batch {
     TravTri *tris = nullptr;
     uint64_t bytes = 0;
     std::vector<void *> allocs;
-    int refs = 0;                  // scene handles pointing here
+    int refs = 0;                  // scene handles and captured graphs pointing here
+    // Bumped whenever what a captured graph baked in goes stale: a scene
+    // handle destroyed, a BVH (re)built or dropped, the selected structure
+    // switched.  rt_graph_launch refuses a graph captured under another value.
+    uint64_t generation = 0;
     // Shading attributes (rt_scene_shading), allocated at their first upload:
     // normals in the vertex layout of xyz, material ids in the triangle
     // layout of idx, and one material table per scene.
@@ -255,6 +259,8 @@ struct rt_ctx {
         hipGraphExec_t exec = nullptr;
         void *jobs = nullptr;
         uint32_t *counter = nullptr;      // frame counter read by the job kernel
+        rt::Batch *batch = nullptr;       // held (refs) until rt_graph_destroy
+        uint64_t generation = 0;          // batch->generation at capture
     };
     std::vector<Graph> graphs;
     // profiling window: events come from a pool and are read back at the end

@@ -173,6 +173,60 @@ def test_graph_replay_equals_eager(ctx, rtcore):
         ctx.scene_destroy(h)
 
 
+def test_graph_refuses_replay_after_its_batch_changed(ctx, rtcore):
+    """A captured graph holds its batch: destroying the scene handle, building
+    or switching to a BVH, or a new upload dropping the BVH makes
+    rt_graph_launch fail with RT_ERR_STATE before anything runs; a fresh
+    capture works again; destroying the graph releases the batch."""
+    xyz, idx = rtcore.gen_object_mix(3000, seed=31)
+    c = xyz.mean(0)
+    cam = rtcore.camera_look_at(tuple(c + [0, 1, -6]), tuple(c), (0, 1, 0), 60.0, 1.0)
+    W = H = 16
+    acc = ctx.alloc(16 * W * H)
+    ctx.zero(acc, 16 * W * H)
+    base = ctx.memory_in_use()
+    s = ctx.add_scene(xyz, idx, 14)
+    ctx.build(s)
+    scene_bytes = ctx.memory_in_use() - base
+
+    def capture():
+        return ctx.graph_capture(s, [ctx.job(s, cam, W, H, acc, rtcore.MODE_GI, 1, 2, seed=3)], rebuild=True)
+
+    def refused(g):
+        with pytest.raises(rtcore.RtError) as e:
+            ctx.graph_launch(g)
+        assert e.value.code == rtcore.RT_ERR_STATE
+
+    g = capture()
+    ctx.graph_launch(g)
+    ctx.sync()
+    ctx.bvh_build(s)  # new BVH arrays, and the scene now traverses them
+    refused(g)
+    ctx.graph_destroy(g)
+    g = capture()
+    ctx.graph_launch(g)
+    ctx.sync()
+    ctx.scene_select(s, rtcore.STRUCTURE_LBVH)
+    refused(g)
+    ctx.graph_destroy(g)
+    ctx.bvh_build(s)
+    g = capture()
+    ctx.scene_upload(s, xyz, idx)  # new vertices drop the BVH the graph reads
+    refused(g)
+    ctx.graph_destroy(g)
+    ctx.build(s)
+    g = capture()
+    ctx.scene_upload(s, xyz, idx)  # an LBVH graph rebuilds from the new vertices: still valid
+    ctx.graph_launch(g)
+    ctx.sync()
+    ctx.scene_destroy(s)
+    refused(g)
+    assert ctx.memory_in_use() - base >= scene_bytes  # the graph still holds the batch
+    ctx.graph_destroy(g)
+    assert ctx.memory_in_use() - base < scene_bytes
+    ctx.free(acc)
+
+
 def test_autotuner_picks_and_caches(ctx, rtcore, tmp_path):
     xyz, idx = rtcore.gen_object_mix(20000, seed=4)
     s = ctx.add_scene(xyz, idx, 18)
@@ -183,13 +237,16 @@ def test_autotuner_picks_and_caches(ctx, rtcore, tmp_path):
     ctx.zero(acc, 16 * 128 * 128)
     jobs = [ctx.job(s, cam, 128, 128, acc, rtcore.MODE_GI, 1, 2, 0.7, 1.0)]
     cache = str(tmp_path / "tune.txt")
+    saved = ctx.tuning()
+    ctx.set_tuning(128, 26, 4, 0, 64)
     best = ctx.autotune(jobs, cache)
     assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] in (2, 4, 8) and best[3] in (0, 1, 2)
     assert ctx.tuning() == best
-    assert best[4] == 0  # quads are for direct frames only
-    assert open(cache).read().strip().endswith(" ".join(str(v) for v in best[:4]))
-    ctx.set_tuning(128, 26, 4)
+    assert best[4] == 64  # quads are a direct-frame choice: the search keeps the one set
+    assert open(cache).read().split()[-6:] == [str(v) for v in best] + ["0"]
+    ctx.set_tuning(128, 26, 4, 0, 64)
     assert ctx.autotune(jobs, cache) == best  # cache hit
+    ctx.set_tuning(*saved)
     # the search rendered into scratch buffers, not into the caller's
     assert np.all(ctx.read_accum(acc, 128, 128) == 0)
     ctx.free(acc)
@@ -228,8 +285,22 @@ def test_autotune_direct_picks_and_caches(ctx, rtcore, mix, tmp_path):
         assert best[0] in (64, 128, 256) and 1 <= best[1] <= 26 and best[2] == saved[2] and best[3] in (0, 1, 2)
         assert best[4] in (0, 64, 256, 1024)  # quads on the heaviest tiles, or none
         assert ctx.tuning() == best
-        assert open(cache).read().split()[-5:] == [str(v) for v in best]
+        assert open(cache).read().split()[-6:] == [str(v) for v in best] + ["0"]
         ctx.set_tuning(64, 8, saved[2])
         assert ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache) == best  # cache hit
+        # one cache file shared by the three searches: every record keeps its own hit
+        acc = ctx.alloc(16 * 64 * 48)
+        jobs = [ctx.job(s, cam, 64, 48, acc, rtcore.MODE_GI, 1, 2, 0.7, 1.0)]
+        bj = ctx.autotune(jobs, cache)
+        bw = ctx.autotune_whitted(s, cam, 96, 64, (0.0, 5.0, 0.0), bounces=1, cache_path=cache)
+        with open(cache, "a") as f:
+            f.write("garbage line with words\n" + "direct:short 1 2\n")
+        ctx.set_tuning(64, 8, saved[2])
+        assert ctx.autotune_direct(s, cam, 96, 64, [(0.0, 5.0, 0.0)], cache_path=cache) == best
+        assert ctx.autotune(jobs, cache)[:4] == bj[:4]
+        tw, sw = ctx.autotune_whitted(s, cam, 96, 64, (0.0, 5.0, 0.0), bounces=1, cache_path=cache)
+        assert tw[:4] == bw[0][:4] and sw == bw[1]
+        assert len(open(cache).read().splitlines()) == 5  # hits append nothing
+        ctx.free(acc)
     finally:
         ctx.set_tuning(*saved)
