@@ -410,6 +410,20 @@ int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t);
  * kernel time and ray counts. */
 int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
                    uint32_t flags, rt_frame_stats *stats);
+/* rt_render_jobs without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, code that must not call hipMalloc).  The job table, the
+ * tile counter and the ray counters live in one static device array per GPU
+ * (RT_NOALLOC_MAX_JOBS job records + 512 bytes, about 512 KiB, held until
+ * the process ends); the caller passes the scenes and accumulators as for
+ * rt_render_jobs, and the results are the same.  More than
+ * RT_NOALLOC_MAX_JOBS jobs return RT_ERR_CAPACITY.  Every context on one GPU
+ * shares that array: calls that overlap on one GPU (two contexts, or two
+ * host threads) race and must be serialised by the caller.  Not allowed
+ * inside a profiling window (RT_ERR_STATE). */
+#define RT_NOALLOC_MAX_JOBS 1024
+int rt_render_jobs_noalloc(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
+                           uint32_t flags, rt_frame_stats *stats);
 /* Times every launch shape on these jobs (into scratch accumulators) and
  * keeps the fastest in the context.  cache_path (optional) is a text file of
  * earlier results keyed by the jobs' shape; a hit skips the search. */

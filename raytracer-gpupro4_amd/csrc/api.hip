@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "rt_internal.h"
 
@@ -287,6 +288,42 @@ int create_batch(rt_ctx *ctx, uint32_t n, const uint32_t *max_tris, const uint32
 
 extern "C" {
 
+// ---------------------------------------------------------------------------
+// Scratch of rt_render_jobs_noalloc: one static device array per GPU holding
+// the job table (RT_NOALLOC_MAX_JOBS records), the persistent launch's tile
+// counter and the ray counters.  In host code the array's name is not a
+// device address: rt_create looks the address up once per device
+// (hipGetSymbolAddress, with the device current) and every host use goes
+// through that pointer.  All calls on one GPU share the array.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocTableBytes = (size_t)RT_NOALLOC_MAX_JOBS * rt::kRenderJobBytesMax;
+constexpr size_t kNoAllocCounterOff = kNoAllocTableBytes;       // 256 B: tile counter
+constexpr size_t kNoAllocStatsOff = kNoAllocTableBytes + 256;   // 8 x u64 ray counters
+constexpr size_t kNoAllocScratchBytes = kNoAllocTableBytes + 512;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char *> g_scratch_ptr;  // per device; null until looked up
+
+static int noalloc_setup(rt_ctx *ctx)
+{
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if ((int)g_scratch_ptr.size() <= ctx->device) g_scratch_ptr.resize(ctx->device + 1, nullptr);
+        if (!g_scratch_ptr[ctx->device]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // device is current
+            if (e != hipSuccess || !p) return hip_fail(ctx, e, "rt_create: hipGetSymbolAddress(scratch)");
+            g_scratch_ptr[ctx->device] = static_cast<unsigned char *>(p);
+        }
+        ctx->noalloc_scratch = g_scratch_ptr[ctx->device];
+    }
+    ctx->noalloc_host.assign((size_t)RT_NOALLOC_MAX_JOBS * rt::render_job_bytes(), 0);
+    ctx->noalloc_offs.assign(RT_NOALLOC_MAX_JOBS + 1, 0u);
+    RT_HIP(ctx, hipEventCreateWithFlags(&ctx->noalloc_ev, hipEventDisableTiming));
+    return RT_OK;
+}
+
 int rt_create(rt_ctx **out, int device)
 {
     if (!out) return RT_ERR_ARG;
@@ -313,6 +350,10 @@ int rt_create(rt_ctx **out, int device)
         return RT_ERR_NOMEM;
     }
     ctx->bytes += 8 * sizeof(unsigned long long) + 256 + 128;
+    if (noalloc_setup(ctx) != RT_OK) {
+        rt_destroy(ctx);
+        return RT_ERR_HIP;
+    }
     *out = ctx;
     return RT_OK;
 }
@@ -344,6 +385,7 @@ void rt_destroy(rt_ctx *ctx)
     hipFree(ctx->wqueue);
     hipFree(ctx->jobs_dev);
     if (ctx->jobs_ev) hipEventDestroy(ctx->jobs_ev);
+    if (ctx->noalloc_ev) hipEventDestroy(ctx->noalloc_ev);
     for (auto &e : ctx->ev)
         if (e) hipEventDestroy(e);
     for (auto &e : ctx->pool) hipEventDestroy(e);
@@ -975,14 +1017,14 @@ int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t)
 }
 
 // Builds the device job table for jobs; returns the total tile count.
-static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const rt_tuning &t,
-                      void *dst_dev, std::vector<uint32_t> *tile_offs, uint32_t *total, uint32_t *max_depth)
+// Checks the jobs and writes their device records into host (render_job_bytes()
+// each) and the first tile of each job into offs[0..njobs] (optional).
+static int fill_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const rt_tuning &t,
+                     unsigned char *host, uint32_t *offs, uint32_t *total, uint32_t *max_depth)
 {
     const size_t jb = rt::render_job_bytes();
-    std::vector<unsigned char> host(jb * njobs);
     uint32_t off = 0;
     *max_depth = 0;
-    if (tile_offs) tile_offs->clear();
     for (uint32_t k = 0; k < njobs; ++k) {
         const rt_job_desc &d = jobs[k];
         Batch *b;
@@ -996,13 +1038,24 @@ static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, cons
             (d.mode != RT_MODE_DIRECT && d.mode != RT_MODE_GI) || !rt::camera_ok(d.cam))
             return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights, mode or camera", k);
         uint32_t tiles = 0;
-        rt::fill_render_job(host.data() + jb * k, *b, i, d, t, off, &tiles);
+        rt::fill_render_job(host + jb * k, *b, i, d, t, off, &tiles);
         if (trav_depth(*b, i) > *max_depth) *max_depth = trav_depth(*b, i);
-        if (tile_offs) tile_offs->push_back(off);
+        if (offs) offs[k] = off;
         off += tiles;
     }
-    if (tile_offs) tile_offs->push_back(off);
+    if (offs) offs[njobs] = off;
     *total = off;
+    return RT_OK;
+}
+
+static int stage_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const rt_tuning &t,
+                      void *dst_dev, std::vector<uint32_t> *tile_offs, uint32_t *total, uint32_t *max_depth)
+{
+    const size_t jb = rt::render_job_bytes();
+    std::vector<unsigned char> host(jb * njobs);
+    if (tile_offs) tile_offs->assign(njobs + 1, 0u);
+    int rc = fill_jobs(ctx, jobs, njobs, t, host.data(), tile_offs ? tile_offs->data() : nullptr, total, max_depth);
+    if (rc) return rc;
     // the previous launch may still read the table: wait for it before overwriting
     RT_HIP(ctx, hipEventSynchronize(ctx->jobs_ev));
     RT_HIP(ctx, hipMemcpy(dst_dev, host.data(), jb * njobs, hipMemcpyHostToDevice));
@@ -1083,6 +1136,59 @@ int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32_
     if (njobs == 0) return RT_OK;
     hipSetDevice(ctx->device);
     return run_jobs(ctx, jobs, njobs, flags, ctx->tune, stats, nullptr);
+}
+
+int rt_render_jobs_noalloc(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32_t flags,
+                           rt_frame_stats *stats)
+{
+    if (!ctx || (njobs && !jobs)) return fail(ctx, RT_ERR_ARG, "rt_render_jobs_noalloc: bad argument");
+    if (njobs > RT_NOALLOC_MAX_JOBS)
+        return fail(ctx, RT_ERR_CAPACITY, "rt_render_jobs_noalloc: %u jobs, at most %d", njobs,
+                    RT_NOALLOC_MAX_JOBS);
+    if (njobs == 0) return RT_OK;
+    if (!ctx->noalloc_scratch) return fail(ctx, RT_ERR_STATE, "rt_render_jobs_noalloc: no scratch on this device");
+    if (ctx->profiling) return fail(ctx, RT_ERR_STATE, "rt_render_jobs_noalloc: inside a profiling window");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    // the previous call's copy may still read the host staging: wait for it
+    RT_HIP(ctx, hipEventSynchronize(ctx->noalloc_ev));
+    uint32_t total = 0, depth = 0;
+    int rc = fill_jobs(ctx, jobs, njobs, ctx->tune, ctx->noalloc_host.data(), ctx->noalloc_offs.data(), &total,
+                       &depth);
+    if (rc) return rc;
+    unsigned char *base = ctx->noalloc_scratch;
+    void *d_jobs = base;
+    uint32_t *d_counter = reinterpret_cast<uint32_t *>(base + kNoAllocCounterOff);
+    unsigned long long *d_stats = reinterpret_cast<unsigned long long *>(base + kNoAllocStatsOff);
+    // ordered on the stream after earlier launches that read the table
+    RT_HIP(ctx, hipMemcpyAsync(d_jobs, ctx->noalloc_host.data(), rt::render_job_bytes() * njobs,
+                               hipMemcpyHostToDevice, st));
+    RT_HIP(ctx, hipEventRecord(ctx->noalloc_ev, st));
+    RT_HIP(ctx, rt::launch_zero_words(reinterpret_cast<uint32_t *>(d_stats), 16, st));
+    if (stats) RT_HIP(ctx, hipEventRecord(ctx->ev[6], st));
+    const uint32_t *offs = ctx->noalloc_offs.data();
+    if (flags & RT_FLAG_PERSISTENT) {
+        RT_HIP(ctx, rt::launch_render_jobs(d_jobs, njobs, total, 0, total, true, d_counter, nullptr, d_stats,
+                                           ctx->tune, depth, ctx->cu_count, st));
+    } else {
+        for (uint32_t k = 0; k < njobs; ++k)
+            RT_HIP(ctx, rt::launch_render_jobs(d_jobs, njobs, total, offs[k], offs[k + 1] - offs[k], false,
+                                               d_counter, nullptr, d_stats, ctx->tune, depth, ctx->cu_count, st));
+    }
+    if (stats) {
+        RT_HIP(ctx, hipEventRecord(ctx->ev[7], st));
+        unsigned long long c[8] = {};
+        RT_HIP(ctx, hipMemcpyAsync(c, d_stats, sizeof(c), hipMemcpyDeviceToHost, st));
+        RT_HIP(ctx, hipStreamSynchronize(st));
+        memset(stats, 0, sizeof(*stats));
+        hipEventElapsedTime(&stats->trace_ms, ctx->ev[6], ctx->ev[7]);
+        stats->primary_rays = c[4];
+        stats->shadow_rays = c[2];
+        stats->bounce_rays = c[3];
+        stats->box_tests = c[0];
+        stats->tri_tests = c[1];
+    }
+    return RT_OK;
 }
 
 static std::string tune_key(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs)

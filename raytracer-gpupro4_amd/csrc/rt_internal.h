@@ -254,6 +254,13 @@ struct rt_ctx {
     void *jobs_dev = nullptr;             // render-job table
     uint64_t jobs_cap = 0;
     hipEvent_t jobs_ev = nullptr;         // last launch that read jobs_dev
+    // rt_render_jobs_noalloc: this device's static scratch array (job table,
+    // tile counter, ray counters; set up by rt_create), and host staging of
+    // RT_NOALLOC_MAX_JOBS job records and tile offsets, sized at rt_create
+    unsigned char *noalloc_scratch = nullptr;
+    std::vector<unsigned char> noalloc_host;
+    std::vector<uint32_t> noalloc_offs;
+    hipEvent_t noalloc_ev = nullptr;      // the last copy out of noalloc_host
     struct Graph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -312,6 +319,9 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
                               int cu_count,
                               hipStream_t st);
 hipError_t launch_bump_counter(uint32_t *c, hipStream_t st);
+hipError_t launch_zero_words(uint32_t *p, uint32_t n, hipStream_t st);
+// Upper bound of render_job_bytes() (checked where the record is defined).
+constexpr size_t kRenderJobBytesMax = 512;
 
 // SAH BVH (bvh.cpp).
 int bvh_build_host(const float *xyz, const uint32_t *idx, uint32_t nt, uint32_t max_prims, uint32_t split,

@@ -2238,6 +2238,7 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
     *tiles = d.tile_count ? (d.tile_count < all - j->local_base ? d.tile_count : all - j->local_base) : all;
 }
 
+static_assert(sizeof(RenderJob) <= kRenderJobBytesMax, "RenderJob outgrew the no-allocation job table");
 size_t render_job_bytes() { return sizeof(RenderJob); }
 
 hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,
@@ -2270,6 +2271,12 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
 hipError_t launch_bump_counter(uint32_t *c, hipStream_t st)
 {
     hipLaunchKernelGGL(k_bump_counter, dim3(1), dim3(64), 0, st, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(uint32_t *p, uint32_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, p, n);
     return hipGetLastError();
 }
 

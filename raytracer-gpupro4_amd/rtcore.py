@@ -33,6 +33,7 @@ FLAG_PERSISTENT = 1 << 4
 FLAG_START_AT_IROOT = 1 << 5
 MODE_DIRECT = 0
 MODE_GI = 1
+NOALLOC_MAX_JOBS = 1024
 CAMERA_BASIS = 0
 CAMERA_WORLD = 1
 WHITTED_SHADOWS = 1 << 0
@@ -226,6 +227,7 @@ def lib() -> C.CDLL:
         "rt_tuning_get": (I32, [P, C.POINTER(Tuning)]),
         "rt_tuning_set": (I32, [P, C.POINTER(Tuning)]),
         "rt_render_jobs": (I32, [P, P, U32, U32, C.POINTER(FrameStats)]),
+        "rt_render_jobs_noalloc": (I32, [P, P, U32, U32, C.POINTER(FrameStats)]),
         "rt_autotune_jobs": (I32, [P, P, U32, C.c_char_p, C.POINTER(Tuning)]),
         "rt_autotune_direct": (I32, [P, I32, C.POINTER(RenderDesc), C.c_char_p, C.POINTER(Tuning)]),
         "rt_graph_capture_frame": (I32, [P, I32, I32, P, U32, C.POINTER(I32)]),
@@ -507,6 +509,15 @@ class Context:
         flags = FLAG_PERSISTENT if persistent else 0
         self.check(self.L.rt_render_jobs(self.h, arr, len(jobs), flags,
                                          C.byref(st) if want_stats else None), "rt_render_jobs")
+        return st.as_dict() if want_stats else None
+
+    def render_jobs_noalloc(self, jobs, persistent=True, want_stats=True):
+        """rt_render_jobs_noalloc: the same render with no device allocation."""
+        arr = (JobDesc * len(jobs))(*jobs)
+        st = FrameStats()
+        flags = FLAG_PERSISTENT if persistent else 0
+        self.check(self.L.rt_render_jobs_noalloc(self.h, arr, len(jobs), flags,
+                                                 C.byref(st) if want_stats else None), "rt_render_jobs_noalloc")
         return st.as_dict() if want_stats else None
 
     def autotune(self, jobs, cache_path=None):
