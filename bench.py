@@ -36,7 +36,9 @@ Other workloads (--workload), each printing one JSON line of the same shape:
   serve    Poisson 200 requests/s for 60 s over a pool of 500 scenes
            (1k..500k triangles), 512^2 or 1920x1080, direct or 2-bounce GI at
            1 spp with progressive accumulation, 200 GB device budget; p50/p99
-           latency and aggregate Mrays/s against one request per launch.
+           latency and aggregate Mrays/s against one request per launch (the
+           server with a 1 ms batch window and centre-first tiles, and without
+           the window).
 """
 import argparse
 import json
@@ -681,6 +683,11 @@ def workload_batched(args, rtcore):
 
 
 # ------------------------------------------------------------------ serve
+# The server holds an arrived request up to this long, while nothing is in
+# flight, so that later arrivals share its launch (rt_server_config.batch_window).
+SERVE_WINDOW = 0.001
+
+
 def workload_serve(args, rtcore):
     rng = np.random.default_rng(55)
     npool = 500
@@ -710,8 +717,9 @@ def workload_serve(args, rtcore):
         light = tuple(xyz.max(0) + 0.5 * (xyz.max(0) - xyz.min(0)))
         return k, int(view_of[r]), w, h, rtcore.MODE_DIRECT, 0, cam, [light]
 
-    def served(ctx, arr):
-        srv = rtcore.Server(ctx, 200 << 30, max_inflight=32, tiles_per_step=32768)
+    def served(ctx, arr, window=SERVE_WINDOW):
+        srv = rtcore.Server(ctx, 200 << 30, max_inflight=32, tiles_per_step=32768,
+                            tile_order=rtcore.TILES_CENTRE, batch_window=window)
         for k, (x, i) in pool.items():
             srv.add_scene(k, x, i, depth_for(len(i)))
         for r, t in enumerate(arr):
@@ -735,7 +743,7 @@ def workload_serve(args, rtcore):
                 "mrays_s_clock": round(st["rays"] / t_end / 1e6, 2),
                 "device_busy_frac": round(st["device_s"] / t_end, 4), "clock_s": round(t_end, 3),
                 "wall_s": round(wall, 2), "steps": int(st["steps"]), "builds": int(st["builds"]),
-                "evictions": int(st["evictions"]), "rays": int(st["rays"])}
+                "evictions": int(st["evictions"]), "rays": int(st["rays"]), "batch_window_ms": window * 1e3}
 
     def plain(ctx, arr):
         """The plain per-frame path: requests in arrival order, one at a time,
@@ -789,6 +797,8 @@ def workload_serve(args, rtcore):
     srv.close()
     s_poisson = served(ctx, arrivals)
     log(f"serve: continuous batching, Poisson arrivals: {s_poisson}")
+    s_nowin = served(ctx, arrivals, window=0.0)
+    log(f"serve: continuous batching without the batch window: {s_nowin}")
     nplain = min(nreq, int(rate * args.plain_seconds))
     p_poisson = plain(ctx, arrivals[:nplain])
     log(f"serve: plain path, Poisson arrivals: {p_poisson}")
@@ -805,7 +815,8 @@ def workload_serve(args, rtcore):
                                "200 GB device budget",
                    "requests": nreq, "pool": npool},
                   p50_ms=s_poisson["p50_ms"], p99_ms=s_poisson["p99_ms"],
-                  served=s_poisson, plain=p_poisson,
+                  served=s_poisson, served_no_window=s_nowin, plain=p_poisson,
+                  served_vs_plain_device=round(s_poisson["mrays_s_device"] / p_poisson["mrays_s_device"], 3),
                   burst={"requests": nburst, "served": s_burst, "plain": p_burst,
                          "speedup_clock": round(s_burst["mrays_s_clock"] / p_burst["mrays_s_clock"], 3),
                          "speedup_device": round(s_burst["mrays_s_device"] / p_burst["mrays_s_device"], 3)},
@@ -839,8 +850,9 @@ def workload_all(args, rtcore):
                     "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"],
                     "served_mrays_s_device": sv["served"]["mrays_s_device"],
                     "plain_mrays_s_device": sv["plain"]["mrays_s_device"],
-                    "served_vs_plain_device": round(sv["served"]["mrays_s_device"] /
-                                                    sv["plain"]["mrays_s_device"], 3),
+                    "served_vs_plain_device": sv["served_vs_plain_device"],
+                    "batch_window_ms": SERVE_WINDOW * 1e3,
+                    "no_window": {k: sv["served_no_window"][k] for k in ("p50_ms", "p99_ms", "mrays_s_device")},
                     "plain_p50_ms": sv["plain"]["p50_ms"], "plain_p99_ms": sv["plain"]["p99_ms"],
                     "device_busy_frac": sv["served"]["device_busy_frac"],
                     "burst_speedup_device": sv["burst"]["speedup_device"],

@@ -172,10 +172,17 @@ typedef struct rt_job_desc {
     uint64_t seed;
     uint32_t frame;
     float *accum;
-    /* optional tile range (tiles of 16 x block/16 pixels, row-major):
-     * tile_count == 0 renders the whole image */
+    /* optional tile range (tiles of 16 x block/16 pixels, positions in the
+     * job's tile order): tile_count == 0 renders the whole image */
     uint32_t tile_begin, tile_count;
+    /* order in which a persistent launch's queue hands out the job's tiles:
+     * RT_TILES_ROWS (row-major) or RT_TILES_CENTRE (nearest the image centre
+     * first, where a camera aimed at its scene puts the longest paths, so
+     * they start early and the launch's tail is made of short tiles).  The
+     * image does not depend on it. */
+    uint32_t tile_order;
 } rt_job_desc;
+enum { RT_TILES_ROWS = 0, RT_TILES_CENTRE = 1 };
 
 typedef struct rt_ctx rt_ctx;
 
@@ -420,7 +427,10 @@ int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
  * RT_NOALLOC_MAX_JOBS jobs return RT_ERR_CAPACITY.  Every context on one GPU
  * shares that array: calls that overlap on one GPU (two contexts, or two
  * host threads) race and must be serialised by the caller.  Not allowed
- * inside a profiling window (RT_ERR_STATE). */
+ * inside a profiling window (RT_ERR_STATE).  A RT_TILES_CENTRE job uses the
+ * order an earlier rt_render_jobs of that image size made, or else renders
+ * in row order (the same image); a tile range over an order not made yet is
+ * RT_ERR_STATE. */
 #define RT_NOALLOC_MAX_JOBS 1024
 int rt_render_jobs_noalloc(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
                            uint32_t flags, rt_frame_stats *stats);
@@ -471,7 +481,17 @@ typedef struct rt_server_config {
     uint32_t tiles_per_step; /* tiles per launch, shared by in-flight requests */
     int32_t dry_run;
     double dry_rate;         /* rays per second assumed by dry_run */
+    int32_t autotune;        /* 1: once RT_SERVER_TUNE_REQUESTS requests have completed,
+                                time the launch shapes of the context's block size on
+                                their whole images (scratch accumulators, as
+                                rt_autotune_jobs) and launch every later step with the
+                                fastest; the context's own tuning is left as it is */
+    uint32_t tile_order;     /* tile order of the server's jobs (RT_TILES_ROWS / _CENTRE) */
+    double batch_window;     /* seconds an arrived request may wait, while nothing is in
+                                flight, for later arrivals to share its launch (0: none;
+                                a full batch of max_inflight requests starts at once) */
 } rt_server_config;
+#define RT_SERVER_TUNE_REQUESTS 8
 typedef struct rt_request {
     uint32_t scene_key, view;
     rt_camera cam;
@@ -496,6 +516,10 @@ typedef struct rt_server_stats {
     uint64_t bytes_used;
     uint32_t pending, inflight, resident_scenes, accumulators;
     double device_s;         /* summed device time of the step launches (dry run: simulated) */
+    double tune_s;           /* wall time of the launch-shape search (autotune) */
+    rt_tuning tuning;        /* the shape the steps launch with */
+    double wake;             /* when the last step held arrived requests for the batch
+                                window: the time it ends (a step then does work); else 0 */
 } rt_server_stats;
 int rt_server_create(rt_ctx *ctx, const rt_server_config *cfg, rt_server **out);
 void rt_server_destroy(rt_server *srv);

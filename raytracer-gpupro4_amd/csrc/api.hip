@@ -386,6 +386,7 @@ void rt_destroy(rt_ctx *ctx)
     hipFree(ctx->jobs_dev);
     if (ctx->jobs_ev) hipEventDestroy(ctx->jobs_ev);
     if (ctx->noalloc_ev) hipEventDestroy(ctx->noalloc_ev);
+    for (auto &kv : ctx->tile_orders) hipFree(kv.second);
     for (auto &e : ctx->ev)
         if (e) hipEventDestroy(e);
     for (auto &e : ctx->pool) hipEventDestroy(e);
@@ -1017,10 +1018,43 @@ int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t)
 }
 
 // Builds the device job table for jobs; returns the total tile count.
+// The device copy of the RT_TILES_CENTRE order of a width x height image in
+// tiles of 16 x tile_h, made on first use and kept until rt_destroy; null with
+// may_alloc false when it is not there yet.
+static int centre_order(rt_ctx *ctx, uint32_t width, uint32_t height, uint32_t tile_h, bool may_alloc,
+                        const uint32_t **out)
+{
+    *out = nullptr;
+    const auto key = std::make_tuple(width, height, tile_h);
+    auto it = ctx->tile_orders.find(key);
+    if (it != ctx->tile_orders.end()) {
+        *out = it->second;
+        return RT_OK;
+    }
+    if (!may_alloc) return RT_OK;
+    std::vector<uint32_t> order;
+    rt::centre_tile_order(width, height, tile_h, order);
+    uint32_t *d = nullptr;
+    RT_HIP(ctx, hipMalloc(&d, sizeof(uint32_t) * order.size()));
+    const hipError_t e = hipMemcpy(d, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return hip_fail(ctx, e, "tile order upload");
+    }
+    ctx->bytes += sizeof(uint32_t) * order.size();
+    ctx->tile_orders.emplace(key, d);
+    *out = d;
+    return RT_OK;
+}
+
 // Checks the jobs and writes their device records into host (render_job_bytes()
 // each) and the first tile of each job into offs[0..njobs] (optional).
+// may_alloc false (rt_render_jobs_noalloc): a RT_TILES_CENTRE job whose order
+// is not on the device yet renders in row order, which gives the same image
+// when it covers the whole image; a tile range over such an order is refused.
 static int fill_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const rt_tuning &t,
-                     unsigned char *host, uint32_t *offs, uint32_t *total, uint32_t *max_depth)
+                     unsigned char *host, uint32_t *offs, uint32_t *total, uint32_t *max_depth,
+                     bool may_alloc = true)
 {
     const size_t jb = rt::render_job_bytes();
     uint32_t off = 0;
@@ -1035,10 +1069,18 @@ static int fill_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
             return fail(ctx, RT_ERR_ARG, "job %u: accum must be a 16-byte aligned device pointer", k);
         if (d.width == 0 || d.height == 0 || d.width > 16384 || d.height > 16384 || d.spp == 0 ||
             d.spp > 65536 || d.max_bounces > 64 || d.nlights > 8 || (d.nlights && !d.lights) ||
-            (d.mode != RT_MODE_DIRECT && d.mode != RT_MODE_GI) || !rt::camera_ok(d.cam))
-            return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights, mode or camera", k);
+            (d.mode != RT_MODE_DIRECT && d.mode != RT_MODE_GI) || !rt::camera_ok(d.cam) ||
+            (d.tile_order != RT_TILES_ROWS && d.tile_order != RT_TILES_CENTRE))
+            return fail(ctx, RT_ERR_ARG, "job %u: bad size, spp, bounces, lights, mode, camera or tile order", k);
+        const uint32_t *order = nullptr;
+        if (d.tile_order == RT_TILES_CENTRE) {
+            int rc = centre_order(ctx, d.width, d.height, (uint32_t)t.block / 16, may_alloc, &order);
+            if (rc) return rc;
+            if (!order && d.tile_count)
+                return fail(ctx, RT_ERR_STATE, "job %u: a tile range over a centre order not made yet", k);
+        }
         uint32_t tiles = 0;
-        rt::fill_render_job(host + jb * k, *b, i, d, t, off, &tiles);
+        rt::fill_render_job(host + jb * k, *b, i, d, t, off, &tiles, order);
         if (trav_depth(*b, i) > *max_depth) *max_depth = trav_depth(*b, i);
         if (offs) offs[k] = off;
         off += tiles;
@@ -1154,7 +1196,7 @@ int rt_render_jobs_noalloc(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
     RT_HIP(ctx, hipEventSynchronize(ctx->noalloc_ev));
     uint32_t total = 0, depth = 0;
     int rc = fill_jobs(ctx, jobs, njobs, ctx->tune, ctx->noalloc_host.data(), ctx->noalloc_offs.data(), &total,
-                       &depth);
+                       &depth, false);
     if (rc) return rc;
     unsigned char *base = ctx->noalloc_scratch;
     void *d_jobs = base;
@@ -1245,6 +1287,69 @@ static void cache_add(const char *path, const char *key, const rt_tuning &t, int
     }
 }
 
+// Times every launch shape (block size fixed_block only, when non-zero) on
+// the jobs in persistent launches into scratch accumulators; *best = the
+// fastest (quad_tiles kept from the context).  The context's tuning is left as is.
+static int search_jobs_shape(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, int fixed_block,
+                             rt_tuning *best)
+{
+    // scratch accumulators so the search leaves the callers' buffers alone
+    std::vector<rt_job_desc> tj(jobs, jobs + njobs);
+    uint64_t need = 0;
+    for (const rt_job_desc &d : tj) need += 16ull * d.width * d.height;
+    void *scratch = nullptr;
+    RT_HIP(ctx, hipMalloc(&scratch, need ? need : 16));
+    uint64_t off = 0;
+    for (rt_job_desc &d : tj) {
+        d.accum = reinterpret_cast<float *>(static_cast<char *>(scratch) + off);
+        off += 16ull * d.width * d.height;
+    }
+    rt_tuning bt = ctx->tune;
+    float bms = 1e30f;
+    bool any = false;
+    const int bpcs[] = {2, 4, 8};
+    int rc = RT_OK;
+    for (int form = 0; form < kForms && rc == RT_OK; ++form) {
+        for (const rt_tuning &shape : kShapes) {
+            if (fixed_block && shape.block != fixed_block) continue;
+            for (int bpc : bpcs) {
+                rt_tuning t = shape;
+                t.blocks_per_cu = bpc;
+                t.traversal = form;
+                t.quad_tiles = ctx->tune.quad_tiles;  // not used by jobs; kept for later direct frames
+                float ms = 0.0f;
+                rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);  // warm
+                if (rc) break;
+                rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);
+                if (rc) break;
+                if (ms < bms) {
+                    bms = ms;
+                    bt = t;
+                    any = true;
+                }
+            }
+            if (rc) break;
+        }
+    }
+    hipStreamSynchronize(ctx->stream);
+    hipFree(scratch);
+    if (rc) return rc;
+    if (!any) return fail(ctx, RT_ERR_ARG, "launch shape search: no shape of block %d", fixed_block);
+    *best = bt;
+    return RT_OK;
+}
+
+}  // extern "C"
+
+namespace rt {
+int search_jobs_shape_block(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, int block, rt_tuning *best)
+{
+    return search_jobs_shape(ctx, jobs, njobs, block, best);
+}
+}  // namespace rt
+
+extern "C" {
+
 int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const char *cache_path,
                      rt_tuning *best)
 {
@@ -1261,43 +1366,8 @@ int rt_autotune_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, const
             return RT_OK;
         }
     }
-    // scratch accumulators so the search leaves the callers' buffers alone
-    std::vector<rt_job_desc> tj(jobs, jobs + njobs);
-    uint64_t need = 0;
-    for (const rt_job_desc &d : tj) need += 16ull * d.width * d.height;
-    void *scratch = nullptr;
-    RT_HIP(ctx, hipMalloc(&scratch, need ? need : 16));
-    uint64_t off = 0;
-    for (rt_job_desc &d : tj) {
-        d.accum = reinterpret_cast<float *>(static_cast<char *>(scratch) + off);
-        off += 16ull * d.width * d.height;
-    }
     rt_tuning bt = ctx->tune;
-    float bms = 1e30f;
-    const int bpcs[] = {2, 4, 8};
-    int rc = RT_OK;
-    for (int form = 0; form < kForms && rc == RT_OK; ++form) {
-        for (const rt_tuning &shape : kShapes) {
-            for (int bpc : bpcs) {
-                rt_tuning t = shape;
-                t.blocks_per_cu = bpc;
-                t.traversal = form;
-                t.quad_tiles = ctx->tune.quad_tiles;  // not used by jobs; kept for later direct frames
-                float ms = 0.0f;
-                rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);  // warm
-                if (rc) break;
-                rc = run_jobs(ctx, tj.data(), njobs, RT_FLAG_PERSISTENT, t, nullptr, &ms);
-                if (rc) break;
-                if (ms < bms) {
-                    bms = ms;
-                    bt = t;
-                }
-            }
-            if (rc) break;
-        }
-    }
-    hipStreamSynchronize(ctx->stream);
-    hipFree(scratch);
+    const int rc = search_jobs_shape(ctx, jobs, njobs, 0, &bt);
     if (rc) return rc;
     ctx->tune = bt;
     if (best) *best = bt;

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "rtcore.h"
@@ -261,6 +263,8 @@ struct rt_ctx {
     std::vector<unsigned char> noalloc_host;
     std::vector<uint32_t> noalloc_offs;
     hipEvent_t noalloc_ev = nullptr;      // the last copy out of noalloc_host
+    // RT_TILES_CENTRE orders on the device, by (width, height, tile rows)
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint32_t *> tile_orders;
     struct Graph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
@@ -310,8 +314,9 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
                                  hipStream_t st);
 size_t whitted_queue_bytes(uint64_t pixels);
 size_t render_job_bytes();
+// order (device, optional): the tile at each queue position of the job.
 void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_desc &d,
-                     const rt_tuning &t, uint32_t tile_off, uint32_t *tiles);
+                     const rt_tuning &t, uint32_t tile_off, uint32_t *tiles, const uint32_t *order);
 hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,
                               uint32_t first_tile, uint32_t tile_count, bool persistent,
                               uint32_t *tile_counter, const uint32_t *frame_counter,
@@ -320,6 +325,13 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
                               hipStream_t st);
 hipError_t launch_bump_counter(uint32_t *c, hipStream_t st);
 hipError_t launch_zero_words(uint32_t *p, uint32_t n, hipStream_t st);
+// Times the launch shapes of one block size on the jobs (persistent
+// launches into scratch accumulators); *best = the fastest.  The context's
+// tuning is left as it is.
+int search_jobs_shape_block(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, int block, rt_tuning *best);
+// RT_TILES_CENTRE: the tiles (16 x tile_h, row-major indices) of a width x
+// height image by distance of their centre from the image's centre, ties by index.
+void centre_tile_order(uint32_t width, uint32_t height, uint32_t tile_h, std::vector<uint32_t> &order);
 // Upper bound of render_job_bytes() (checked where the record is defined).
 constexpr size_t kRenderJobBytesMax = 512;
 

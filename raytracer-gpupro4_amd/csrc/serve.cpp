@@ -17,7 +17,9 @@
 // tiles are all done.  A pixel's samples depend only on (seed, frame, sample,
 // pixel), so a request's image is the same alone or in any mix.  In dry-run
 // mode no device work happens: a step's duration is its estimated rays /
-// dry_rate, which lets the scheduling be tested without a GPU.
+// dry_rate, which lets the scheduling be tested without a GPU.  With a batch
+// window, arrived requests wait (while nothing is in flight) up to the window
+// for later arrivals to share their launch.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -78,6 +80,13 @@ struct rt_server {
     uint64_t next_id = 1, tick = 0, used = 0;
     rt_server_stats st{};
     std::string err;
+    // launch shape (autotune): whole-image jobs of the first completed
+    // requests, searched once there are RT_SERVER_TUNE_REQUESTS of them
+    bool tuned = false;
+    rt_tuning tune{};
+    std::vector<rt_job_desc> tune_jobs;
+    std::vector<uint32_t> tune_keys;              // pool key of each job's scene
+    std::vector<std::vector<float>> tune_lights;  // each job's lights
 };
 
 namespace {
@@ -222,7 +231,9 @@ extern "C" {
 int rt_server_create(rt_ctx *ctx, const rt_server_config *cfg, rt_server **out)
 {
     if (!cfg || !out || (!ctx && !cfg->dry_run)) return RT_ERR_ARG;
-    if (cfg->max_inflight == 0 || cfg->tiles_per_step == 0 || (cfg->dry_run && !(cfg->dry_rate > 0.0)))
+    if (cfg->max_inflight == 0 || cfg->tiles_per_step == 0 || (cfg->dry_run && !(cfg->dry_rate > 0.0)) ||
+        !(cfg->batch_window >= 0.0) ||
+        (cfg->tile_order != RT_TILES_ROWS && cfg->tile_order != RT_TILES_CENTRE))
         return RT_ERR_ARG;
     rt_server *s = new rt_server();
     s->ctx = cfg->dry_run ? nullptr : ctx;
@@ -291,6 +302,21 @@ int rt_server_step(rt_server *s, double now, double *elapsed)
     if (s->ctx && (uint32_t)s->ctx->tune.block / 16 != s->tile_h)
         return fail(s, RT_ERR_STATE, "rt_server_step: the context's block size changed since rt_server_create");
     const Clock::time_point t0 = Clock::now();
+    s->st.wake = 0.0;
+    if (s->cfg.batch_window > 0.0 && s->inflight.empty()) {
+        // hold arrived requests up to batch_window so that later arrivals share
+        // their launch; a full batch starts at once
+        size_t arrived = 0;
+        double oldest = 0.0;
+        for (const Req &q : s->pending) {
+            if (q.r.arrival > now) break;
+            if (arrived++ == 0) oldest = q.r.arrival;
+        }
+        if (arrived > 0 && arrived < s->cfg.max_inflight && now < oldest + s->cfg.batch_window) {
+            s->st.wake = oldest + s->cfg.batch_window;
+            return RT_OK;
+        }
+    }
     // admission in arrival order; requests of a busy stream wait without
     // holding up the others, a request waiting for memory holds the queue
     for (auto it = s->pending.begin(); it != s->pending.end() && s->inflight.size() < s->cfg.max_inflight;) {
@@ -357,6 +383,7 @@ int rt_server_step(rt_server *s, double now, double *elapsed)
         j.accum = s->accums.at(q.key).dev;
         j.tile_begin = q.tiles_done;
         j.tile_count = share[k];
+        j.tile_order = s->cfg.tile_order;
         jobs.push_back(j);
         tiles_step += share[k];
         const uint64_t per_px = q.r.mode == RT_MODE_GI ? 1ull + q.r.max_bounces : 1ull + j.nlights;
@@ -370,13 +397,15 @@ int rt_server_step(rt_server *s, double now, double *elapsed)
         s->st.device_s += dt;
     } else {
         rt_frame_stats fs{};
+        const rt_tuning ctx_tune = s->ctx->tune;
+        if (s->tuned) s->ctx->tune = s->tune;  // the server's own shape for its launches
         const int rc = rt_render_jobs(s->ctx, jobs.data(), (uint32_t)jobs.size(), RT_FLAG_PERSISTENT, &fs);
+        s->ctx->tune = ctx_tune;
         if (rc != RT_OK) return fail(s, rc, rt_last_error(s->ctx));
         rays = fs.primary_rays + fs.shadow_rays + fs.bounce_rays;
         s->st.device_s += fs.trace_ms * 1e-3;
         dt = std::chrono::duration<double>(Clock::now() - t0).count();
     }
-    if (elapsed) *elapsed = dt;
     s->st.rays += rays;
     s->st.steps++;
     const double t_done = now + dt;
@@ -408,8 +437,55 @@ int rt_server_step(rt_server *s, double now, double *elapsed)
         res.status = RT_OK;
         s->done.push_back(res);
         s->st.completed++;
+        if (s->cfg.autotune && !s->tuned && !s->cfg.dry_run && s->tune_jobs.size() < RT_SERVER_TUNE_REQUESTS) {
+            rt_job_desc j{};
+            j.scene = p.handle;
+            j.cam = q.r.cam;
+            j.width = q.r.width;
+            j.height = q.r.height;
+            j.mode = q.r.mode;
+            j.spp = q.r.spp;
+            j.max_bounces = q.r.max_bounces;
+            j.albedo = q.r.albedo;
+            j.sky = q.r.sky;
+            j.ambient = q.r.ambient;
+            j.nlights = (uint32_t)(q.lights.size() / 3);
+            j.seed = q.r.seed;
+            j.frame = q.frame;
+            j.accum = a.dev;  // the search renders into its own scratch accumulators
+            j.tile_order = s->cfg.tile_order;
+            s->tune_jobs.push_back(j);
+            s->tune_keys.push_back(q.r.scene_key);
+            s->tune_lights.push_back(q.lights);
+        }
     }
     s->inflight.swap(keep);
+    if (s->cfg.autotune && !s->tuned && s->tune_jobs.size() >= RT_SERVER_TUNE_REQUESTS) {
+        // the collected requests whose scenes are still resident, whole images
+        std::vector<rt_job_desc> jobs_t;
+        for (size_t k = 0; k < s->tune_jobs.size(); ++k) {
+            const PoolScene &p = s->pool.at(s->tune_keys[k]);
+            if (!p.resident || p.handle != s->tune_jobs[k].scene) continue;
+            rt_job_desc j = s->tune_jobs[k];
+            j.lights = s->tune_lights[k].empty() ? nullptr : s->tune_lights[k].data();
+            jobs_t.push_back(j);
+        }
+        const Clock::time_point t1 = Clock::now();
+        rt_tuning best = s->ctx->tune;
+        const int rc = jobs_t.empty() ? RT_OK
+                                      : rt::search_jobs_shape_block(s->ctx, jobs_t.data(), (uint32_t)jobs_t.size(),
+                                                                    s->ctx->tune.block, &best);
+        if (rc != RT_OK) return fail(s, rc, rt_last_error(s->ctx));
+        const double tune_s = std::chrono::duration<double>(Clock::now() - t1).count();
+        s->st.tune_s += tune_s;
+        dt += tune_s;  // the search holds the device: the clock moves on
+        s->tune = best;
+        s->tuned = !jobs_t.empty();
+        s->tune_jobs.clear();
+        s->tune_keys.clear();
+        s->tune_lights.clear();
+    }
+    if (elapsed) *elapsed = dt;
     return RT_OK;
 }
 
@@ -427,6 +503,7 @@ int rt_server_status(rt_server *s, rt_server_stats *out)
 {
     if (!s || !out) return RT_ERR_ARG;
     *out = s->st;
+    if (s->ctx) out->tuning = s->tuned ? s->tune : s->ctx->tune;
     out->bytes_used = s->used;
     out->pending = (uint32_t)s->pending.size();
     out->inflight = (uint32_t)s->inflight.size();

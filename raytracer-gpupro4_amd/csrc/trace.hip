@@ -23,6 +23,7 @@
 // (trav_enter).  Frame kernels take their tiles heaviest first, from the
 // last frame's per-tile costs (k_tile_order), and direct frames may trace the
 // heaviest tiles with four lanes per pixel (quads, traverse_quad).
+#include <algorithm>
 #include <cstring>
 
 #include "rt_internal.h"
@@ -1816,6 +1817,7 @@ struct RenderJob {
     const TravTri *tris;
     const SceneHdr *hdr;
     float4 *accum;
+    const uint32_t *order;  // tile at each position of the job's queue; null: row-major
     rt_camera cam;
     uint32_t width, height, tiles_x, tile_off;
     uint32_t local_base, mode, spp, max_bounces, nlights, pad1;
@@ -1855,7 +1857,8 @@ __global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restri
             else hi = mid;
         }
         const RenderJob &J = jobs[lo];
-        const uint32_t lt = tile - J.tile_off + J.local_base;
+        const uint32_t rank = tile - J.tile_off + J.local_base;
+        const uint32_t lt = J.order ? J.order[rank] : rank;
         uint32_t ox_, oy_;
         Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
         const uint32_t px = (lt % J.tiles_x) * TW + ox_;
@@ -2208,7 +2211,7 @@ size_t whitted_queue_bytes(uint64_t pixels) { return 2 * sizeof(RayRec) * (size_
 uint32_t job_tile_pixels(const rt_tuning &t) { return (uint32_t)t.block; }
 
 void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_desc &d,
-                     const rt_tuning &t, uint32_t tile_off, uint32_t *tiles)
+                     const rt_tuning &t, uint32_t tile_off, uint32_t *tiles, const uint32_t *order)
 {
     RenderJob *j = static_cast<RenderJob *>(dst);
     memset(j, 0, sizeof(*j));
@@ -2218,6 +2221,7 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
     j->tris = sv.tris;
     j->hdr = sv.hdr;
     j->accum = reinterpret_cast<float4 *>(d.accum);
+    j->order = order;
     j->cam = device_camera(d.cam);
     j->width = d.width;
     j->height = d.height;
@@ -2239,6 +2243,21 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
 }
 
 static_assert(sizeof(RenderJob) <= kRenderJobBytesMax, "RenderJob outgrew the no-allocation job table");
+
+void centre_tile_order(uint32_t width, uint32_t height, uint32_t tile_h, std::vector<uint32_t> &order)
+{
+    const uint32_t tx = (width + 15) / 16, ty = (height + tile_h - 1) / tile_h;
+    std::vector<std::pair<uint64_t, uint32_t>> key(tx * ty);
+    for (uint32_t t = 0; t < tx * ty; ++t) {
+        // doubled pixel coordinates keep the tile and image centres integral
+        const int64_t cx = 2 * (int64_t)((t % tx) * 16) + 16 - (int64_t)width;
+        const int64_t cy = 2 * (int64_t)((t / tx) * tile_h) + tile_h - (int64_t)height;
+        key[t] = {(uint64_t)(cx * cx + cy * cy), t};
+    }
+    std::sort(key.begin(), key.end());
+    order.resize(key.size());
+    for (size_t k = 0; k < key.size(); ++k) order[k] = key[k].second;
+}
 size_t render_job_bytes() { return sizeof(RenderJob); }
 
 hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,

@@ -34,6 +34,9 @@ FLAG_START_AT_IROOT = 1 << 5
 MODE_DIRECT = 0
 MODE_GI = 1
 NOALLOC_MAX_JOBS = 1024
+TILES_ROWS = 0
+TILES_CENTRE = 1
+SERVER_TUNE_REQUESTS = 8
 CAMERA_BASIS = 0
 CAMERA_WORLD = 1
 WHITTED_SHADOWS = 1 << 0
@@ -130,12 +133,13 @@ class JobDesc(C.Structure):
                 ("albedo", C.c_float), ("sky", C.c_float), ("ambient", C.c_float),
                 ("lights", C.POINTER(C.c_float)), ("nlights", C.c_uint32), ("seed", C.c_uint64),
                 ("frame", C.c_uint32), ("accum", C.c_void_p), ("tile_begin", C.c_uint32),
-                ("tile_count", C.c_uint32)]
+                ("tile_count", C.c_uint32), ("tile_order", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("tiles_per_step", C.c_uint32),
-                ("dry_run", C.c_int32), ("dry_rate", C.c_double)]
+                ("dry_run", C.c_int32), ("dry_rate", C.c_double), ("autotune", C.c_int32),
+                ("tile_order", C.c_uint32), ("batch_window", C.c_double)]
 
 
 class Request(C.Structure):
@@ -159,10 +163,12 @@ class ServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "rejected", "steps", "builds",
                                           "evictions", "rays", "bytes_used")] + \
                [(k, C.c_uint32) for k in ("pending", "inflight", "resident_scenes", "accumulators")] + \
-               [("device_s", C.c_double)]
+               [("device_s", C.c_double), ("tune_s", C.c_double), ("tuning", Tuning), ("wake", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["tuning"] = self.tuning.as_tuple()
+        return d
 
 
 class FrameStats(C.Structure):
@@ -489,7 +495,7 @@ class Context:
 
     @staticmethod
     def job(scene, cam, width, height, accum, mode=MODE_GI, spp=1, max_bounces=4, albedo=0.7,
-            sky=1.0, ambient=0.1, lights=None, seed=1, frame=0, tile_begin=0, tile_count=0):
+            sky=1.0, ambient=0.1, lights=None, seed=1, frame=0, tile_begin=0, tile_count=0, tile_order=0):
         """Builds a JobDesc; keeps the light array alive on the object."""
         j = JobDesc()
         j.scene, j.cam, j.width, j.height = scene, cam, width, height
@@ -500,7 +506,7 @@ class Context:
         j.lights = arr.ctypes.data_as(C.POINTER(C.c_float)) if arr.size else None
         j.nlights = arr.size // 3
         j.seed, j.frame, j.accum = seed, frame, accum
-        j.tile_begin, j.tile_count = tile_begin, tile_count
+        j.tile_begin, j.tile_count, j.tile_order = tile_begin, tile_count, tile_order
         return j
 
     def render_jobs(self, jobs, persistent=True, want_stats=True):
@@ -730,10 +736,12 @@ class Server:
     ctx=None requires dry_run (scheduling only, no device work)."""
 
     def __init__(self, ctx: Context | None, hbm_budget: int, max_inflight: int = 16,
-                 tiles_per_step: int = 4096, dry_run: bool = False, dry_rate: float = 1e9):
+                 tiles_per_step: int = 4096, dry_run: bool = False, dry_rate: float = 1e9,
+                 autotune: bool = False, tile_order: int = TILES_ROWS, batch_window: float = 0.0):
         self.L = lib()
         self.ctx = ctx
-        cfg = ServerConfig(hbm_budget, max_inflight, tiles_per_step, 1 if dry_run else 0, dry_rate)
+        cfg = ServerConfig(hbm_budget, max_inflight, tiles_per_step, 1 if dry_run else 0, dry_rate,
+                           1 if autotune else 0, tile_order, batch_window)
         h = C.c_void_p()
         rc = self.L.rt_server_create(ctx.h if ctx else None, C.byref(cfg), C.byref(h))
         if rc != RT_OK:
@@ -832,6 +840,9 @@ class Server:
             if st["inflight"] == 0:
                 while k < len(pending_t) and pending_t[k] <= t:
                     k += 1
+                if st["wake"] > t:  # arrived requests held for the batch window
+                    t = min(st["wake"], pending_t[k]) if k < len(pending_t) else st["wake"]
+                    continue
                 if k == len(pending_t):
                     break  # nothing left that could arrive (rejected or waiting for memory)
                 t = pending_t[k]

@@ -276,6 +276,41 @@ def test_tile_ranges_compose_to_whole_image(ctx, rtcore, mix):
     ctx.free(acc)
 
 
+def test_centre_tile_order_keeps_the_image(ctx, rtcore, mix):
+    """RT_TILES_CENTRE hands a job's tiles out nearest the image centre first:
+    the same image as row order, whole or split into tile ranges, in one
+    persistent launch or one launch per job, and through the no-allocation
+    entry point (which reuses the order made here)."""
+    s, xyz, idx, ref, _ = mix
+    W, H = 70, 45
+    block = ctx.tuning()[0]
+    ntiles = ((W + 15) // 16) * ((H + block // 16 - 1) // (block // 16))
+    acc = ctx.alloc(16 * W * H)
+
+    def render(jobs, persistent, fn=None):
+        ctx.zero(acc, 16 * W * H)
+        (fn or ctx.render_jobs)(jobs, persistent=persistent, want_stats=False)
+        return ctx.read_accum(acc, W, H)
+
+    def job(order, b=0, c=0, mode=rtcore.MODE_GI):
+        return ctx.job(s, ref_cam(), W, H, acc, mode, 2, 3, lights=[tuple(xyz.mean(0) + [0, 4, 0])], seed=12,
+                       tile_begin=b, tile_count=c, tile_order=order)
+
+    def ref_cam():
+        c = xyz.mean(0)
+        return rtcore.camera_look_at(tuple(c + [0.5, 2.0, -7.0]), tuple(c), (0, 1, 0), 60.0, W / H)
+
+    for mode in (rtcore.MODE_GI, rtcore.MODE_DIRECT):
+        rows = render([job(rtcore.TILES_ROWS, mode=mode)], True)
+        for persistent in (True, False):
+            assert np.array_equal(render([job(rtcore.TILES_CENTRE, mode=mode)], persistent), rows)
+            cuts = [(0, 5), (5, ntiles // 2), (ntiles // 2, ntiles)]
+            parts = [job(rtcore.TILES_CENTRE, b, e - b, mode) for b, e in cuts]
+            assert np.array_equal(render(parts, persistent), rows), persistent
+            assert np.array_equal(render(parts, persistent, ctx.render_jobs_noalloc), rows)
+    ctx.free(acc)
+
+
 def test_autotune_direct_picks_and_caches(ctx, rtcore, mix, tmp_path):
     s, xyz, idx, ref, cam = mix
     cache = str(tmp_path / "tune_direct.txt")

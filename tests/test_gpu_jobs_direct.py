@@ -125,7 +125,8 @@ def test_served_stream_shape_matches_oracle(ctx, oracle, rtcore):
     pool = {k: rtcore.gen_object_mix(n, seed=100 + k) for k, n in ((1, 1500), (2, 60000), (3, 420000))}
     depth = {1: 14, 2: 19, 3: 22}
     refs = {k: oracle.build_scene(xyz, idx, depth[k]) for k, (xyz, idx) in pool.items()}
-    srv = rtcore.Server(ctx, 200 << 30, max_inflight=8, tiles_per_step=4096)
+    srv = rtcore.Server(ctx, 200 << 30, max_inflight=8, tiles_per_step=4096, autotune=True,
+                        tile_order=rtcore.TILES_CENTRE, batch_window=0.001)
     for k, (xyz, idx) in pool.items():
         srv.add_scene(k, xyz, idx, depth[k])
     # (scene, view, w, h, mode) per stream; every stream gets three passes
@@ -153,6 +154,9 @@ def test_served_stream_shape_matches_oracle(ctx, oracle, rtcore):
     assert len(res) == 12 and all(r["status"] == rtcore.RT_OK for r in res)
     st = srv.status()
     assert st["builds"] == 3 and st["steps"] >= 3
+    # the server tuned its launch shape on its first 8 requests; the last 4 ran with it
+    assert st["tune_s"] > 0 and st["tuning"][0] == ctx.tuning()[0]
+    print(f"served stream: server shape {st['tuning']} searched in {st['tune_s']:.2f} s", flush=True)
     for k, view, w, h, mode in streams:
         got, n = srv.read_stream(k, view, w, h, mode)
         assert n == 3

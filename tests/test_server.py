@@ -182,3 +182,37 @@ def test_argument_errors(rtcore, scenes):
         s.submit(1, 0, c, 16, 16, lights=np.zeros((9, 3)))
     with pytest.raises(rtcore.RtError):
         s.read_stream(1, 0, 16, 16, rtcore.MODE_GI)  # nothing rendered yet
+
+
+def test_batch_window_groups_arrivals(rtcore, scenes):
+    """With a batch window, an arrived request waits (while nothing is in
+    flight) until the window since the oldest arrival ends, so requests that
+    arrive within it share one launch; a full batch starts at once; without
+    a window each request starts when it arrives."""
+    c = cam(rtcore)
+    arrivals = [0.0, 0.0004, 0.0008, 0.010, 0.0102]
+    out = {}
+    for window in (0.0, 0.001):
+        s = rtcore.Server(None, 1 << 34, 8, 1 << 20, dry_run=True, dry_rate=1e12, batch_window=window)
+        for k, (xyz, idx) in scenes.items():
+            s.add_scene(k, xyz, idx, 18)
+        for k, t in enumerate(arrivals):
+            s.submit(1 + k % 3, k, c, 32, 32, rtcore.MODE_DIRECT, 1, 0, lights=[(0, 3, 0)], arrival=t)
+        res, _ = s.run(arrivals)
+        assert len(res) == 5 and all(r["status"] == rtcore.RT_OK for r in res)
+        out[window] = (s.status()["steps"], sorted((r["arrival"], r["admitted"]) for r in res))
+        s.close()
+    steps0, adm0 = out[0.0]
+    steps1, adm1 = out[0.001]
+    assert steps0 == 5 and all(a == t for t, a in adm0)  # each starts on arrival, alone
+    assert steps1 == 2  # {0, 0.4, 0.8 ms} then {10, 10.2 ms}
+    assert [a for _, a in adm1] == pytest.approx([0.001] * 3 + [0.011] * 2)
+    # a full batch (max_inflight arrived requests) does not wait
+    s = rtcore.Server(None, 1 << 34, 2, 1 << 20, dry_run=True, dry_rate=1e12, batch_window=0.5)
+    for k, (xyz, idx) in scenes.items():
+        s.add_scene(k, xyz, idx, 18)
+    for k in range(2):
+        s.submit(1, k, c, 32, 32, rtcore.MODE_DIRECT, 1, 0, lights=[(0, 3, 0)], arrival=0.0)
+    res, t_end = s.run([0.0, 0.0])
+    assert len(res) == 2 and all(r["admitted"] == 0.0 for r in res) and t_end < 0.5
+    s.close()
