@@ -75,8 +75,11 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "render_direct_soup262k_1080p_pmc.m
 SPHERE_PMC_SUMMARY = os.path.join(ROOT, "profiles", "render_direct_quad_sphere2m_1080p_pmc.md")
 
 
+_T0 = time.perf_counter()
+
+
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def depth_for(nt):
@@ -845,7 +848,7 @@ def workload_all(args, rtcore):
                       "identical_images": b["identical_images"], "tuning": b["tuning"],
                       "gi_rmse_vs_fp64": b["gi_rmse_vs_fp64"], "wall_s": round(time.perf_counter() - t0, 1)}
     t0 = time.perf_counter()
-    sv = workload_serve(types.SimpleNamespace(duration=60.0, plain_seconds=10.0, burst=2000), rtcore)
+    sv = workload_serve(types.SimpleNamespace(duration=60.0, plain_seconds=10.0, burst=500), rtcore)
     out["serve"] = {"workload": sv["config"]["workload"], "requests": sv["config"]["requests"],
                     "p50_ms": sv["p50_ms"], "p99_ms": sv["p99_ms"],
                     "served_mrays_s_device": sv["served"]["mrays_s_device"],
@@ -855,6 +858,7 @@ def workload_all(args, rtcore):
                     "no_window": {k: sv["served_no_window"][k] for k in ("p50_ms", "p99_ms", "mrays_s_device")},
                     "plain_p50_ms": sv["plain"]["p50_ms"], "plain_p99_ms": sv["plain"]["p99_ms"],
                     "device_busy_frac": sv["served"]["device_busy_frac"],
+                    "burst_requests": sv["burst"]["requests"],
                     "burst_speedup_device": sv["burst"]["speedup_device"],
                     "burst_speedup_clock": sv["burst"]["speedup_clock"], "clock": sv["clock"],
                     "wall_s": round(time.perf_counter() - t0, 1)}
