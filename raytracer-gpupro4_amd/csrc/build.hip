@@ -368,6 +368,7 @@ __global__ __launch_bounds__(kBoundsThreads) void k_morton(
 // scene of the id, global base = the scene's first sorted position) groups the
 // scenes of a batch after the four code passes.
 // ---------------------------------------------------------------------------
+constexpr int kLookBack = 4;  // status words a look-back step loads at once
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
@@ -453,18 +454,35 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         __hip_atomic_store(my, kFlagInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
         __hip_atomic_store(my, kFlagAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Look back kLookBack tiles per round trip: their status words are
+        // loaded together, then taken newest first until an inclusive prefix
+        // (tile 0 always publishes one) or a tile not ready yet, which is
+        // loaded again from there.
         int32_t j = (int32_t)tile - 1;
         while (j >= 0) {
-            const uint32_t sv = __hip_atomic_load(status + (size_t)j * kRadix + dg, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t flag = sv & ~kValMask;
-            if (flag == 0u) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+            uint32_t sv[kLookBack];
+#pragma unroll
+            for (int u = 0; u < kLookBack; ++u)
+                sv[u] = j - u >= 0 ? __hip_atomic_load(status + (size_t)(j - u) * kRadix + dg, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                                   : 0u;
+            int taken = 0;
+            bool done = false;
+#pragma unroll
+            for (int u = 0; u < kLookBack; ++u) {
+                if (j - u < 0) break;
+                const uint32_t flag = sv[u] & ~kValMask;
+                if (flag == 0u) break;
+                prefix += sv[u] & kValMask;
+                taken = u + 1;
+                if (flag == kFlagInc) {
+                    done = true;
+                    break;
+                }
             }
-            prefix += sv & kValMask;
-            if (flag == kFlagInc) break;
-            --j;
+            if (done) break;
+            if (taken == 0) __builtin_amdgcn_s_sleep(1);
+            j -= taken;
         }
         __hip_atomic_store(my, kFlagInc | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
