@@ -88,7 +88,7 @@ static void *whitted_worker(void *arg)
 {
     wjob *j = (wjob *)arg;
     const orc_whitted_params *p = j->p;
-    const int shadows = (p->switches & 1u) != 0, phong = (p->switches & 2u) != 0;
+    const int shadows = (p->switches & 1u) != 0, phong = (p->switches & 2u) != 0 && j->normals != NULL;
     const float gloss = (p->switches & 4u) ? 1.0f : 0.0f;
     for (uint32_t pix = j->begin; pix < j->end; ++pix) {
         orc_ray ray = j->prim[pix];
