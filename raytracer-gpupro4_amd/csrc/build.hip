@@ -656,20 +656,27 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
 // ---------------------------------------------------------------------------
 // Stage 4c: small subtrees.  A persistent block takes one entry of the
 // subtree list at a time.  A leaf (of any size) is reduced by the whole block;
-// an internal node (<= kSubtreeMax primitives) has its codes staged in LDS,
-// its subtree emitted level by level with the same rules as k_top, then
+// an internal node (<= kSubtreeMax primitives) has its codes staged in LDS and
+// its subtree emitted level by level with the same rules as k_top, in LDS
+// only; then every leaf triangle of the subtree is gathered at once (one
+// round of memory latency instead of one per level), and the subtree is
 // refitted bottom-up (LBVH_BuildAABB, LBVH.hlsl:228-270) from LDS child boxes.
 // ---------------------------------------------------------------------------
 struct SubtreeLds {
     uint32_t codes[kSubtreeMax];
-    int4 fr[2][kSubtreeMax];
-    int4 inode[kSubtreeMax];      // k, heap index, parent, first
-    int32_t iword[kSubtreeMax];   // (count << 2) | axis
-    float cbox[kSubtreeMax][12];  // child boxes by local internal index
+    union {
+        int4 fr[2][kSubtreeMax];        // emission frontiers
+        float pbox[kSubtreeMax][6];     // then: each primitive's box, by local position
+    };
+    int4 inode[kSubtreeMax];            // k, heap index, parent, first
+    int32_t iword[kSubtreeMax];         // (count << 2) | axis
+    float cbox[kSubtreeMax][12];        // child boxes by local internal index
     int32_t cref[kSubtreeMax][2];
+    int4 leaf[kSubtreeMax];             // heap index, local first, (count << 2) | axis, parent
+    uint16_t pos_leaf[kSubtreeMax];     // the leaf of each local position
     uint32_t lvl[kMaxLevels + 2];
     float red[kSubtreeThreads / 64 * 6];
-    uint32_t item, n, next, nint;
+    uint32_t item, n, next, nint, nleaf;
 };
 
 __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
@@ -733,13 +740,15 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
             __syncthreads();
             continue;
         }
-        // internal subtree root: rcnt <= kSubtreeMax
+        // internal subtree root: rcnt <= kSubtreeMax.  The topology is emitted
+        // level by level from the codes in LDS; leaves are only listed here.
         for (int32_t j = tid; j < rcnt; j += kSubtreeThreads) L.codes[j] = codes[rfirst + j];
         if (tid == 0) {
             L.fr[0][0] = root;
             L.n = 1;
             L.next = 0;
             L.nint = 0;
+            L.nleaf = 0;
             L.lvl[0] = 0;
         }
         __syncthreads();
@@ -756,23 +765,9 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
                 const int pp = pr < 0 ? 31 : (pr & 31);
                 int axis = 0, p = 0;
                 if (classify(h, cnt, pp, leaf_level, L.codes[lf], L.codes[ll], &axis, &p)) {
-                    float lo[3] = {INFINITY, INFINITY, INFINITY};
-                    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-                    for (int32_t j = 0; j < cnt; ++j)
-                        leaf_prim(vals, trec, tris, first + j, j, cnt, h, lo, hi);
-                    write_heap(&sheap[h], lo, hi, (cnt << 3) | 4 | axis, (int32_t)(first - D.prim_off));
-                    if (h < leaf_level) {
-                        write_heap_word(&sheap[2 * h], kInvalid, kInvalid);
-                        write_heap_word(&sheap[2 * h + 1], kInvalid, kInvalid);
-                    }
-                    // the root of this subtree is internal, so a leaf's parent is local
-                    const uint32_t ploc = (uint32_t)(pr >> 6) - rfirst;
-                    const int side = (pr >> 5) & 1;
-                    for (int k = 0; k < 3; ++k) {
-                        L.cbox[ploc][side * 6 + k] = lo[k];
-                        L.cbox[ploc][side * 6 + 3 + k] = hi[k];
-                    }
-                    L.cref[ploc][side] = (int32_t)leaf_ref(first, (uint32_t)cnt);
+                    const uint32_t slot = atomicAdd(&L.nleaf, 1u);
+                    L.leaf[slot] = make_int4((int)h, (int)lf, (cnt << 2) | axis, pr);
+                    for (int32_t j = 0; j < cnt; ++j) L.pos_leaf[lf + j] = (uint16_t)slot;
                     continue;
                 }
                 const uint32_t X = ((L.codes[lf] >> p) | 1u) << p;
@@ -808,6 +803,50 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
             ++level;
             __syncthreads();
         }
+        // every leaf triangle at once: one thread per position gathers its
+        // record, writes its sorted-order triangle and keeps its box in LDS
+        for (int32_t p = tid; p < rcnt; p += kSubtreeThreads) {
+            const int4 lf = L.leaf[L.pos_leaf[p]];
+            float lo[3] = {INFINITY, INFINITY, INFINITY};
+            float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            leaf_prim(vals, trec, tris, rfirst + p, p - lf.y, lf.z >> 2, (uint32_t)lf.x, lo, hi);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                L.pbox[p][c] = lo[c];
+                L.pbox[p][3 + c] = hi[c];
+            }
+        }
+        __syncthreads();
+        // leaf boxes (exact min / max, any order), heap words and parent slots
+        for (uint32_t i = tid; i < L.nleaf; i += kSubtreeThreads) {
+            const int4 lf = L.leaf[i];
+            const uint32_t h = (uint32_t)lf.x, first = rfirst + (uint32_t)lf.y;
+            const int32_t cnt = lf.z >> 2, axis = lf.z & 3, pr = lf.w;
+            float lo[3] = {INFINITY, INFINITY, INFINITY};
+            float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int32_t j = 0; j < cnt; ++j) {
+                const float *b = L.pbox[lf.y + j];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    lo[c] = fminf(lo[c], b[c]);
+                    hi[c] = fmaxf(hi[c], b[3 + c]);
+                }
+            }
+            write_heap(&sheap[h], lo, hi, (cnt << 3) | 4 | axis, (int32_t)(first - D.prim_off));
+            if (h < leaf_level) {
+                write_heap_word(&sheap[2 * h], kInvalid, kInvalid);
+                write_heap_word(&sheap[2 * h + 1], kInvalid, kInvalid);
+            }
+            // the root of this subtree is internal, so a leaf's parent is local
+            const uint32_t ploc = (uint32_t)(pr >> 6) - rfirst;
+            const int side = (pr >> 5) & 1;
+            for (int c = 0; c < 3; ++c) {
+                L.cbox[ploc][side * 6 + c] = lo[c];
+                L.cbox[ploc][side * 6 + 3 + c] = hi[c];
+            }
+            L.cref[ploc][side] = (int32_t)leaf_ref(first, (uint32_t)cnt);
+        }
+        __syncthreads();
         // refit bottom-up; level l holds inode[lvl[l] .. lvl[l+1])
         for (int32_t l = (int32_t)level - 1; l >= 0; --l) {
             for (uint32_t i = L.lvl[l] + tid; i < L.lvl[l + 1]; i += kSubtreeThreads) {
