@@ -567,6 +567,86 @@ __device__ __forceinline__ uint32_t split_search(const uint32_t *__restrict__ co
 // parent} with parent = -1 for the root, else (k << 6) | (side << 5) | split
 // bit of the parent.  Leaves and small nodes go to the subtree list.
 // ---------------------------------------------------------------------------
+// One big node (more than kSubtreeMax primitives) of the top levels: a leaf
+// or a node too small for this stage goes to the subtree list; otherwise the
+// node's heap word and traversal node are written, it goes to the tops (for
+// the refit), and its children go to the next frontier or to the subtree
+// list.  The caller appends the plan's entries to those lists.
+struct TopPlan {
+    int4 c0, c1, rec;    // leaf: c0 = the node itself; internal: the children and the tops record
+    bool valid = false;  // a node was planned
+    bool leaf = false;
+    bool big0 = false, big1 = false;  // internal: child goes to the next frontier
+    __device__ uint32_t nsmall() const { return !valid ? 0u : leaf ? 1u : (uint32_t)!big0 + (uint32_t)!big1; }
+    __device__ uint32_t nnext() const { return valid && !leaf ? (uint32_t)big0 + (uint32_t)big1 : 0u; }
+    __device__ uint32_t ntop() const { return valid && !leaf ? 1u : 0u; }
+    // writes the plan's entries from slots small[bs..] and nxt[bn..], children in order
+    __device__ void write_lists(int4 *small, uint32_t bs, int4 *nxt, uint32_t bn) const
+    {
+        if (!valid) return;
+        if (leaf) {
+            small[bs] = c0;
+            return;
+        }
+        if (big0) nxt[bn++] = c0;
+        else small[bs++] = c0;
+        if (big1) nxt[bn] = c1;
+        else small[bs] = c1;
+    }
+};
+
+__device__ __forceinline__ void top_plan(const int4 e, const SceneDesc &D, uint32_t s, uint32_t leaf_level,
+                                         const uint32_t *__restrict__ codes, const uint32_t *__restrict__ table,
+                                         rt_node32 *__restrict__ sheap, TravNode *__restrict__ trav,
+                                         SceneHdr *__restrict__ hdr, TopPlan &t)
+{
+    t.valid = true;
+    const uint32_t h = (uint32_t)e.x, first = (uint32_t)e.y;
+    const int32_t cnt = e.z, pr = e.w;
+    const uint32_t last = first + cnt - 1;
+    const int pp = pr < 0 ? 31 : (pr & 31);
+    const uint32_t cf = codes[first], cl = codes[last];
+    int axis = 0, p = 0;
+    if (classify(h, cnt, pp, leaf_level, cf, cl, &axis, &p)) {
+        t.leaf = true;
+        t.c0 = e;
+        return;
+    }
+    const uint32_t X = ((cf >> p) | 1u) << p;
+    const uint32_t sp = split_search(codes, table, D, first, last, X);
+    const int32_t k = (int32_t)sp - 1;
+    write_heap_word(&sheap[h], (cnt << 3) | axis, (int32_t)(first - D.prim_off));
+    TravNode *tn = &trav[k];
+    tn->axis = axis;
+    tn->heap = (int32_t)h;
+    if (pr < 0) hdr[s].root_ref = k;
+    else trav[pr >> 6].ref[(pr >> 5) & 1] = k;
+    t.rec = make_int4(k, (int)h, pr, (int)first);
+    const int32_t cl_cnt = (int32_t)(sp - first), cr_cnt = (int32_t)(last - sp + 1);
+    t.c0 = make_int4((int)(2 * h), (int)first, cl_cnt, (k << 6) | p);
+    t.c1 = make_int4((int)(2 * h + 1), (int)sp, cr_cnt, (k << 6) | (1 << 5) | p);
+    t.big0 = cl_cnt > kSubtreeMax;
+    t.big1 = cr_cnt > kSubtreeMax;
+}
+
+// Reserves cnt slots of a global list for each lane of the wave with one
+// atomic: returns the lane's first slot.  Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *counter, uint32_t cnt)
+{
+    const int lane = __lane_id();
+    uint32_t x = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    const uint32_t total = __shfl(x, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + x - cnt;
+}
+
 __global__ __launch_bounds__(kTopThreads) void k_top(
     const SceneDesc *__restrict__ desc, SceneHdr *__restrict__ hdr, const uint32_t *__restrict__ codes,
     const uint32_t *__restrict__ table, int4 *__restrict__ big_a, int4 *__restrict__ big_b,
@@ -651,6 +731,72 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
         __syncthreads();
     }
     if (tid == 0) hdr[s].levels = level;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 4b': the same emission for one big scene (a batch of one scene of at
+// least kTopWideMinN primitives), one launch per level over the whole chip
+// instead of one block: launch l processes the frontier of level l (level 0:
+// the root), whose size the launch before counted in counters[kCntFront + l];
+// the kernel boundaries order the levels.  Tops of level l land after those
+// of the levels above (top_counts[l] counts them), as k_top lays them out.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kTopLevelThreads) void k_top_level(
+    const SceneDesc *__restrict__ desc, SceneHdr *__restrict__ hdr, const uint32_t *__restrict__ codes,
+    const uint32_t *__restrict__ table, int4 *__restrict__ big_a, int4 *__restrict__ big_b,
+    int4 *__restrict__ small, uint32_t *__restrict__ counters, int4 *__restrict__ top_nodes,
+    uint32_t *__restrict__ top_counts, rt_node32 *__restrict__ heap, TravNode *__restrict__ trav,
+    uint32_t level)
+{
+    __shared__ uint32_t s_base;
+    const SceneDesc D = desc[0];
+    uint32_t *front = counters + kCntFront;
+    const int tid = threadIdx.x;
+    if (level == 0) {  // one block: clear the level counts, then the root
+        for (int l = tid; l < kMaxLevels; l += kTopLevelThreads) top_counts[l] = 0u;
+        __syncthreads();
+        if (tid == 0) {
+            hdr[0].levels = 1;
+            TopPlan t;
+            top_plan(make_int4(1, (int)D.prim_off, (int)D.nt, -1), D, 0, 1u << (D.depth - 1), codes, table,
+                     heap + D.heap_off, trav, hdr, t);
+            if (t.leaf) {
+                small[atomicAdd(&counters[kCntSmall], 1u)] = t.c0;
+            } else {
+                if (t.big0) big_b[D.prim_off + atomicAdd(&front[1], 1u)] = t.c0;
+                else small[atomicAdd(&counters[kCntSmall], 1u)] = t.c0;
+                if (t.big1) big_b[D.prim_off + atomicAdd(&front[1], 1u)] = t.c1;
+                else small[atomicAdd(&counters[kCntSmall], 1u)] = t.c1;
+                top_nodes[D.prim_off + atomicAdd(&top_counts[0], 1u)] = t.rec;
+            }
+        }
+        return;
+    }
+    const uint32_t n = front[level];
+    if (n == 0) return;
+    if (tid == 0) {
+        uint32_t o = 0;
+        for (uint32_t l = 0; l < level; ++l) o += top_counts[l];
+        s_base = o;
+        if (blockIdx.x == 0) hdr[0].levels = level + 1;
+    }
+    __syncthreads();
+    const int4 *cur = ((level & 1) ? big_b : big_a) + D.prim_off;
+    int4 *nxt = ((level & 1) ? big_a : big_b) + D.prim_off;
+    int4 *tops = top_nodes + D.prim_off + s_base;
+    // wave-uniform loop: every lane takes part in the list reservations
+    const uint32_t lane = (uint32_t)__lane_id();
+    for (uint32_t i0 = blockIdx.x * kTopLevelThreads + (uint32_t)(tid & ~63); i0 < n;
+         i0 += gridDim.x * kTopLevelThreads) {
+        TopPlan t;
+        if (i0 + lane < n)
+            top_plan(cur[i0 + lane], D, 0, 1u << (D.depth - 1), codes, table, heap + D.heap_off, trav, hdr, t);
+        const uint32_t bs = wave_reserve(&counters[kCntSmall], t.nsmall());
+        const uint32_t bn = wave_reserve(&front[level + 1], t.nnext());
+        const uint32_t bt = wave_reserve(&top_counts[level], t.ntop());
+        t.write_lists(small, bs, nxt, bn);
+        if (t.ntop()) tops[bt] = t.rec;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -955,6 +1101,56 @@ __global__ __launch_bounds__(kTopThreads) void k_top_refit(
     }
 }
 
+// Stage 5': the refit of one big scene's top levels, one launch per level
+// (deepest first) over the whole chip; launches past the scene's levels exit.
+__global__ __launch_bounds__(kTopLevelThreads) void k_top_refit_level(
+    SceneHdr *__restrict__ hdr, const SceneDesc *__restrict__ desc, const int4 *__restrict__ top_nodes,
+    const uint32_t *__restrict__ top_counts, rt_node32 *__restrict__ heap, TravNode *trav, uint32_t level)
+{
+    __shared__ uint32_t s_base;
+    if (level >= hdr[0].levels) return;
+    const uint32_t n = top_counts[level];
+    if (n == 0) return;
+    const SceneDesc D = desc[0];
+    if (threadIdx.x == 0) {
+        uint32_t o = 0;
+        for (uint32_t l = 0; l < level; ++l) o += top_counts[l];
+        s_base = o;
+    }
+    __syncthreads();
+    const int4 *tops = top_nodes + D.prim_off + s_base;
+    rt_node32 *sheap = heap + D.heap_off;
+    for (uint32_t i = blockIdx.x * kTopLevelThreads + threadIdx.x; i < n; i += gridDim.x * kTopLevelThreads) {
+        const int4 nd = tops[i];
+        const int32_t k = nd.x, pr = nd.z;
+        const TravNode *tn = &trav[k];
+        float lo[3], hi[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = fminf(tn->box[c], tn->box[6 + c]);
+            hi[c] = fmaxf(tn->box[3 + c], tn->box[9 + c]);
+        }
+        rt_node32 *hn = &sheap[nd.y];
+        for (int c = 0; c < 3; ++c) {
+            hn->mn[c] = lo[c];
+            hn->mx[c] = hi[c];
+        }
+        if (pr < 0) {
+            for (int c = 0; c < 3; ++c) {
+                hdr[0].root_box[c] = lo[c];
+                hdr[0].root_box[3 + c] = hi[c];
+            }
+        } else {
+            TravNode *pn = &trav[pr >> 6];
+            const int side = (pr >> 5) & 1;
+            for (int c = 0; c < 3; ++c) {
+                pn->box[side * 6 + c] = lo[c];
+                pn->box[side * 6 + 3 + c] = hi[c];
+            }
+        }
+    }
+}
+
 __global__ void k_transform(const float *__restrict__ rest, float *__restrict__ xyz, uint32_t nv,
                             Xform x)
 {
@@ -1047,9 +1243,24 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
         hipLaunchKernelGGL(k_split_table, dim3(blocks), dim3(256), 0, st, b.d_desc, S, b.total_table,
                            b.keys[0], b.table);
     }
-    hipLaunchKernelGGL(k_top, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.keys[0], b.table,
-                       b.big[0], b.big[1], b.small, b.counters, b.top_nodes, b.top_counts, b.heap,
-                       b.trav);
+    // one big scene: the top levels over the whole chip, one launch per level
+    const bool wide_top = S == 1 && n >= kTopWideMinN;
+    const uint32_t depth = S == 1 ? b.desc[0].depth : 0u;
+    auto level_grid = [&](uint32_t l) {  // at most 2^l nodes, and at most n / kSubtreeMax
+        const uint64_t most = std::min<uint64_t>(l < 40 ? (1ull << l) : ~0ull, (uint64_t)n / kSubtreeMax + 1);
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((most + kTopLevelThreads - 1) / kTopLevelThreads,
+                                                                   4ull * (uint32_t)cu_count));
+    };
+    if (wide_top) {
+        for (uint32_t l = 0; l < depth && l < (uint32_t)kMaxLevels; ++l)
+            hipLaunchKernelGGL(k_top_level, dim3(l == 0 ? 1u : level_grid(l)), dim3(kTopLevelThreads), 0, st,
+                               b.d_desc, b.hdr, b.keys[0], b.table, b.big[0], b.big[1], b.small, b.counters,
+                               b.top_nodes, b.top_counts, b.heap, b.trav, l);
+    } else {
+        hipLaunchKernelGGL(k_top, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.keys[0], b.table,
+                           b.big[0], b.big[1], b.small, b.counters, b.top_nodes, b.top_counts, b.heap,
+                           b.trav);
+    }
     if (ev) hipEventRecord(ev[4], st);
     if (n > 0) {
         const uint32_t blocks = min_u32(2u * (uint32_t)cu_count, (n + 63) / 64);
@@ -1057,8 +1268,14 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
                            b.d_desc, S, b.hdr, b.keys[0], b.vals[0], b.trec, b.small,
                            b.counters, b.heap, b.trav, b.tris);
     }
-    hipLaunchKernelGGL(k_top_refit, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.top_nodes,
-                       b.top_counts, b.heap, b.trav);
+    if (wide_top) {
+        for (int32_t l = (int32_t)std::min<uint32_t>(depth, kMaxLevels) - 1; l >= 0; --l)
+            hipLaunchKernelGGL(k_top_refit_level, dim3(level_grid((uint32_t)l)), dim3(kTopLevelThreads), 0, st,
+                               b.hdr, b.d_desc, b.top_nodes, b.top_counts, b.heap, b.trav, (uint32_t)l);
+    } else {
+        hipLaunchKernelGGL(k_top_refit, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.top_nodes,
+                           b.top_counts, b.heap, b.trav);
+    }
     if (ev) hipEventRecord(ev[5], st);
     return hipGetLastError();
 }

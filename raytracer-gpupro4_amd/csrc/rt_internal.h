@@ -39,6 +39,11 @@ constexpr int kBoundsTile = kBoundsThreads * kBoundsItems;
 constexpr int kSubtreeMax = 512;
 constexpr int kSubtreeThreads = 256;
 constexpr int kTopThreads = 1024;
+// A batch of one scene from this many primitives emits and refits its top
+// levels with one launch per level over the whole chip (k_top_level,
+// k_top_refit_level) instead of one block.
+constexpr uint32_t kTopWideMinN = 1u << 22;
+constexpr int kTopLevelThreads = 256;
 constexpr int kMaxLevels = 32;
 
 // Leaf references in traversal nodes carry bit 31; bits 28-30 hold the leaf's
@@ -122,7 +127,8 @@ struct SceneHdr {
 constexpr int kCntSmall = 0;
 constexpr int kCntWork = 1;
 constexpr int kCntTiles = 2;
-constexpr int kCounters = 16;
+constexpr int kCntFront = 8;  // frontier size of each top level (k_top_level)
+constexpr int kCounters = kCntFront + 40;
 
 // SAH BVH of one scene as built on the host (LinearBVHNode order) plus its
 // depth; see bvh.cpp.

@@ -106,6 +106,31 @@ def test_build_sphere_2m(ctx, oracle, rtcore):
     check_build(ctx, oracle, xyz, idx, 22)
 
 
+@pytest.mark.parametrize("kind", ["soup", "dup_grid"])
+def test_build_one_big_scene_top_levels_per_launch(ctx, oracle, rtcore, kind):
+    """A batch of one scene of at least 4,194,304 triangles emits and refits
+    its top levels with one launch per level over the whole chip: the heap
+    words, boxes, codes and ids equal the oracle's, and rays traced through
+    the scene (traversal nodes refitted by those launches) equal the
+    oracle's trace on its heap.  dup_grid: clustered centroids, so the big
+    nodes form long unbalanced chains and leaves at the depth limit."""
+    n = (1 << 22) + 12345
+    xyz, idx = soup(n, 0.002, 9) if kind == "soup" else dup_grid(n, cells=24, seed=3)
+    check_build(ctx, oracle, xyz, idx, 23)
+    s = ctx.add_scene(xyz, idx, 23)
+    ctx.build(s)
+    ref = oracle.build_scene(xyz, idx, 23)
+    lo, hi = xyz.min(0), xyz.max(0)
+    c = (lo + hi) / 2
+    cam = rtcore.camera_look_at(tuple(c + [0.3, 0.4, -1.6]), tuple(c), (0, 1, 0), 50.0, 1.0)
+    rays = oracle.raygen_pinhole(cam.as_array(), 96, 96)
+    got = ctx.trace(s, rays)
+    want = oracle.trace(ref["nodes"], ref["ids"], xyz, idx, rays, nthreads=16)
+    assert (want["tri"] >= 0).mean() > 0.2
+    assert np.array_equal(got, want)
+    ctx.scene_destroy(s)
+
+
 def test_empty_scene(ctx):
     s = ctx.scene_create(16, 16, 8)
     ctx.scene_upload(s, np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
