@@ -226,7 +226,9 @@ int rt_scene_info(rt_ctx *ctx, int scene, uint32_t *nt, uint32_t *nv,
 /* ---- build (StructureConstruction::GPUBuildStructure, StructureConstruction.cpp:86-131) */
 /* Rebuilds the LBVH of the scene's current vertices (and of every other scene
  * of its batch) on the context stream.
- * times != NULL synchronises and fills the per-stage device times. */
+ * times != NULL synchronises and fills the per-stage device times; the event
+ * at each stage boundary stalls the stream (about 5 us each on gfx950), so
+ * total_ms is then a little above an unobserved build's time. */
 int rt_lbvh_build(rt_ctx *ctx, int scene, rt_build_times *times);
 /* Copies build results to the host; any pointer may be NULL.
  * nodes: 1 << depth entries (AccelerationStructure::GetPtrToElements);
@@ -399,7 +401,10 @@ const char *rt_image_last_error(void);
  *      reset the counters, so do not pass them inside a window. */
 typedef struct rt_profile {
     uint32_t builds;     /* rt_lbvh_build calls in the window */
-    float build_ms[6];   /* summed stage times, as rt_build_times */
+    float build_ms[6];   /* [5]: summed end-to-end build times; [0..4] are 0: a
+                            window records no stage events, which would each
+                            stall the stream between two kernels (rt_lbvh_build
+                            with rt_build_times gives the stages) */
     uint32_t launches;   /* trace / render kernel launches in the window */
     float trace_ms;      /* summed device time of those launches */
     uint64_t primary_rays, shadow_rays, bounce_rays, box_tests, tri_tests;

@@ -4,7 +4,8 @@
 // Reference stages (StructureConstruction::GPUBuildStructure,
 // RayTracerCS/StructureConstruction.cpp:86-131):
 //   LBVH_PreBuild   (LBVH.hlsl:15-132)       -> k_bounds + k_morton
-//   BitonicSort     (BitonicSort.cpp:56-117) -> k_onesweep x 4 (+1 scene pass)
+//   BitonicSort     (BitonicSort.cpp:56-117) -> k_sort_scatter x 4 (+1 scene pass),
+//                                               with k_sort_count + k_sort_scan from 1M keys
 //   LBVH_Build      (LBVH.hlsl:134-226)      -> k_split_table + k_top + k_subtree
 //   LBVH_BuildAABB  (LBVH.hlsl:228-270)      -> k_subtree + k_top_refit
 // Node words follow the CPU builder (LBVH.cpp:190-314), including its axis
@@ -248,6 +249,8 @@ __global__ __launch_bounds__(kBoundsThreads) void k_bounds(
 // few KB each, no extra launch); larger ones would make that O(blocks^2), so
 // one block of this kernel per scene reduces them once into the scene header.
 constexpr uint32_t kBoxInline = 512;
+// k_morton blocks per CU (each walks several bounds tiles)
+constexpr uint32_t kMortonBlocksPerCu = 4;
 
 __global__ __launch_bounds__(1024) void k_scene_box(const SceneDesc *__restrict__ desc,
                                                     const float *__restrict__ partials,
@@ -289,68 +292,78 @@ __global__ __launch_bounds__(1024) void k_scene_box(const SceneDesc *__restrict_
     }
 }
 
+// Each block walks the bounds tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+// and keeps its digit histograms in LDS across them, so the batch-wide
+// histograms take one round of global atomics per block, not per tile (at
+// 33.5M triangles: about 1M atomics on 1,024 words instead of 27M).
 __global__ __launch_bounds__(kBoundsThreads) void k_morton(
-    const SceneDesc *__restrict__ desc, uint32_t S, const float *__restrict__ cent,
+    const SceneDesc *__restrict__ desc, uint32_t S, uint32_t total_blocks, const float *__restrict__ cent,
     const float *__restrict__ partials, uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
     uint32_t *__restrict__ hist, SceneHdr *__restrict__ hdr)
 {
     __shared__ float red[24];
     __shared__ uint32_t lhist[kCodePasses * kRadix];
-    const uint32_t s = upper_scene(S, blockIdx.x, [&](uint32_t i) { return desc[i].block_off; });
-    const SceneDesc D = desc[s];
-    float lo[3] = {INFINITY, INFINITY, INFINITY};
-    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const uint32_t lb = blockIdx.x - D.block_off;
     for (int j = threadIdx.x; j < kCodePasses * kRadix; j += kBoundsThreads) lhist[j] = 0;
-    if (D.nblocks <= kBoxInline) {
-        for (uint32_t j = threadIdx.x; j < D.nblocks; j += kBoundsThreads) {
-            const float *p = partials + 6 * ((size_t)D.block_off + j);
+    __syncthreads();  // lhist cleared before the atomics below
+    uint32_t cur = 0xFFFFFFFFu;  // scene whose box m, f hold (block-uniform)
+    float m[3] = {0.0f, 0.0f, 0.0f}, f[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t blk = blockIdx.x; blk < total_blocks; blk += gridDim.x) {
+        const uint32_t s = upper_scene(S, blk, [&](uint32_t i) { return desc[i].block_off; });
+        const SceneDesc D = desc[s];
+        const uint32_t lb = blk - D.block_off;
+        if (s != cur) {
+            cur = s;
+            float lo[3] = {INFINITY, INFINITY, INFINITY};
+            float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            if (D.nblocks <= kBoxInline) {
+                for (uint32_t j = threadIdx.x; j < D.nblocks; j += kBoundsThreads) {
+                    const float *p = partials + 6 * ((size_t)D.block_off + j);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        lo[k] = fminf(lo[k], p[k]);
+                        hi[k] = fmaxf(hi[k], p[3 + k]);
+                    }
+                }
+                block_box_reduce<4>(lo, hi, red);
+                if (lb == 0 && threadIdx.x == 0) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        hdr[s].scene_box[k] = lo[k];
+                        hdr[s].scene_box[3 + k] = hi[k];
+                    }
+                }
+            } else {  // k_scene_box reduced them
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = hdr[s].scene_box[k];
+                    hi[k] = hdr[s].scene_box[3 + k];
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                lo[k] = fminf(lo[k], p[k]);
-                hi[k] = fmaxf(hi[k], p[3 + k]);
+                const float hk = 3.0f * hi[k];
+                const float lk = 3.0f * lo[k];
+                const float ext = hk - lk;
+                m[k] = lk;
+                f[k] = 1024.0f / ext;
             }
         }
-        block_box_reduce<4>(lo, hi, red);
-        if (lb == 0 && threadIdx.x == 0) {
+        const uint32_t base = lb * (uint32_t)kBoundsTile;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                hdr[s].scene_box[k] = lo[k];
-                hdr[s].scene_box[3 + k] = hi[k];
+        for (int r = 0; r < kBoundsItems; ++r) {
+            const uint32_t i = base + r * kBoundsThreads + threadIdx.x;
+            if (i < D.nt) {
+                const size_t g = (size_t)D.prim_off + i;
+                const uint32_t qx = quantise(cent[3 * g + 0], m[0], f[0]);
+                const uint32_t qy = quantise(cent[3 * g + 1], m[1], f[1]);
+                const uint32_t qz = quantise(cent[3 * g + 2], m[2], f[2]);
+                const uint32_t code = (part1by2(qx) << 2) | (part1by2(qy) << 1) | part1by2(qz);
+                keys[g] = code;
+                vals[g] = (int32_t)g;
+#pragma unroll
+                for (int p = 0; p < kCodePasses; ++p)
+                    atomicAdd(&lhist[p * kRadix + ((code >> (p * kRadixBits)) & (kRadix - 1))], 1u);
             }
-        }
-    } else {  // k_scene_box reduced them
-        __syncthreads();  // lhist cleared before the atomics below
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = hdr[s].scene_box[k];
-            hi[k] = hdr[s].scene_box[3 + k];
-        }
-    }
-    float m[3], f[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float hk = 3.0f * hi[k];
-        const float lk = 3.0f * lo[k];
-        const float ext = hk - lk;
-        m[k] = lk;
-        f[k] = 1024.0f / ext;
-    }
-    const uint32_t base = lb * (uint32_t)kBoundsTile;
-#pragma unroll
-    for (int r = 0; r < kBoundsItems; ++r) {
-        const uint32_t i = base + r * kBoundsThreads + threadIdx.x;
-        if (i < D.nt) {
-            const size_t g = (size_t)D.prim_off + i;
-            const uint32_t qx = quantise(cent[3 * g + 0], m[0], f[0]);
-            const uint32_t qy = quantise(cent[3 * g + 1], m[1], f[1]);
-            const uint32_t qz = quantise(cent[3 * g + 2], m[2], f[2]);
-            const uint32_t code = (part1by2(qx) << 2) | (part1by2(qy) << 1) | part1by2(qz);
-            keys[g] = code;
-            vals[g] = (int32_t)g;
-#pragma unroll
-            for (int p = 0; p < kCodePasses; ++p)
-                atomicAdd(&lhist[p * kRadix + ((code >> (p * kRadixBits)) & (kRadix - 1))], 1u);
         }
     }
     __syncthreads();
@@ -359,26 +372,44 @@ __global__ __launch_bounds__(kBoundsThreads) void k_morton(
 }
 
 // ---------------------------------------------------------------------------
-// Stage 3: one stable LSD pass of an onesweep radix sort (8-bit digit).
-// Tiles are claimed in launch order through an atomic counter, so a tile only
-// waits (decoupled look-back) on tiles that are already running.  Keys inside
-// a tile are ranked in load order (wave, item, lane), which keeps the sort
-// stable; ids start as the batch-global index, so each scene ends in the
-// (code, id) order of cmpMorton (LBVH.cpp:21-38).  The scene pass (digit =
-// scene of the id, global base = the scene's first sorted position) groups the
-// scenes of a batch after the four code passes.
+// Stage 3: stable LSD passes of 8-bit digits.  Keys inside a tile are ranked
+// in load order (wave, item, lane), which keeps each pass stable; ids start as
+// the batch-global index, so each scene ends in the (code, id) order of
+// cmpMorton (LBVH.cpp:21-38).  The scene pass (digit = scene of the id, global
+// base = the scene's first sorted position) groups the scenes of a batch after
+// the four code passes.
+//
+// Two forms of a pass:
+//  - below kSortLargeN keys, onesweep: one launch; tiles are claimed in launch
+//    order through an atomic counter and take their global offsets by a
+//    decoupled look-back over earlier tiles (so a tile only waits on tiles
+//    that are already running);
+//  - from kSortLargeN keys, count / scan / scatter: k_sort_count writes every
+//    tile's digit counts, k_sort_scan turns them into each tile's global
+//    offsets, and k_sort_scatter places the keys.  4 more bytes read per key,
+//    but no tile waits on another: at that size the look-back chain, not the
+//    bytes, bounded the onesweep form.
 // ---------------------------------------------------------------------------
 constexpr int kLookBack = 4;  // status words a look-back step loads at once
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 
-template <bool SCENE_PASS, int ITEMS>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(
+// The digit of a key of one pass: 8 bits of the code, or the scene of the id.
+template <bool SCENE_PASS>
+__device__ __forceinline__ uint32_t sort_digit(uint32_t key, int32_t val, uint32_t shift, uint32_t S,
+                                               const uint32_t *s_soff)
+{
+    if (SCENE_PASS) return upper_scene(S, (uint32_t)val, [&](uint32_t i) { return s_soff[i]; });
+    return (key >> shift) & (kRadix - 1);
+}
+
+template <bool SCENE_PASS, int ITEMS, bool LOOKBACK>
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin, uint32_t *__restrict__ kout,
     int32_t *__restrict__ vout, uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
     const SceneDesc *__restrict__ desc, uint32_t S, uint32_t *__restrict__ status,
-    uint32_t *__restrict__ tile_counter)
+    uint32_t *__restrict__ tile_counter, const uint32_t *__restrict__ offs, uint32_t ntiles)
 {
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t s_tile;
@@ -391,35 +422,41 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     __shared__ uint32_t s_soff[SCENE_PASS ? kRadix : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    if (LOOKBACK && tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     for (int j = tid; j < kWaves * kRadix; j += kSortThreads) (&s_wave[0][0])[j] = 0u;
     if (SCENE_PASS)
         for (int j = tid; j < kRadix; j += kSortThreads) s_soff[j] = j < (int)S ? desc[j].prim_off : 0xFFFFFFFFu;
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = LOOKBACK ? s_tile : blockIdx.x;
     const uint32_t base = tile * (uint32_t)(kSortThreads * ITEMS);
-    auto digit_of = [&](uint32_t key, int32_t val) -> uint32_t {
-        if (SCENE_PASS) return upper_scene(S, (uint32_t)val, [&](uint32_t i) { return s_soff[i]; });
-        return (key >> shift) & (kRadix - 1);
-    };
 
     uint32_t key[ITEMS];
     int32_t val[ITEMS];
     uint32_t rank[ITEMS];
-    uint32_t dig[ITEMS];
+    uint32_t dig[SCENE_PASS ? ITEMS : 1];  // the scene pass keeps its digits (a search each)
     bool valid[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const uint32_t i = base + wave * (64 * ITEMS) + r * 64 + lane;
         valid[r] = i < n;
-        key[r] = valid[r] ? kin[i] : 0u;
-        val[r] = valid[r] ? vin[i] : 0;
-        dig[r] = valid[r] ? digit_of(key[r], val[r]) : 0u;
+        const uint32_t ic = valid[r] ? i : n - 1;  // past n: the last key again, then masked off
+        key[r] = kin[ic];
+        val[r] = vin[ic];
+        if (SCENE_PASS) dig[SCENE_PASS ? r : 0] = valid[r] ? sort_digit<SCENE_PASS>(key[r], val[r], shift, S, s_soff) : 0u;
     }
+    auto digit = [&](int r) -> uint32_t {
+        if (SCENE_PASS) return dig[SCENE_PASS ? r : 0];
+        return (key[r] >> shift) & (kRadix - 1);
+    };
+    // Rank: the lanes of a wave holding item r's digit are matched by eight
+    // ballots; the lowest of them adds their number to the wave's counter of
+    // that digit.  (Issuing the items' counter updates as LDS atomics without
+    // this read-modify-write chain, the old values broadcast afterwards,
+    // measured slower: 179 against 143 us per pass at 33.5M keys.)
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const uint32_t d = dig[r];
+        const uint32_t d = digit(r);
         uint64_t peers = __ballot(valid[r]);
 #pragma unroll
         for (int b = 0; b < kRadixBits; ++b) {
@@ -447,58 +484,62 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t excl = block_exclusive_scan(cnt, s_scan, &tile_total);
     s_excl[dg] = excl;
 
-    // Decoupled look-back over earlier tiles for this digit.
-    uint32_t *my = status + (size_t)tile * kRadix + dg;
-    uint32_t prefix = 0;
-    if (tile == 0) {
-        __hip_atomic_store(my, kFlagInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(my, kFlagAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // Look back kLookBack tiles per round trip: their status words are
-        // loaded together, then taken newest first until an inclusive prefix
-        // (tile 0 always publishes one) or a tile not ready yet, which is
-        // loaded again from there.
-        int32_t j = (int32_t)tile - 1;
-        while (j >= 0) {
-            uint32_t sv[kLookBack];
+    if (LOOKBACK) {
+        // Decoupled look-back over earlier tiles for this digit.
+        uint32_t *my = status + (size_t)tile * kRadix + dg;
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(my, kFlagInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(my, kFlagAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // Look back kLookBack tiles per round trip: their status words are
+            // loaded together, then taken newest first until an inclusive
+            // prefix (tile 0 always publishes one) or a tile not ready yet,
+            // which is loaded again from there.
+            int32_t j = (int32_t)tile - 1;
+            while (j >= 0) {
+                uint32_t sv[kLookBack];
 #pragma unroll
-            for (int u = 0; u < kLookBack; ++u)
-                sv[u] = j - u >= 0 ? __hip_atomic_load(status + (size_t)(j - u) * kRadix + dg, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT)
-                                   : 0u;
-            int taken = 0;
-            bool done = false;
+                for (int u = 0; u < kLookBack; ++u)
+                    sv[u] = j - u >= 0 ? __hip_atomic_load(status + (size_t)(j - u) * kRadix + dg, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT)
+                                       : 0u;
+                int taken = 0;
+                bool done = false;
 #pragma unroll
-            for (int u = 0; u < kLookBack; ++u) {
-                if (j - u < 0) break;
-                const uint32_t flag = sv[u] & ~kValMask;
-                if (flag == 0u) break;
-                prefix += sv[u] & kValMask;
-                taken = u + 1;
-                if (flag == kFlagInc) {
-                    done = true;
-                    break;
+                for (int u = 0; u < kLookBack; ++u) {
+                    if (j - u < 0) break;
+                    const uint32_t flag = sv[u] & ~kValMask;
+                    if (flag == 0u) break;
+                    prefix += sv[u] & kValMask;
+                    taken = u + 1;
+                    if (flag == kFlagInc) {
+                        done = true;
+                        break;
+                    }
                 }
+                if (done) break;
+                if (taken == 0) __builtin_amdgcn_s_sleep(1);
+                j -= taken;
             }
-            if (done) break;
-            if (taken == 0) __builtin_amdgcn_s_sleep(1);
-            j -= taken;
+            __hip_atomic_store(my, kFlagInc | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __hip_atomic_store(my, kFlagInc | (prefix + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    uint32_t gbase;
-    if (SCENE_PASS) {
-        gbase = dg < S ? s_soff[dg] : 0u;
+        uint32_t gbase;
+        if (SCENE_PASS) {
+            gbase = dg < S ? s_soff[dg] : 0u;
+        } else {
+            gbase = block_exclusive_scan(hist[dg], s_scan, nullptr);
+        }
+        s_gofs[dg] = gbase + prefix;
     } else {
-        gbase = block_exclusive_scan(hist[dg], s_scan, nullptr);
+        s_gofs[dg] = offs[(size_t)dg * ntiles + tile];  // k_sort_scan: global base + earlier tiles
     }
-    s_gofs[dg] = gbase + prefix;
     __syncthreads();
 
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         if (valid[r]) {
-            const uint32_t d = dig[r];
+            const uint32_t d = digit(r);
             const uint32_t lp = s_excl[d] + s_wave[wave][d] + rank[r];
             s_keys[lp] = key[r];
             s_vals[lp] = val[r];
@@ -509,10 +550,111 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
         const uint32_t k = s_keys[i];
         const int32_t v = s_vals[i];
-        const uint32_t d = digit_of(k, v);
+        const uint32_t d = sort_digit<SCENE_PASS>(k, v, shift, S, s_soff);
         const uint32_t dst = s_gofs[d] + (i - s_excl[d]);
         kout[dst] = k;
         vout[dst] = v;
+    }
+}
+
+// Digit counts of every tile of a count / scan / scatter pass (the same tiles
+// and digits as k_sort_scatter): counts[d * ntiles + tile].
+template <bool SCENE_PASS, int ITEMS>
+__global__ __launch_bounds__(kSortThreads) void k_sort_count(
+    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin, uint32_t n, uint32_t shift,
+    const SceneDesc *__restrict__ desc, uint32_t S, uint32_t *__restrict__ counts, uint32_t ntiles)
+{
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t s_wave[kWaves][kRadix];
+    __shared__ uint32_t s_soff[SCENE_PASS ? kRadix : 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int j = tid; j < kWaves * kRadix; j += kSortThreads) (&s_wave[0][0])[j] = 0u;
+    if (SCENE_PASS)
+        for (int j = tid; j < kRadix; j += kSortThreads) s_soff[j] = j < (int)S ? desc[j].prim_off : 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)(kSortThreads * ITEMS);
+    uint32_t key[ITEMS];
+    int32_t val[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {  // all loads before any use; past n: the last key again, not counted
+        const uint32_t i = base + wave * (64 * ITEMS) + r * 64 + lane;
+        const uint32_t ic = i < n ? i : n - 1;
+        key[r] = kin[ic];
+        val[r] = SCENE_PASS ? vin[ic] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const uint32_t i = base + wave * (64 * ITEMS) + r * 64 + lane;
+        if (i < n) atomicAdd(&s_wave[wave][sort_digit<SCENE_PASS>(key[r], val[r], shift, S, s_soff)], 1u);
+    }
+    __syncthreads();
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) c += s_wave[w][tid];
+    counts[(size_t)tid * ntiles + tile] = c;
+}
+
+// Global offset of every (digit, tile) of a count / scan / scatter pass: the
+// digit's base (the exclusive scan of the pass histogram, or the scene's first
+// sorted position) plus the counts of the digit in earlier tiles.  One block
+// per digit.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(const uint32_t *__restrict__ counts,
+                                                            uint32_t *__restrict__ offs, uint32_t ntiles,
+                                                            const uint32_t *__restrict__ hist,
+                                                            const SceneDesc *__restrict__ desc, uint32_t S,
+                                                            uint32_t scene_pass)
+{
+    constexpr int kPer = 16;  // consecutive tiles per thread and round
+    // a round's counts, staged through LDS so that global loads and stores
+    // are coalesced: tile j of the round at row j / kPer, column j % kPer,
+    // rows padded to kPer + 1 words so that a thread's row reads without
+    // bank conflicts
+    __shared__ uint32_t s_c[kSortThreads * (kPer + 1)];
+    __shared__ uint32_t s_scan[kSortThreads / 64];
+    __shared__ uint32_t s_base;
+    const uint32_t d = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (scene_pass) {
+        if (tid == 0) s_base = d < S ? desc[d].prim_off : 0u;
+    } else {
+        const uint32_t e = block_exclusive_scan(hist[tid], s_scan, nullptr);
+        if ((uint32_t)tid == d) s_base = e;
+    }
+    __syncthreads();
+    uint32_t run = s_base;
+    const uint32_t *c = counts + (size_t)d * ntiles;
+    uint32_t *o = offs + (size_t)d * ntiles;
+    auto at = [](uint32_t j) { return (j / kPer) * (kPer + 1) + j % kPer; };
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += kSortThreads * kPer) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t j = (uint32_t)(k * kSortThreads + tid), t = t0 + j;
+            s_c[at(j)] = t < ntiles ? c[t] : 0u;
+        }
+        __syncthreads();
+        uint32_t v[kPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            v[k] = s_c[tid * (kPer + 1) + k];
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t x = run + block_exclusive_scan(sum, s_scan, &total);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            s_c[tid * (kPer + 1) + k] = x;
+            x += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t j = (uint32_t)(k * kSortThreads + tid), t = t0 + j;
+            if (t < ntiles) o[t] = s_c[at(j)];
+        }
+        run += total;
+        __syncthreads();
     }
 }
 
@@ -1202,42 +1344,58 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
     const uint32_t n = b.total_tris;
     const uint32_t tiles = sort_tiles(n);
     const uint32_t passes = kCodePasses + (S > 1 ? 1 : 0);
-    const uint32_t nzero = kCodePasses * kRadix + kCounters + passes * tiles * kRadix;
-    if (ev) hipEventRecord(ev[0], st);
+    const bool large = n >= kSortLargeN;  // count / scan / scatter passes, no look-back status to clear
+    const uint32_t nzero = kCodePasses * kRadix + kCounters + (large ? 0u : passes * tiles * kRadix);
+    if (ev && ev[0]) hipEventRecord(ev[0], st);
     const uint32_t nb = b.total_blocks > 0 ? b.total_blocks : 1;
     hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(kBoundsThreads), 0, st, b.d_desc, S, b.total_blocks,
                        b.xyz, b.idx, b.trec, b.cent, b.partials, b.hist, nzero);
     bool big = false;
     for (uint32_t i = 0; i < S; ++i) big = big || b.desc[i].nblocks > kBoxInline;
     if (big) hipLaunchKernelGGL(k_scene_box, dim3(S), dim3(1024), 0, st, b.d_desc, b.partials, b.hdr);
-    if (ev) hipEventRecord(ev[1], st);
+    if (ev && ev[1]) hipEventRecord(ev[1], st);
     const int start = (int)(passes & 1);  // the sorted result ends in buffer 0
     if (b.total_blocks > 0)
-        hipLaunchKernelGGL(k_morton, dim3(b.total_blocks), dim3(kBoundsThreads), 0, st, b.d_desc, S,
-                           b.cent, b.partials, b.keys[start], b.vals[start], b.hist, b.hdr);
-    if (ev) hipEventRecord(ev[2], st);
+        hipLaunchKernelGGL(k_morton, dim3(min_u32(b.total_blocks, kMortonBlocksPerCu * (uint32_t)cu_count)),
+                           dim3(kBoundsThreads), 0, st, b.d_desc, S, b.total_blocks, b.cent, b.partials,
+                           b.keys[start], b.vals[start], b.hist, b.hdr);
+    if (ev && ev[2]) hipEventRecord(ev[2], st);
     if (n > 0) {
+        // large: the tiles' digit counts and offsets in the status words (2 x lt x 256 of them)
+        constexpr uint32_t kLargeTile = kSortThreads * kSortItemsLarge;
+        const uint32_t lt = (n + kLargeTile - 1) / kLargeTile;
+        uint32_t *counts = b.status, *offs = b.status + (size_t)lt * kRadix;
         for (uint32_t p = 0; p < passes; ++p) {
             const int src = (start + p) & 1, dst = src ^ 1;
+            const bool scene_pass = p >= (uint32_t)kCodePasses;
+            const uint32_t shift = scene_pass ? 0u : p * kRadixBits;
+            const uint32_t *hist = b.hist + (scene_pass ? 0u : p * kRadix);
+            if (large) {
+                if (scene_pass)
+                    hipLaunchKernelGGL((k_sort_count<true, kSortItemsLarge>), dim3(lt), dim3(kSortThreads), 0, st,
+                                       b.keys[src], b.vals[src], n, shift, b.d_desc, S, counts, lt);
+                else
+                    hipLaunchKernelGGL((k_sort_count<false, kSortItemsLarge>), dim3(lt), dim3(kSortThreads), 0, st,
+                                       b.keys[src], b.vals[src], n, shift, b.d_desc, S, counts, lt);
+                hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(kSortThreads), 0, st, counts, offs, lt, hist,
+                                   b.d_desc, S, (uint32_t)scene_pass);
+            }
             uint32_t *status = b.status + (size_t)p * tiles * kRadix;
             uint32_t *tc = b.counters + kCntTiles + p;
-            const bool large = n >= kSortLargeN;
-            const uint32_t lt = large ? (n + kSortThreads * kSortItemsLarge - 1) / (kSortThreads * kSortItemsLarge)
-                                      : tiles;
-#define RT_SORT(SC, IT, SHIFT, HIST)                                                                      \
-    hipLaunchKernelGGL((k_onesweep<SC, IT>), dim3(lt), dim3(kSortThreads), 0, st, b.keys[src], b.vals[src], \
-                       b.keys[dst], b.vals[dst], n, SHIFT, HIST, b.d_desc, S, status, tc)
-            if (p < (uint32_t)kCodePasses) {
-                if (large) RT_SORT(false, kSortItemsLarge, p * kRadixBits, b.hist + p * kRadix);
-                else RT_SORT(false, kSortItemsSmall, p * kRadixBits, b.hist + p * kRadix);
+#define RT_SORT(SC, IT, LB, GRID)                                                                          \
+    hipLaunchKernelGGL((k_sort_scatter<SC, IT, LB>), dim3(GRID), dim3(kSortThreads), 0, st, b.keys[src],    \
+                       b.vals[src], b.keys[dst], b.vals[dst], n, shift, hist, b.d_desc, S, status, tc, offs, lt)
+            if (large) {
+                if (scene_pass) RT_SORT(true, kSortItemsLarge, false, lt);
+                else RT_SORT(false, kSortItemsLarge, false, lt);
             } else {
-                if (large) RT_SORT(true, kSortItemsLarge, 0u, b.hist);
-                else RT_SORT(true, kSortItemsSmall, 0u, b.hist);
+                if (scene_pass) RT_SORT(true, kSortItemsSmall, true, tiles);
+                else RT_SORT(false, kSortItemsSmall, true, tiles);
             }
 #undef RT_SORT
         }
     }
-    if (ev) hipEventRecord(ev[3], st);
+    if (ev && ev[3]) hipEventRecord(ev[3], st);
     if (b.total_table > 0 && n > 0) {
         const uint32_t blocks = (uint32_t)min_u32((uint32_t)((b.total_table + 255) / 256), 4096u);
         hipLaunchKernelGGL(k_split_table, dim3(blocks), dim3(256), 0, st, b.d_desc, S, b.total_table,
@@ -1261,7 +1419,7 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
                            b.big[0], b.big[1], b.small, b.counters, b.top_nodes, b.top_counts, b.heap,
                            b.trav);
     }
-    if (ev) hipEventRecord(ev[4], st);
+    if (ev && ev[4]) hipEventRecord(ev[4], st);
     if (n > 0) {
         const uint32_t blocks = min_u32(2u * (uint32_t)cu_count, (n + 63) / 64);
         hipLaunchKernelGGL(k_subtree, dim3(blocks > 0 ? blocks : 1), dim3(kSubtreeThreads), 0, st,
@@ -1276,7 +1434,7 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
         hipLaunchKernelGGL(k_top_refit, dim3(S), dim3(kTopThreads), 0, st, b.d_desc, b.hdr, b.top_nodes,
                            b.top_counts, b.heap, b.trav);
     }
-    if (ev) hipEventRecord(ev[5], st);
+    if (ev && ev[5]) hipEventRecord(ev[5], st);
     return hipGetLastError();
 }
 
