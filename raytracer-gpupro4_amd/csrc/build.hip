@@ -166,12 +166,11 @@ __device__ __forceinline__ void write_heap_word(rt_node32 *n, int32_t prim_count
     *reinterpret_cast<int2 *>(&n->prim_count) = make_int2(prim_count, prim_pos);
 }
 
-// Box and sorted-order triangle record of one leaf primitive.
-__device__ __forceinline__ void leaf_prim(const int32_t *__restrict__ vals, const TriRec *__restrict__ trec,
-                                          TravTri *__restrict__ tris, uint32_t pos, int32_t j, int32_t cnt,
-                                          uint32_t h, float lo[3], float hi[3])
+// Box and sorted-order triangle record of one leaf primitive, from its
+// triangle record T (trec[vals[pos]]).
+__device__ __forceinline__ void leaf_prim_rec(const TriRec &T, TravTri *__restrict__ tris, uint32_t pos, int32_t j,
+                                              int32_t cnt, uint32_t h, float lo[3], float hi[3])
 {
-    const TriRec T = trec[vals[pos]];
     const float A[3] = {T.a.x, T.a.y, T.a.z}, B[3] = {T.b.x, T.b.y, T.b.z}, C[3] = {T.c.x, T.c.y, T.c.z};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {  // the triangle's box, as the bounds pass computed it
@@ -183,6 +182,13 @@ __device__ __forceinline__ void leaf_prim(const int32_t *__restrict__ vals, cons
     t.e1 = make_float4(B[0] - A[0], B[1] - A[1], B[2] - A[2], __int_as_float(j == 0 ? cnt : 0));
     t.e2 = make_float4(C[0] - A[0], C[1] - A[1], C[2] - A[2], __int_as_float(j == 0 ? (int32_t)h : 0));
     tris[pos] = t;
+}
+
+__device__ __forceinline__ void leaf_prim(const int32_t *__restrict__ vals, const TriRec *__restrict__ trec,
+                                          TravTri *__restrict__ tris, uint32_t pos, int32_t j, int32_t cnt,
+                                          uint32_t h, float lo[3], float hi[3])
+{
+    leaf_prim_rec(trec[vals[pos]], tris, pos, j, cnt, h, lo, hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -690,9 +696,15 @@ __device__ __forceinline__ uint32_t split_search(const uint32_t *__restrict__ co
                                                  const SceneDesc &D, uint32_t first, uint32_t last,
                                                  uint32_t X)
 {
-    const uint32_t w = X >> (30 - D.tbits);
-    uint32_t lo = table[D.table_off + w], hi = table[D.table_off + w + 1];
+    const uint32_t sh = 30 - D.tbits;
+    const uint32_t w = X >> sh;
+    uint32_t lo = table[D.table_off + w];
     lo = lo < first + 1 ? first + 1 : lo;
+    // X on a bucket boundary (a split bit p >= 30 - tbits, as at most nodes
+    // of the top levels): the bucket's first position is the answer, since
+    // every code from there on is >= X; the search would only confirm it.
+    if ((X & ((1u << sh) - 1u)) == 0u) return lo;
+    uint32_t hi = table[D.table_off + w + 1];
     hi = hi > last ? last : hi;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -795,7 +807,12 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
     int4 *__restrict__ small, uint32_t *__restrict__ counters, int4 *__restrict__ top_nodes,
     uint32_t *__restrict__ top_counts, rt_node32 *__restrict__ heap, TravNode *__restrict__ trav)
 {
+    // A level of at most kTopLds nodes is also kept in LDS with its first and
+    // last codes, so that a level costs two global round trips (the split
+    // table, then the codes on either side of the split), not five.
     __shared__ uint32_t s_n, s_next, s_ntop;
+    __shared__ int4 s_fr[2][kTopLds];
+    __shared__ uint2 s_cc[2][kTopLds];
     const uint32_t s = blockIdx.x;
     const SceneDesc D = desc[s];
     const int tid = threadIdx.x;
@@ -815,6 +832,8 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
         const int4 root = make_int4(1, (int)D.prim_off, (int)D.nt, -1);
         if (D.nt > (uint32_t)kSubtreeMax) {
             cur[0] = root;
+            s_fr[0][0] = root;
+            s_cc[0][0] = make_uint2(codes[D.prim_off], codes[D.prim_off + D.nt - 1]);
             s_n = 1;
         } else {
             small[atomicAdd(&counters[kCntSmall], 1u)] = root;
@@ -825,18 +844,28 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
     }
     __syncthreads();
     uint32_t level = 0;
+    int c = 0;  // LDS frontier buffer of this level
     while (true) {
         const uint32_t n = s_n;
         if (n == 0) break;
+        const bool in_lds = n <= (uint32_t)kTopLds;
         const uint32_t level_start = s_ntop;
         __syncthreads();
         for (uint32_t i = tid; i < n; i += kTopThreads) {
-            const int4 e = cur[i];
+            const int4 e = in_lds ? s_fr[c][i] : cur[i];
             const uint32_t h = (uint32_t)e.x, first = (uint32_t)e.y;
             const int32_t cnt = e.z, pr = e.w;
             const uint32_t last = first + cnt - 1;
             const int pp = pr < 0 ? 31 : (pr & 31);
-            const uint32_t cf = codes[first], cl = codes[last];
+            uint32_t cf, cl;
+            if (in_lds) {
+                const uint2 cc = s_cc[c][i];
+                cf = cc.x;
+                cl = cc.y;
+            } else {
+                cf = codes[first];
+                cl = codes[last];
+            }
             int axis = 0, p = 0;
             if (classify(h, cnt, pp, leaf_level, cf, cl, &axis, &p)) {
                 small[atomicAdd(&counters[kCntSmall], 1u)] = e;
@@ -844,6 +873,7 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
             }
             const uint32_t X = ((cf >> p) | 1u) << p;
             const uint32_t sp = split_search(codes, table, D, first, last, X);
+            const uint32_t c_left = codes[sp - 1], c_right = codes[sp];  // the children's inner codes
             const int32_t k = (int32_t)sp - 1;
             write_heap_word(&sheap[h], (cnt << 3) | axis, (int32_t)(first - D.prim_off));
             TravNode *tn = &trav[k];
@@ -855,10 +885,26 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
             const int32_t cl_cnt = (int32_t)(sp - first), cr_cnt = (int32_t)(last - sp + 1);
             const int4 c0 = make_int4((int)(2 * h), (int)first, cl_cnt, (k << 6) | p);
             const int4 c1 = make_int4((int)(2 * h + 1), (int)sp, cr_cnt, (k << 6) | (1 << 5) | p);
-            if (cl_cnt > kSubtreeMax) nxt[atomicAdd(&s_next, 1u)] = c0;
-            else small[atomicAdd(&counters[kCntSmall], 1u)] = c0;
-            if (cr_cnt > kSubtreeMax) nxt[atomicAdd(&s_next, 1u)] = c1;
-            else small[atomicAdd(&counters[kCntSmall], 1u)] = c1;
+            if (cl_cnt > kSubtreeMax) {
+                const uint32_t slot = atomicAdd(&s_next, 1u);
+                nxt[slot] = c0;
+                if (slot < (uint32_t)kTopLds) {
+                    s_fr[c ^ 1][slot] = c0;
+                    s_cc[c ^ 1][slot] = make_uint2(cf, c_left);
+                }
+            } else {
+                small[atomicAdd(&counters[kCntSmall], 1u)] = c0;
+            }
+            if (cr_cnt > kSubtreeMax) {
+                const uint32_t slot = atomicAdd(&s_next, 1u);
+                nxt[slot] = c1;
+                if (slot < (uint32_t)kTopLds) {
+                    s_fr[c ^ 1][slot] = c1;
+                    s_cc[c ^ 1][slot] = make_uint2(c_right, cl);
+                }
+            } else {
+                small[atomicAdd(&counters[kCntSmall], 1u)] = c1;
+            }
         }
         __syncthreads();
         if (tid == 0) {
@@ -869,6 +915,7 @@ __global__ __launch_bounds__(kTopThreads) void k_top(
         int4 *t = cur;
         cur = nxt;
         nxt = t;
+        c ^= 1;
         ++level;
         __syncthreads();
     }
@@ -950,6 +997,8 @@ __global__ __launch_bounds__(kTopLevelThreads) void k_top_level(
 // round of memory latency instead of one per level), and the subtree is
 // refitted bottom-up (LBVH_BuildAABB, LBVH.hlsl:228-270) from LDS child boxes.
 // ---------------------------------------------------------------------------
+static_assert(kSubtreeMax % kSubtreeThreads == 0, "a subtree's positions split evenly over the threads");
+
 struct SubtreeLds {
     uint32_t codes[kSubtreeMax];
     union {
@@ -1030,7 +1079,19 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
         }
         // internal subtree root: rcnt <= kSubtreeMax.  The topology is emitted
         // level by level from the codes in LDS; leaves are only listed here.
-        for (int32_t j = tid; j < rcnt; j += kSubtreeThreads) L.codes[j] = codes[rfirst + j];
+        // The subtree's triangle records are fetched now, by sorted id, so that
+        // their latency (a random gather: the records are in input order)
+        // overlaps the emission; the barriers below do not wait for them.
+        constexpr int kPerThread = kSubtreeMax / kSubtreeThreads;
+        TriRec pre[kPerThread];
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const int32_t j = tid + q * kSubtreeThreads;
+            if (j < rcnt) {
+                L.codes[j] = codes[rfirst + j];
+                pre[q] = trec[vals[rfirst + j]];
+            }
+        }
         if (tid == 0) {
             L.fr[0][0] = root;
             L.n = 1;
@@ -1093,15 +1154,19 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
         }
         // every leaf triangle at once: one thread per position gathers its
         // record, writes its sorted-order triangle and keeps its box in LDS
-        for (int32_t p = tid; p < rcnt; p += kSubtreeThreads) {
-            const int4 lf = L.leaf[L.pos_leaf[p]];
-            float lo[3] = {INFINITY, INFINITY, INFINITY};
-            float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            leaf_prim(vals, trec, tris, rfirst + p, p - lf.y, lf.z >> 2, (uint32_t)lf.x, lo, hi);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                L.pbox[p][c] = lo[c];
-                L.pbox[p][3 + c] = hi[c];
+        for (int q = 0; q < kPerThread; ++q) {
+            const int32_t p = tid + q * kSubtreeThreads;
+            if (p < rcnt) {
+                const int4 lf = L.leaf[L.pos_leaf[p]];
+                float lo[3] = {INFINITY, INFINITY, INFINITY};
+                float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                leaf_prim_rec(pre[q], tris, rfirst + p, p - lf.y, lf.z >> 2, (uint32_t)lf.x, lo, hi);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    L.pbox[p][c] = lo[c];
+                    L.pbox[p][3 + c] = hi[c];
+                }
             }
         }
         __syncthreads();
@@ -1194,6 +1259,7 @@ __global__ __launch_bounds__(kTopThreads) void k_top_refit(
     rt_node32 *__restrict__ heap, TravNode *trav)
 {
     __shared__ uint32_t s_off[kMaxLevels + 1];
+    __shared__ int4 s_tops[kTopLds];
     const uint32_t s = blockIdx.x;
     const SceneDesc D = desc[s];
     if (D.nt == 0) return;
@@ -1209,9 +1275,17 @@ __global__ __launch_bounds__(kTopThreads) void k_top_refit(
     __syncthreads();
     const int4 *tops = top_nodes + D.prim_off;
     rt_node32 *sheap = heap + D.heap_off;
+    // all of the scene's big nodes read at once when they fit in LDS: a level
+    // then costs one global round trip (its children's boxes), not two
+    const uint32_t total = s_off[levels];
+    const bool in_lds = total <= (uint32_t)kTopLds;
+    if (in_lds) {
+        for (uint32_t i = threadIdx.x; i < total; i += kTopThreads) s_tops[i] = tops[i];
+        __syncthreads();
+    }
     for (int32_t l = (int32_t)levels - 1; l >= 0; --l) {
         for (uint32_t i = s_off[l] + threadIdx.x; i < s_off[l + 1]; i += kTopThreads) {
-            const int4 nd = tops[i];
+            const int4 nd = in_lds ? s_tops[i] : tops[i];
             const int32_t k = nd.x, pr = nd.z;
             const TravNode *tn = &trav[k];
             float lo[3], hi[3];

@@ -39,6 +39,7 @@ constexpr int kBoundsTile = kBoundsThreads * kBoundsItems;
 constexpr int kSubtreeMax = 512;
 constexpr int kSubtreeThreads = 256;
 constexpr int kTopThreads = 1024;
+constexpr int kTopLds = 1024;  // k_top: frontier entries of a level kept in LDS
 // A batch of one scene from this many primitives emits and refits its top
 // levels with one launch per level over the whole chip (k_top_level,
 // k_top_refit_level) instead of one block.
