@@ -992,29 +992,37 @@ __global__ __launch_bounds__(kTopLevelThreads) void k_top_level(
 // Stage 4c: small subtrees.  A persistent block takes one entry of the
 // subtree list at a time.  A leaf (of any size) is reduced by the whole block;
 // an internal node (<= kSubtreeMax primitives) has its codes staged in LDS and
-// its subtree emitted level by level with the same rules as k_top, in LDS
-// only; then every leaf triangle of the subtree is gathered at once (one
-// round of memory latency instead of one per level), and the subtree is
-// refitted bottom-up (LBVH_BuildAABB, LBVH.hlsl:228-270) from LDS child boxes.
+// its subtree emitted with the same rules as k_top, all nodes at once, in LDS
+// only; its triangle records are fetched meanwhile and laid out once the
+// leaves are known, and the subtree is refitted bottom-up (LBVH_BuildAABB,
+// LBVH.hlsl:228-270) from LDS child boxes.
 // ---------------------------------------------------------------------------
 static_assert(kSubtreeMax % kSubtreeThreads == 0, "a subtree's positions split evenly over the threads");
 
 struct SubtreeLds {
     uint32_t codes[kSubtreeMax];
-    union {
-        int4 fr[2][kSubtreeMax];        // emission frontiers
-        float pbox[kSubtreeMax][6];     // then: each primitive's box, by local position
-    };
-    int4 inode[kSubtreeMax];            // k, heap index, parent, first
+    float pbox[kSubtreeMax][6];         // each primitive's box, by local position
+    int4 inode[kSubtreeMax];            // k, heap index, parent, first (by depth)
     int32_t iword[kSubtreeMax];         // (count << 2) | axis
     float cbox[kSubtreeMax][12];        // child boxes by local internal index
     int32_t cref[kSubtreeMax][2];
     int4 leaf[kSubtreeMax];             // heap index, local first, (count << 2) | axis, parent
     uint16_t pos_leaf[kSubtreeMax];     // the leaf of each local position
+    // emission, per split position i (between local positions i and i + 1)
+    int8_t dl[kSubtreeMax];             // highest differing bit of codes i and i + 1; -1: equal codes
+    uint16_t ra[kSubtreeMax], rb[kSubtreeMax];  // the node's positions [ra, rb]
+    uint16_t par[kSubtreeMax];          // its parent split position (the root: itself)
+    uint8_t side[kSubtreeMax];          // its side under the parent
+    uint16_t anc[kSubtreeMax];          // pointer jumping: an ancestor,
+    uint8_t dist[kSubtreeMax];          //   the edges up to it (kDeepPath: more than any heap index holds)
+    uint32_t bits[kSubtreeMax];         //   and the side bits of that path, first edge highest
+    uint8_t claim[2][kSubtreeMax];      // child side taken by an internal node
+    uint32_t dcnt[kMaxLevels + 2];      // internal nodes per depth
     uint32_t lvl[kMaxLevels + 2];
     float red[kSubtreeThreads / 64 * 6];
-    uint32_t item, n, next, nint, nleaf;
+    uint32_t item, nleaf, levels;
 };
+constexpr uint32_t kDeepPath = 31;
 
 __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
     const SceneDesc *__restrict__ desc, uint32_t S, SceneHdr *__restrict__ hdr,
@@ -1078,7 +1086,7 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
             continue;
         }
         // internal subtree root: rcnt <= kSubtreeMax.  The topology is emitted
-        // level by level from the codes in LDS; leaves are only listed here.
+        // from the codes in LDS; leaves are only listed here.
         // The subtree's triangle records are fetched now, by sorted id, so that
         // their latency (a random gather: the records are in input order)
         // overlaps the emission; the barriers below do not wait for them.
@@ -1092,66 +1100,166 @@ __global__ __launch_bounds__(kSubtreeThreads) void k_subtree(
                 pre[q] = trec[vals[rfirst + j]];
             }
         }
-        if (tid == 0) {
-            L.fr[0][0] = root;
-            L.n = 1;
-            L.next = 0;
-            L.nint = 0;
-            L.nleaf = 0;
-            L.lvl[0] = 0;
+        const uint32_t C = (uint32_t)rcnt;
+        const uint32_t hroot = (uint32_t)root.x;
+        if (tid < kMaxLevels + 2) L.dcnt[tid] = 0u;
+        if (tid == 0) L.nleaf = 0u;
+        __syncthreads();
+        // The topology of the subtree, with every node at once instead of
+        // level by level (the radix tree of the sorted codes, as in Karras
+        // 2012).  Split position i (between local positions i and i + 1) is
+        // a node when the codes there differ; its split bit is their highest
+        // differing bit d, it spans the positions sharing the code bits above
+        // d, and its parent is whichever split bounding that span has the
+        // lower split bit.  These are the nodes, ranges and splits that
+        // build_recursive (LBVH.cpp:190-314) reaches level by level; the
+        // leaves are the child sides no node takes, and the depth limit drops
+        // every node whose heap index reaches 2^(depth-1).
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const uint32_t i = (uint32_t)tid + q * kSubtreeThreads;
+            if (i + 1 < C) {
+                const uint32_t x = L.codes[i] ^ L.codes[i + 1];
+                L.dl[i] = x ? (int8_t)(31 - __clz(x)) : (int8_t)-1;
+            }
+            if (i < C) L.claim[0][i] = L.claim[1][i] = 0;
         }
         __syncthreads();
-        int cur = 0;
-        uint32_t level = 0;
-        while (true) {
-            const uint32_t n = L.n;
-            if (n == 0) break;
-            for (uint32_t i = tid; i < n; i += kSubtreeThreads) {
-                const int4 e = L.fr[cur][i];
-                const uint32_t h = (uint32_t)e.x, first = (uint32_t)e.y;
-                const int32_t cnt = e.z, pr = e.w;
-                const uint32_t lf = first - rfirst, ll = lf + cnt - 1;
-                const int pp = pr < 0 ? 31 : (pr & 31);
-                int axis = 0, p = 0;
-                if (classify(h, cnt, pp, leaf_level, L.codes[lf], L.codes[ll], &axis, &p)) {
-                    const uint32_t slot = atomicAdd(&L.nleaf, 1u);
-                    L.leaf[slot] = make_int4((int)h, (int)lf, (cnt << 2) | axis, pr);
-                    for (int32_t j = 0; j < cnt; ++j) L.pos_leaf[lf + j] = (uint16_t)slot;
-                    continue;
-                }
-                const uint32_t X = ((L.codes[lf] >> p) | 1u) << p;
-                uint32_t lo = lf + 1, hi = ll;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (L.codes[mid] < X) lo = mid + 1;
-                    else hi = mid;
-                }
-                const uint32_t sp = rfirst + lo;
-                const int32_t k = (int32_t)sp - 1;
-                const uint32_t slot = atomicAdd(&L.nint, 1u);
-                L.inode[slot] = make_int4(k, (int)h, pr, (int)first);
-                L.iword[slot] = (cnt << 2) | axis;
-                if (level == 0) {
-                    if (pr < 0) hdr[s].root_ref = k;
-                    else trav[pr >> 6].ref[(pr >> 5) & 1] = k;
+        bool node[kPerThread];
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const uint32_t i = (uint32_t)tid + q * kSubtreeThreads;
+            node[q] = i + 1 < C && L.dl[i] >= 0;
+            if (!node[q]) continue;
+            const uint32_t sh = (uint32_t)L.dl[i] + 1u;  // <= 30
+            const uint32_t lowX = (L.codes[i] >> sh) << sh, highX = lowX + (1u << sh);
+            uint32_t lo = 0, hi = i;  // first position whose code is >= lowX
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (L.codes[mid] < lowX) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t ra = lo;
+            lo = i + 2;  // first position whose code is >= highX (i + 1 shares the prefix)
+            hi = C;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (L.codes[mid] < highX) lo = mid + 1;
+                else hi = mid;
+            }
+            const uint32_t rb = lo - 1;
+            uint32_t parent = i, sd = 0;
+            if (ra > 0 || rb + 1 < C) {
+                if (ra == 0 || (rb + 1 < C && L.dl[rb] < L.dl[ra - 1])) {
+                    parent = rb;  // the split after the span: this node is its left child
                 } else {
-                    L.cref[(uint32_t)(pr >> 6) - rfirst][(pr >> 5) & 1] = k;
+                    parent = ra - 1;  // the split before the span: its right child
+                    sd = 1;
                 }
-                const uint32_t c = atomicAdd(&L.next, 2u);
-                L.fr[cur ^ 1][c] = make_int4((int)(2 * h), (int)first, (int32_t)(sp - first), (k << 6) | p);
-                L.fr[cur ^ 1][c + 1] = make_int4((int)(2 * h + 1), (int)sp,
-                                                 (int32_t)(first + cnt - sp), (k << 6) | (1 << 5) | p);
             }
-            __syncthreads();
-            if (tid == 0) {
-                L.lvl[level + 1] = L.nint;
-                L.n = L.next;
-                L.next = 0;
-            }
-            cur ^= 1;
-            ++level;
-            __syncthreads();
+            L.ra[i] = (uint16_t)ra;
+            L.rb[i] = (uint16_t)rb;
+            L.par[i] = (uint16_t)parent;
+            L.side[i] = (uint8_t)sd;
+            L.anc[i] = (uint16_t)parent;
+            L.dist[i] = parent == i ? 0 : 1;
+            L.bits[i] = sd;
         }
+        // each node's heap index: its path to the subtree root by pointer
+        // jumping, h = (h_root << depth) | side bits
+        while (true) {
+            uint32_t na[kPerThread], nd[kPerThread], nb[kPerThread];
+            bool ch[kPerThread];
+            bool any = false;
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                const uint32_t i = (uint32_t)tid + q * kSubtreeThreads;
+                ch[q] = false;
+                if (!node[q]) continue;
+                const uint32_t a0 = L.anc[i];
+                const uint32_t a1 = L.anc[a0];
+                if (a1 == a0) continue;  // a0 is the root
+                const uint32_t d0 = L.dist[i], d1 = L.dist[a0];
+                const uint32_t dsum = d0 + d1;
+                nd[q] = dsum > kDeepPath - 1 ? kDeepPath : dsum;
+                nb[q] = nd[q] == kDeepPath ? 0u : (L.bits[a0] << d0) | L.bits[i];
+                na[q] = a1;
+                ch[q] = any = true;
+            }
+            if (!__syncthreads_or(any)) break;
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                if (!ch[q]) continue;
+                const uint32_t i = (uint32_t)tid + q * kSubtreeThreads;
+                L.anc[i] = (uint16_t)na[q];
+                L.dist[i] = (uint8_t)nd[q];
+                L.bits[i] = nb[q];
+            }
+        }
+        // nodes above the depth limit: depth counts, the child sides they take
+        uint32_t rank[kPerThread], hh[kPerThread];
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const uint32_t i = (uint32_t)tid + q * kSubtreeThreads;
+            if (!node[q]) continue;
+            const uint32_t d = L.dist[i];
+            const uint64_t H = d < kDeepPath ? (((uint64_t)hroot << d) | L.bits[i]) : ~0ull;
+            node[q] = H < (uint64_t)leaf_level;
+            if (!node[q]) continue;
+            hh[q] = (uint32_t)H;
+            rank[q] = atomicAdd(&L.dcnt[d], 1u);
+            if (L.par[i] != i) L.claim[L.side[i]][L.par[i]] = 1;
+        }
+        __syncthreads();
+        if (tid < 64) {  // depth offsets: the depths in use are 0 .. levels - 1
+            const uint32_t c = tid < kMaxLevels + 2 ? L.dcnt[tid] : 0u;
+            uint32_t x = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o);
+                if (tid >= o) x += y;
+            }
+            if (tid < kMaxLevels + 1) L.lvl[tid + 1] = x;
+            if (tid == 0) L.lvl[0] = 0u;
+            const uint64_t used = __ballot(c != 0u);
+            if (tid == 0) L.levels = used ? 64u - (uint32_t)__clzll(used) : 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const uint32_t i = (uint32_t)tid + q * kSubtreeThreads;
+            if (!node[q]) continue;
+            const uint32_t d = L.dist[i], h = hh[q], pi = L.par[i];
+            const int dd = L.dl[i];
+            const uint32_t ra = L.ra[i], rb = L.rb[i];
+            const int32_t k = (int32_t)(rfirst + i);
+            const int32_t pr = pi == i ? rpr : (((int32_t)(rfirst + pi) << 6) | ((int32_t)L.side[i] << 5) | L.dl[pi]);
+            const uint32_t slot = L.lvl[d] + rank[q];
+            L.inode[slot] = make_int4(k, (int)h, pr, (int)(rfirst + ra));
+            L.iword[slot] = ((int32_t)(rb - ra + 1) << 2) | ((30 - dd) % 3);
+            if (pi == i) {  // the subtree root, a child of the top levels
+                if (rpr < 0) hdr[s].root_ref = k;
+                else trav[rpr >> 6].ref[(rpr >> 5) & 1] = k;
+            } else {
+                L.cref[pi][L.side[i]] = k;
+            }
+            // the child sides no node took are leaves (LBVH.cpp:194-209):
+            // axis (31 - p) % 3 for one primitive, at the depth limit, or under
+            // a split at bit 0; axis 1 for equal codes
+            for (int sd = 0; sd < 2; ++sd) {
+                if (L.claim[sd][i]) continue;
+                const uint32_t lf = sd ? i + 1 : ra, ll = sd ? rb : i;
+                const int32_t lc = (int32_t)(ll - lf + 1);
+                const uint32_t hl = 2 * h + (uint32_t)sd;
+                const int axis = (lc <= 1 || hl >= leaf_level || dd == 0) ? (31 - dd) % 3 : 1;
+                const uint32_t ls = atomicAdd(&L.nleaf, 1u);
+                L.leaf[ls] = make_int4((int)hl, (int)lf, (lc << 2) | axis, (k << 6) | (sd << 5) | dd);
+                for (int32_t j = 0; j < lc; ++j) L.pos_leaf[lf + j] = (uint16_t)ls;
+            }
+        }
+        __syncthreads();
+        const uint32_t level = L.levels;
         // every leaf triangle at once: one thread per position gathers its
         // record, writes its sorted-order triangle and keeps its box in LDS
 #pragma unroll
