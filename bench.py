@@ -408,8 +408,9 @@ def workload_whitted(args, rtcore):
         tris += st["tri_tests"]
         counted += st["primary_rays"] + st["shadow_rays"] + st["bounce_rays"]
     # the other device forms over the same frames (same images and rays)
-    forms = {"wavefront": 0, "one-pass": rtcore.WHITTED_ONE_PASS, "split": rtcore.WHITTED_SPLIT}
-    base_sw = kw["switches"] & ~(rtcore.WHITTED_ONE_PASS | rtcore.WHITTED_SPLIT)
+    forms = {"wavefront": 0, "one-pass": rtcore.WHITTED_ONE_PASS, "split": rtcore.WHITTED_SPLIT,
+             "split-quads": rtcore.WHITTED_SPLIT | rtcore.WHITTED_QUADS}
+    base_sw = kw["switches"] & ~(rtcore.WHITTED_ONE_PASS | rtcore.WHITTED_SPLIT | rtcore.WHITTED_QUADS)
     form = next(k for k, bit in forms.items() if kw["switches"] == base_sw | bit)
     others = []
     for name, bit in forms.items():
@@ -431,7 +432,8 @@ def workload_whitted(args, rtcore):
     achieved = alg / (per_launch_ms * 1e-3) / 1e9
     ctx.close()
     kernels = {"wavefront": "k_whitted_first + k_whitted_next per bounce", "one-pass": "k_render_whitted",
-               "split": "k_whitted_first + k_whitted_rest"}
+               "split": "k_whitted_first + k_whitted_rest",
+               "split-quads": "k_whitted_first + k_whitted_rest_quad"}
     out = result(METRIC, mrays, "Mrays/s", args.steps, args.warmup, wall,
                  {"workload": f"{NTRI}-triangle soup (static), 5 materials (matte, mirror, glass 1.5, "
                               f"half mirror, default), {W}x{H} Whitted frames, {WHITTED_BOUNCES} bounces, "

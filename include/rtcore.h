@@ -315,9 +315,13 @@ enum {
     RT_WHITTED_ONE_PASS = 1u << 3, /* one launch, each lane runs its pixel's passes; by default
                                       pass 0 runs per pixel and each bounce pass over a compacted
                                       queue of the live rays (same results) */
-    RT_WHITTED_SPLIT = 1u << 4     /* pass 0 per pixel, then one launch in which each live path
+    RT_WHITTED_SPLIT = 1u << 4,    /* pass 0 per pixel, then one launch in which each live path
                                       of the compacted queue runs all its remaining passes (same
                                       results; not with RT_WHITTED_ONE_PASS) */
+    RT_WHITTED_QUADS = 1u << 5     /* with RT_WHITTED_SPLIT: four lanes per live path share each
+                                      remaining pass's closest-hit and shadow traversals (same
+                                      results; an LBVH scene without RT_FLAG_COUNT, else one lane
+                                      per path) */
 };
 typedef struct rt_whitted_desc {
     rt_camera cam;
@@ -332,11 +336,12 @@ typedef struct rt_whitted_desc {
 } rt_whitted_desc;
 int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc,
                       uint32_t flags, rt_frame_stats *stats);
-/* Times every launch shape, traversal form and the three Whitted pass forms
- * (per-pass wavefront, RT_WHITTED_ONE_PASS, RT_WHITTED_SPLIT) on this frame
- * (renders into the context's frame buffer), keeps the fastest shape in the
- * context and sets RT_WHITTED_ONE_PASS / RT_WHITTED_SPLIT in desc->switches to
- * the fastest form.  cache_path as for rt_autotune_jobs. */
+/* Times every launch shape, traversal form and the four Whitted pass forms
+ * (per-pass wavefront, RT_WHITTED_ONE_PASS, RT_WHITTED_SPLIT, RT_WHITTED_SPLIT
+ * | RT_WHITTED_QUADS) on this frame (renders into the context's frame
+ * buffer), keeps the fastest shape in the context and sets RT_WHITTED_ONE_PASS
+ * / RT_WHITTED_SPLIT / RT_WHITTED_QUADS in desc->switches to the fastest
+ * form.  cache_path as for rt_autotune_jobs. */
 int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc,
                         const char *cache_path, rt_tuning *best);
 

@@ -933,10 +933,12 @@ int rt_render_whitted(rt_ctx *ctx, int scene, const rt_whitted_desc *desc, uint3
     if (desc->env_flag != -1 && desc->env_flag != 0)
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: env_flag %d (-1 or 0)", desc->env_flag);
     if (desc->switches & ~(uint32_t)(RT_WHITTED_SHADOWS | RT_WHITTED_PHONG | RT_WHITTED_GLOSS | RT_WHITTED_ONE_PASS |
-                                     RT_WHITTED_SPLIT))
+                                     RT_WHITTED_SPLIT | RT_WHITTED_QUADS))
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: unknown switches 0x%x", desc->switches);
     if ((desc->switches & RT_WHITTED_ONE_PASS) && (desc->switches & RT_WHITTED_SPLIT))
         return fail(ctx, RT_ERR_ARG, "rt_render_whitted: RT_WHITTED_ONE_PASS and RT_WHITTED_SPLIT together");
+    if ((desc->switches & RT_WHITTED_QUADS) && !(desc->switches & RT_WHITTED_SPLIT))
+        return fail(ctx, RT_ERR_ARG, "rt_render_whitted: RT_WHITTED_QUADS without RT_WHITTED_SPLIT");
     const SceneDesc &d = b->desc[i];
     const bool have = i < b->shading.size();
     if (have && (b->shading[i].nv != d.nv || b->shading[i].nt != d.nt) &&
@@ -1459,12 +1461,14 @@ int rt_autotune_direct(rt_ctx *ctx, int scene, const rt_render_desc *desc, const
 }
 
 // The Whitted pass forms the autotuner tries: 0 per-pass wavefront,
-// 1 RT_WHITTED_ONE_PASS, 2 RT_WHITTED_SPLIT.
+// 1 RT_WHITTED_ONE_PASS, 2 RT_WHITTED_SPLIT, 3 RT_WHITTED_SPLIT | RT_WHITTED_QUADS.
+constexpr int kWhittedForms = 4;
 static uint32_t whitted_pass_form(uint32_t switches, int form)
 {
-    switches &= ~(uint32_t)(RT_WHITTED_ONE_PASS | RT_WHITTED_SPLIT);
+    switches &= ~(uint32_t)(RT_WHITTED_ONE_PASS | RT_WHITTED_SPLIT | RT_WHITTED_QUADS);
     if (form == 1) switches |= RT_WHITTED_ONE_PASS;
     if (form == 2) switches |= RT_WHITTED_SPLIT;
+    if (form == 3) switches |= RT_WHITTED_SPLIT | RT_WHITTED_QUADS;
     return switches;
 }
 
@@ -1481,7 +1485,7 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     {
         rt_tuning t{};
         int pf = 0;
-        if (cache_find(ctx, cache_path, key, &t, &pf) && pf >= 0 && pf <= 2) {
+        if (cache_find(ctx, cache_path, key, &t, &pf) && pf >= 0 && pf < kWhittedForms) {
             t.quad_tiles = ctx->tune.quad_tiles;  // a direct-frame choice: kept as set
             ctx->tune = t;
             desc->switches = whitted_pass_form(desc->switches, pf);
@@ -1495,7 +1499,8 @@ int rt_autotune_whitted(rt_ctx *ctx, int scene, rt_whitted_desc *desc, const cha
     float bms = 1e30f;
     int rc = RT_OK;
     rt_whitted_desc d = *desc;
-    for (int pf = 0; pf < 3 && rc == RT_OK; ++pf) {
+    for (int pf = 0; pf < kWhittedForms && rc == RT_OK; ++pf) {
+        if (pf == 3 && uses_bvh(*b, i)) break;  // quads need the LBVH's heap indices
         d.switches = whitted_pass_form(desc->switches, pf);
         for (int form = 0; form < kForms && rc == RT_OK; ++form) {
             for (const rt_tuning &shape : kShapes) {

@@ -1468,16 +1468,29 @@ struct WCount {
 // One intersection + colour pass of ray r.  Adds the pass's colour into acc
 // (or sets white), and returns true when the path continues, with r turned
 // into the continuation ray.  first: store the hit and shadow result.
-template <int FORM, class Stack>
+// With QUAD the four lanes of a quad (quad_lane: this lane's place) hold the
+// same path; each traces its share of the pass's rays (traverse_quad), the
+// quad takes the closest hit, and the rest of the pass runs on all four.
+template <int FORM, bool QUAD = false, class Stack>
 __device__ __forceinline__ bool whitted_pass(const SceneView &sv, const ShadeView &sh, const WhittedParams &wp,
                                              int32_t root_ref, const float *root_box, WRay &r, float *acc,
                                              bool &white, rt_hit24 *hit_out, uint8_t *occ_out, Stack &stk,
-                                             WCount &cnt)
+                                             WCount &cnt, uint32_t quad_lane = 0)
 {
-    const TraceOut h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, r.ox, r.oy, r.oz, r.dx,
-                                                  r.dy, r.dz, 10000000000000000.0f, r.self, stk);
-    cnt.boxes += h.boxes;
-    cnt.ntris += h.tris;
+    TraceOut h;
+    if constexpr (QUAD) {
+        uint32_t rank;
+        h = traverse_quad<false>(sv.nodes, sv.tris, root_ref, root_box, quad_lane, kQuadDepth, r.ox, r.oy, r.oz,
+                                 r.dx, r.dy, r.dz, 10000000000000000.0f, r.self, stk, rank);
+        cnt.boxes += h.boxes;
+        cnt.ntris += h.tris;
+        quad_closest(h, rank);
+    } else {
+        h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz,
+                                       10000000000000000.0f, r.self, stk);
+        cnt.boxes += h.boxes;
+        cnt.ntris += h.tris;
+    }
     if (hit_out) {
         store_hit(hit_out, h);
         *occ_out = 2;
@@ -1542,11 +1555,21 @@ __device__ __forceinline__ bool whitted_pass(const SceneView &sv, const ShadeVie
     if ((dot3(nx, ny, nz, lx, ly, lz) * dot3(nx, ny, nz, r.dx, r.dy, r.dz)) > 0.0f) {
         bool unoccluded = true;
         if (wp.switches & RT_WHITTED_SHADOWS) {
-            const TraceOut s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz, -lx,
-                                                         -ly, -lz, dist, h.tri, stk);
-            cnt.boxes += s.boxes;
-            cnt.ntris += s.tris;
-            cnt.nshadow += 1;
+            TraceOut s;
+            if constexpr (QUAD) {
+                uint32_t rank;
+                s = traverse_quad<true>(sv.nodes, sv.tris, root_ref, root_box, quad_lane, kQuadDepth, hx, hy, hz,
+                                        -lx, -ly, -lz, dist, h.tri, stk, rank);
+                cnt.boxes += s.boxes;
+                cnt.ntris += s.tris;
+                s.tri = quad_any(s.tri >= 0) ? 0 : -1;
+            } else {
+                s = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz, -lx, -ly, -lz, dist,
+                                              h.tri, stk);
+                cnt.boxes += s.boxes;
+                cnt.ntris += s.tris;
+            }
+            cnt.nshadow += quad_lane == 0 ? 1u : 0u;
             unoccluded = s.tri < 0;
             if (occ_out) *occ_out = unoccluded ? 0 : 1;
         }
@@ -1802,6 +1825,48 @@ __global__ __launch_bounds__(BLOCK) void k_whitted_rest(SceneView sv, ShadeView 
     if (count) {
         count_add(stats, 0, cnt.boxes);
         count_add(stats, 1, cnt.ntris);
+    }
+    count_add(stats, 2, cnt.nshadow);
+    count_add(stats, 3, nbounce);
+}
+
+// k_whitted_rest with a quad per path: the four lanes of a quad share each
+// pass's closest-hit and shadow traversals (the heaviest paths' chains of
+// dependent iterations are what the launch waits for).  Same frame.
+template <int BLOCK, int KLDS, int FORM>
+__global__ __launch_bounds__(BLOCK) void k_whitted_rest_quad(SceneView sv, ShadeView sh, WhittedParams wp,
+                                                             float4 *__restrict__ rgba,
+                                                             const RayRec *__restrict__ qin,
+                                                             const uint32_t *in_count, unsigned long long *stats)
+{
+    __shared__ uint2 lds[KLDS * BLOCK];
+    RayStack<BLOCK, KLDS> stk;
+    stk.lds = &lds[threadIdx.x];
+    int32_t root_ref;
+    float root_box[6];
+    load_root(sv.hdr, root_ref, root_box);
+    WCount cnt;
+    uint32_t nbounce = 0;
+    const uint32_t n = *in_count;
+    const uint32_t s = threadIdx.x & 3u;
+    constexpr uint32_t kPaths = BLOCK / 4;  // paths per block and round
+    for (uint32_t base = blockIdx.x * kPaths; base < n; base += gridDim.x * kPaths) {
+        const uint32_t i = base + (threadIdx.x >> 2);
+        if (i < n) {  // the same for the four lanes of a quad
+            WRay r = get_ray(qin, i);
+            const float4 a = rgba[r.pix];
+            float acc[4] = {a.x, a.y, a.z, a.w};
+            bool white = false;
+            for (uint32_t pass = 1; pass <= wp.bounces; ++pass) {
+                nbounce += s == 0 ? 1u : 0u;
+                if (!whitted_pass<FORM, true>(sv, sh, wp, root_ref, root_box, r, acc, white, nullptr, nullptr, stk,
+                                              cnt, s))
+                    break;
+            }
+            if (s == 0)
+                rgba[r.pix] =
+                    white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
     }
     count_add(stats, 2, cnt.nshadow);
     count_add(stats, 3, nbounce);
@@ -2163,6 +2228,9 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     const int keep = (flags & RT_FLAG_KEEP_HITS) ? 1 : 0, cnt = (flags & RT_FLAG_COUNT) ? 1 : 0;
     const bool one_pass = (d.switches & RT_WHITTED_ONE_PASS) != 0 || d.bounces == 0;
     const bool split = (d.switches & RT_WHITTED_SPLIT) != 0;
+    // quads need heap indices in the nodes (the LBVH's); counted frames keep
+    // one lane per ray (the same counts as the other forms)
+    const bool quads = (d.switches & RT_WHITTED_QUADS) != 0 && !cnt && !uses_bvh(b, scene);
     const uint64_t px = (uint64_t)d.width * d.height;
     RayRec *q[2] = {static_cast<RayRec *>(queue), static_cast<RayRec *>(queue) + px};
     if (!one_pass) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, qcount, d.bounces + 1);
@@ -2188,7 +2256,10 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
             uint32_t blocks = (uint32_t)((px + B - 1) / B);                                               \
             const uint32_t cap = (uint32_t)cu_count * 8u;                                                 \
             if (blocks > cap) blocks = cap;                                                               \
-            if (split)                                                                                    \
+            if (split && quads)                                                                           \
+                hipLaunchKernelGGL((k_whitted_rest_quad<B, K, F>), dim3(blocks), dim3(B), 0, st, sv, sh,  \
+                                   wp, fb.rgba, q[0], qcount, stats);                                     \
+            else if (split)                                                                               \
                 hipLaunchKernelGGL((k_whitted_rest<B, K, F>), dim3(blocks), dim3(B), 0, st, sv, sh, wp,   \
                                    fb.rgba, q[0], qcount, cnt, stats);                                    \
             else                                                                                          \

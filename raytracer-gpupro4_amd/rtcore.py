@@ -44,6 +44,7 @@ WHITTED_PHONG = 1 << 1
 WHITTED_GLOSS = 1 << 2
 WHITTED_ONE_PASS = 1 << 3
 WHITTED_SPLIT = 1 << 4
+WHITTED_QUADS = 1 << 5
 
 # Byte layouts of RayTracerCS/Shaders/Structures.hlsl (Node 32 B, Ray 48 B,
 # Intersection 24 B) as numpy dtypes.

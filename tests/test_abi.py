@@ -112,7 +112,7 @@ def test_python_constants_match_the_header(rtcore):
     for n, v in mirrored.items():
         assert getattr(rtcore, py[n]) == v, n
     for n in ("RT_FLAG_ANYHIT", "RT_FLAG_COUNT", "RT_FLAG_KEEP_HITS", "RT_FLAG_PERSISTENT",
-              "RT_FLAG_START_AT_IROOT", "RT_WHITTED_ONE_PASS", "RT_WHITTED_SPLIT", "RT_BVH_SAH",
+              "RT_FLAG_START_AT_IROOT", "RT_WHITTED_ONE_PASS", "RT_WHITTED_SPLIT", "RT_WHITTED_QUADS", "RT_BVH_SAH",
               "RT_STRUCTURE_LBVH", "RT_MODE_GI", "RT_OK", "RT_ERR_BUDGET"):
         assert n in mirrored, n
 

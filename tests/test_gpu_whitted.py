@@ -11,8 +11,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 F = np.float32
-SHADOWS, PHONG, GLOSS, ONE_PASS, SPLIT = 1, 2, 4, 8, 16
-FORM_BITS = {"wavefront": 0, "one-pass": ONE_PASS, "split": SPLIT}
+SHADOWS, PHONG, GLOSS, ONE_PASS, SPLIT, QUADS = 1, 2, 4, 8, 16, 32
+FORM_BITS = {"wavefront": 0, "one-pass": ONE_PASS, "split": SPLIT, "split-quads": SPLIT | QUADS}
 
 
 def psnr(a, b):
@@ -96,12 +96,13 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("form", ["wavefront", "one-pass", "split"])
+@pytest.mark.parametrize("form", ["wavefront", "one-pass", "split", "split-quads"])
 @pytest.mark.parametrize("tag,kw", CASES, ids=[c[0] for c in CASES])
 def test_whitted_room_matches_oracle(ctx, oracle, rtcore, room, tag, kw, form):
-    """The three forms of the device frame (a compacted queue per bounce
+    """The four forms of the device frame (a compacted queue per bounce
     pass; one lane per pixel through every pass; pass 0 per pixel, then one
-    lane per live path through the rest) against the oracle."""
+    lane per live path through the rest; the same with four lanes per live
+    path) against the oracle."""
     s, xyz, idx, normals, mat_ids, mats, ref = room
     kw = dict(kw)
     kw["switches"] = kw["switches"] | FORM_BITS[form]
@@ -127,11 +128,33 @@ def test_whitted_forms_identical_at_1080p(ctx, rtcore, room):
         a, fa = out["wavefront"]
         print(f"{w}x{h}: " + ", ".join(f"{k} {v[0]['trace_ms']:.3f} ms" for k, v in out.items()) +
               f", rays {a['primary_rays'] + a['shadow_rays'] + a['bounce_rays']}", flush=True)
-        for form in ("one-pass", "split"):
+        for form in ("one-pass", "split", "split-quads"):
             b, fb = out[form]
             assert np.array_equal(fa, fb), form
             for k in ("primary_rays", "shadow_rays", "bounce_rays", "box_tests", "tri_tests"):
                 assert a[k] == b[k], (form, k)
+
+
+def test_whitted_quads_identical_uncounted(ctx, rtcore, room):
+    """Frames rendered without RT_FLAG_COUNT (so the split form's live paths
+    do take four lanes each) equal the one-lane split form's, and so do
+    their ray counts, at 1080p and on a small image."""
+    s = room[0]
+    for w, h in ((1920, 1080), (333, 197)):
+        cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
+        kw = dict(bounces=5, env=(0.2, 0.3, 0.4), seed=11)
+        out = {}
+        for form in ("split", "split-quads"):
+            st = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | PHONG | GLOSS | FORM_BITS[form],
+                                    **kw)
+            out[form] = (st, ctx.frame_read(w, h, 1)[0])
+        (a, fa), (b, fb) = out["split"], out["split-quads"]
+        print(f"{w}x{h}: split {a['trace_ms']:.3f} ms, split-quads {b['trace_ms']:.3f} ms", flush=True)
+        assert np.array_equal(fa, fb)
+        for k in ("primary_rays", "shadow_rays", "bounce_rays"):
+            assert a[k] == b[k], k
+    with pytest.raises(rtcore.RtError):  # quads belong to the split form
+        ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | QUADS, **kw)
 
 
 def test_whitted_repeated_frames_identical(ctx, rtcore, room):
@@ -141,7 +164,7 @@ def test_whitted_repeated_frames_identical(ctx, rtcore, room):
     w, h = 640, 360
     cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
     kw = dict(bounces=3, env=(0.2, 0.3, 0.4), seed=5)
-    for sw in (SHADOWS | PHONG, SHADOWS | PHONG | ONE_PASS, SHADOWS | PHONG | SPLIT):
+    for sw in (SHADOWS | PHONG, SHADOWS | PHONG | ONE_PASS, SHADOWS | PHONG | SPLIT, SHADOWS | PHONG | SPLIT | QUADS):
         ref = None
         for rep in range(3):
             st = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=sw, flags=rtcore.FLAG_COUNT, **kw)
@@ -238,7 +261,7 @@ def test_autotune_whitted(ctx, rtcore, room, tmp_path):
     kw = dict(bounces=3, switches=SHADOWS | PHONG, env=(0.2, 0.3, 0.4))
     cache = tmp_path / "tune.txt"
     t1, sw1 = ctx.autotune_whitted(s, cam, w, h, (0.8, -0.6, 1.5), cache_path=str(cache), **kw)
-    assert sw1 & ~(ONE_PASS | SPLIT) == kw["switches"] and len(cache.read_text().splitlines()) == 1
+    assert sw1 & ~(ONE_PASS | SPLIT | QUADS) == kw["switches"] and len(cache.read_text().splitlines()) == 1
     t2, sw2 = ctx.autotune_whitted(s, cam, w, h, (0.8, -0.6, 1.5), cache_path=str(cache), **kw)
     assert (t1, sw1) == (t2, sw2) and len(cache.read_text().splitlines()) == 1
     ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), **dict(kw, switches=sw1))
