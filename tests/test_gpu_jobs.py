@@ -5,10 +5,11 @@ and the autotuner.
 GI tolerance: the device computes in fp32 (device sin/cos, IEEE div/sqrt);
 with the same PCG streams it follows the same paths as the FP64 oracle except
 where a path grazes an edge, so the same-stream image is compared by RMSE of
-the per-pixel mean radiance (bar 0.02 for radiance in [0, 1]) and >= 98 % of
-pixels within 1e-3; the converged comparison (1024 device samples against
-4096 independent FP64 samples) reports the RMSE and bounds it by the Monte
-Carlo noise of both estimates (bar 0.03)."""
+the per-pixel mean radiance (bar 1e-3 for radiance in [0, 1]; measured below
+5e-6, while a wrong jitter, seed or bounce rule gives 0.05 or more) and
+>= 99.5 % of pixels within 1e-3; the converged comparison (1024 device
+samples against 4096 independent FP64 samples) reports the RMSE and bounds it
+by the Monte Carlo noise of both estimates (bar 0.01; measured 0.0016)."""
 import numpy as np
 import pytest
 
@@ -51,7 +52,7 @@ def test_gi_same_streams_match_fp64(ctx, oracle, rtcore, mix):
     print(f"GI same streams: rays gpu={st['primary_rays'] + st['bounce_rays']} cpu={rays} "
           f"RMSE={rmse:.5f} within1e-3={close:.4f}", flush=True)
     assert abs((st["primary_rays"] + st["bounce_rays"]) - rays) <= 0.01 * rays
-    assert rmse < 0.02 and close >= 0.98
+    assert rmse < 1e-3 and close >= 0.995
 
 
 def test_gi_converged_against_fp64(ctx, oracle, rtcore, mix):
@@ -64,7 +65,7 @@ def test_gi_converged_against_fp64(ctx, oracle, rtcore, mix):
     r = r / 4096
     rmse = float(np.sqrt(np.mean((g - r) ** 2)))
     print(f"GI 1024 spp vs FP64 4096 spp (independent samples): RMSE={rmse:.5f}", flush=True)
-    assert rmse < 0.03
+    assert rmse < 0.01
 
 
 def test_jobs_mix_and_launch_mode_invariance(ctx, rtcore):
@@ -339,3 +340,78 @@ def test_autotune_direct_picks_and_caches(ctx, rtcore, mix, tmp_path):
         ctx.free(acc)
     finally:
         ctx.set_tuning(*saved)
+
+
+def test_batched_object_mix_full_size(ctx, oracle, rtcore):
+    """The batched GI workload at its full size: 64 object-mix scenes of
+    1,024..200,000 triangles (log-uniform, about 2.4M in all), each at 256^2,
+    512^2 or 1024^2, 4 spp x 4 bounces.  Two frames of the whole batch in
+    one persistent launch each equal, bit for bit, the same frames rendered
+    scene by scene; and a 256^2 scene's frame matches the FP64 oracle
+    following the same PCG streams (the same-stream bars above)."""
+    import math
+    n = 64
+    rng = np.random.default_rng(44)
+    sizes = np.exp(rng.uniform(math.log(1024), math.log(200000), n)).astype(np.int64)
+    res = rng.choice([256, 512, 1024], n)
+    scenes = [rtcore.gen_object_mix(int(sizes[k]), seed=1000 + k) for k in range(n)]
+    depths = [int(min(24, max(8, math.ceil(math.log2(max(len(i), 2))) + 2))) for _, i in scenes]
+    cams = []
+    for (x, _), r in zip(scenes, res):
+        lo, hi = x.min(0), x.max(0)
+        c = 0.5 * (lo + hi)
+        rad = 0.5 * float(np.linalg.norm(hi - lo))
+        a = math.radians(30.0)
+        pos = c + 1.6 * rad * np.array([math.cos(a), 0.45, math.sin(a)], np.float32)
+        cams.append(rtcore.camera_look_at(tuple(pos), tuple(c), (0, 1, 0), 60.0, 1.0))
+    hs = ctx.batch_create([len(i) for _, i in scenes], [len(x) for x, _ in scenes], depths)
+    for h_, (x, i) in zip(hs, scenes):
+        ctx.scene_upload(h_, x, i)
+    ps = [ctx.add_scene(x, i, d) for (x, i), d in zip(scenes, depths)]
+    acc_b = [ctx.alloc(16 * int(r) * int(r)) for r in res]
+    acc_p = [ctx.alloc(16 * int(r) * int(r)) for r in res]
+    for a, r in zip(acc_b + acc_p, list(res) + list(res)):
+        ctx.zero(a, 16 * int(r) * int(r))
+
+    def jobs(handles, accs, frame):
+        return [ctx.job(handles[k], cams[k], int(res[k]), int(res[k]), accs[k], rtcore.MODE_GI, 4, 4, 0.7, 1.0,
+                        seed=7, frame=frame) for k in range(n)]
+
+    rays_b = rays_p = 0
+    for f in range(2):
+        ctx.build(hs[0])
+        st = ctx.render_jobs(jobs(hs, acc_b, f), persistent=True)
+        rays_b += st["primary_rays"] + st["bounce_rays"]
+        js = jobs(ps, acc_p, f)
+        for k in range(n):
+            ctx.build(ps[k])
+            st = ctx.render_jobs([js[k]], persistent=False)
+            rays_p += st["primary_rays"] + st["bounce_rays"]
+    print(f"64 scenes, {int(sizes.sum())} tris, {int((res.astype(np.int64) ** 2).sum())} px: "
+          f"{rays_b} rays batched, {rays_p} scene by scene", flush=True)
+    assert rays_b == rays_p
+    for k in range(n):
+        r = int(res[k])
+        assert np.array_equal(ctx.read_accum(acc_b[k], r, r), ctx.read_accum(acc_p[k], r, r)), k
+    # the smallest 256^2 scene against the FP64 oracle, both frames' streams
+    k = min((k for k in range(n) if res[k] == 256), key=lambda k: sizes[k])
+    xyz, idx = scenes[k]
+    ref = oracle.build_scene(xyz, idx, depths[k])
+    want = np.zeros((256, 256))
+    for f in range(2):
+        want += oracle.render_gi(ref["nodes"], ref["ids"], xyz, idx, cams[k].as_array(), 256, 256, 4, 4, 0.7, 1.0,
+                                 7, frame=f, nthreads=16)[0]
+    got = ctx.read_accum(acc_b[k], 256, 256)
+    assert np.all(got[..., 3] == 8)
+    g, w = got[..., 0] / 8, want / 8
+    rmse = float(np.sqrt(np.mean((g - w) ** 2)))
+    close = float(np.mean(np.abs(g - w) <= 1e-3))
+    print(f"scene {k} ({len(idx)} tris, 256^2, 8 spp over two frames): GI vs FP64 same streams "
+          f"RMSE {rmse:.5f}, within 1e-3 {close:.4f}", flush=True)
+    assert rmse < 1e-3 and close >= 0.995
+    for a in acc_b + acc_p:
+        ctx.free(a)
+    for p in ps:
+        ctx.scene_destroy(p)
+    for h_ in hs:
+        ctx.scene_destroy(h_)

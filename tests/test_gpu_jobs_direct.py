@@ -195,5 +195,8 @@ def test_served_stream_shape_matches_oracle(ctx, oracle, rtcore):
             rmse = float(np.sqrt(np.mean((g - r) ** 2)))
             close = float(np.mean(np.abs(g - r) <= 1e-3))
             print(f"{tag}: GI vs FP64 same streams RMSE {rmse:.5f}, within 1e-3 {close:.4f}", flush=True)
-            assert np.all(got[..., 3] == 3) and rmse < 0.02 and close >= 0.98
+            # bars: a few paths that graze an edge take another way in fp32 than in
+            # FP64 (measured RMSE 0.0005-0.0010 with every pixel but a handful within
+            # 1e-3); a wrong jitter, seed or bounce gives 0.05 or more
+            assert np.all(got[..., 3] == 3) and rmse < 0.005 and close >= 0.999
     srv.close()
