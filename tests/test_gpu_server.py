@@ -99,3 +99,81 @@ def test_eviction_and_rebuild_give_same_images(ctx, rtcore, pool):
     st = srv.status()
     assert st["evictions"] > 0 and st["builds"] >= 4
     srv.close()
+
+
+def test_served_workload_shape_at_scale(ctx, rtcore):
+    """The served workload's shape at a tenth of its pool and a fiftieth of
+    its requests: 60 object-mix scenes of 1k..500k triangles (log-uniform),
+    240 requests arriving at 200 per second (Poisson), 512^2 or 1920x1080,
+    direct or 2-bounce GI at 1 spp, four views per scene so that streams
+    accumulate progressively, a 200 GB budget, the 1 ms batch window and
+    centre-first tiles of the bench's server.  Every request completes, every
+    stream holds as many passes as it was sent, latencies are finite, and one
+    stream of each kind (direct 512^2, direct 1080p, GI 512^2, GI 1080p) equals
+    its passes rendered alone."""
+    import math
+    rng = np.random.default_rng(56)
+    npool, nreq = 60, 240
+    sizes = np.exp(rng.uniform(math.log(1000), math.log(500000), npool)).astype(np.int64)
+    arrivals = np.cumsum(rng.exponential(1.0 / 200.0, nreq))
+    scene_of = rng.integers(0, npool, nreq)
+    big = rng.random(nreq) < 0.5
+    gi = rng.random(nreq) < 0.5
+    view_of = rng.integers(0, 4, nreq)
+    pool = {int(k): rtcore.gen_object_mix(int(sizes[k]), seed=7000 + int(k)) for k in np.unique(scene_of)}
+    depth = {k: int(min(24, max(8, math.ceil(math.log2(max(len(i), 2))) + 2))) for k, (_, i) in pool.items()}
+    srv = rtcore.Server(ctx, 200 << 30, max_inflight=32, tiles_per_step=32768, tile_order=rtcore.TILES_CENTRE,
+                        batch_window=0.001)
+    for k, (x, i) in pool.items():
+        srv.add_scene(k, x, i, depth[k])
+    streams = {}
+    for r in range(nreq):
+        k, v = int(scene_of[r]), int(view_of[r])
+        xyz = pool[k][0]
+        w, h = (1920, 1080) if big[r] else (512, 512)
+        lo, hi = xyz.min(0), xyz.max(0)
+        c = 0.5 * (lo + hi)
+        a = math.radians(30.0 + 90.0 * v)
+        pos = c + 1.6 * 0.5 * float(np.linalg.norm(hi - lo)) * np.array([math.cos(a), 0.45, math.sin(a)], np.float32)
+        cam = rtcore.camera_look_at(tuple(pos), tuple(c), (0, 1, 0), 60.0, w / h)
+        if gi[r]:
+            mode, p = rtcore.MODE_GI, dict(spp=1, bounces=2, albedo=0.7, sky=1.0, ambient=0.0, lights=None)
+        else:
+            mode, p = rtcore.MODE_DIRECT, dict(spp=1, bounces=0, albedo=0.8, sky=0.0, ambient=0.1,
+                                               lights=[tuple(hi + 0.5 * (hi - lo))])
+        srv.submit(k, v, cam, w, h, mode, p["spp"], p["bounces"], p["albedo"], p["sky"], p["ambient"],
+                   lights=p["lights"], seed=3, arrival=float(arrivals[r]))
+        streams.setdefault((k, v, w, h, mode), []).append(dict(p, cam=cam))
+    res, t_end = srv.run(list(arrivals))
+    st = srv.status()
+    assert len(res) == nreq and all(q["status"] == rtcore.RT_OK for q in res)
+    lat = np.array([q["completed"] - q["arrival"] for q in res])
+    print(f"{nreq} requests over {len(pool)} scenes ({int(sizes[np.unique(scene_of)].sum())} tris), "
+          f"{len(streams)} streams: p50 {np.percentile(lat, 50) * 1e3:.2f} ms, p99 {np.percentile(lat, 99) * 1e3:.2f} ms, "
+          f"{st['steps']} steps, {st['builds']} builds, {st['rays'] / st['device_s'] / 1e6:.0f} Mrays/s of device time",
+          flush=True)
+    assert np.all(np.isfinite(lat)) and np.all(lat >= 0) and st["builds"] == len(pool)
+    checked = set()
+    for key, passes in sorted(streams.items(), key=lambda kv: -len(kv[1])):
+        k, v, w, h, mode = key
+        got, n = srv.read_stream(*key)
+        assert n == len(passes), key
+        kind = (w, mode)
+        if kind in checked:
+            continue
+        checked.add(kind)
+        x, i = pool[k]
+        s = ctx.add_scene(x, i, depth[k])
+        ctx.build(s)
+        acc = ctx.alloc(16 * w * h)
+        ctx.zero(acc, 16 * w * h)
+        for f, p in enumerate(passes):
+            ctx.render_jobs([ctx.job(s, p["cam"], w, h, acc, mode, p["spp"], p["bounces"], p["albedo"], p["sky"],
+                                     p["ambient"], lights=p["lights"], seed=3, frame=f)], persistent=False,
+                            want_stats=False)
+        want = ctx.read_accum(acc, w, h)
+        ctx.free(acc)
+        ctx.scene_destroy(s)
+        assert np.array_equal(got, want), key
+    assert len(checked) == 4
+    srv.close()
