@@ -64,6 +64,14 @@ struct RayStack {
         --sp;
         return (!kSpill || sp < KLDS) ? lds[sp * BLOCK] : spill[sp - KLDS];
     }
+    // pop() with both words of the entry read by one LDS load: the node
+    // comes with the T0 that culls it, instead of a second load after the test
+    __device__ __forceinline__ uint2 pop_both()
+    {
+        uint2 e = pop();
+        asm volatile("" : "+v"(e.x), "+v"(e.y));
+        return e;
+    }
 };
 
 __device__ __forceinline__ void box_test(const float *bx, float ox, float oy, float oz, float ix,
@@ -261,14 +269,10 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
             const int32_t fref = first_is_b ? q3.y : q3.x;
             const int32_t sref = first_is_b ? q3.x : q3.y;
             const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-            const bool s_live = !(s1 < 0.0f);
-            if (f_ok) {
-                if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
-                cur = fref;
-                continue;
-            }
-            if (s_live && !(s0 > r.t)) {
-                cur = sref;
+            const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+            if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+            if (f_ok || s_ok) {
+                cur = f_ok ? fref : sref;
                 continue;
             }
         } else {
@@ -334,14 +338,12 @@ __device__ __forceinline__ TraceOut traverse_shared(const TravNode *__restrict__
             const int32_t fref = first_is_b ? q3.y : q3.x;
             const int32_t sref = first_is_b ? q3.x : q3.y;
             const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-            const bool s_live = !(s1 < 0.0f);
-            if (f_ok) {
-                if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
-                cur = fref;
-                continue;
-            }
-            if (s_live && !(s0 > r.t)) {
-                cur = sref;
+            const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+            // both children: the first now, the second pushed; one of them:
+            // that one; neither: a pop
+            if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+            if (f_ok || s_ok) {
+                cur = f_ok ? fref : sref;
                 continue;
             }
         } else {
@@ -350,7 +352,7 @@ __device__ __forceinline__ TraceOut traverse_shared(const TravNode *__restrict__
         // pop the next live entry
         bool found = false;
         while (stk.sp > 0) {
-            const uint2 e = stk.pop();
+            const uint2 e = stk.pop_both();
             if (!(__uint_as_float(e.y) > r.t)) {
                 cur = (int32_t)e.x;
                 found = true;
@@ -459,14 +461,10 @@ __device__ __forceinline__ TraceOut traverse_from(const TravNode *__restrict__ n
             const int32_t fref = first_is_b ? q3.y : q3.x;
             const int32_t sref = first_is_b ? q3.x : q3.y;
             const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-            const bool s_live = !(s1 < 0.0f);
-            if (f_ok) {
-                if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
-                cur = fref;
-                continue;
-            }
-            if (s_live && !(s0 > r.t)) {
-                cur = sref;
+            const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+            if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+            if (f_ok || s_ok) {
+                cur = f_ok ? fref : sref;
                 continue;
             }
         } else {
@@ -561,14 +559,10 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
         const int32_t fref = first_is_b ? q3.y : q3.x;
         const int32_t sref = first_is_b ? q3.x : q3.y;
         const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-        const bool s_live = !(s1 < 0.0f);
-        if (f_ok) {
-            if (s_live && !(s0 > r.t)) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
-            T.cur = fref;
-            return false;
-        }
-        if (s_live && !(s0 > r.t)) {
-            T.cur = sref;
+        const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+        if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+        if (f_ok || s_ok) {
+            T.cur = f_ok ? fref : sref;
             return false;
         }
     } else {
@@ -836,13 +830,9 @@ __device__ __forceinline__ TraceOut traverse_quad(const TravNode *__restrict__ n
                 const float lim = top ? tmax : r.t;
                 const bool f_ok = !(f0 > lim || f1 < 0.0f);
                 const bool s_ok = !(s1 < 0.0f) && !(s0 > lim);
-                if (f_ok) {
-                    if (s_ok) stk.push(make_uint2((uint32_t)sref, top ? kNeverCulled : __float_as_uint(s0)));
-                    cur = fref;
-                    continue;
-                }
-                if (s_ok) {
-                    cur = sref;
+                if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, top ? kNeverCulled : __float_as_uint(s0)));
+                if (f_ok || s_ok) {
+                    cur = f_ok ? fref : sref;
                     continue;
                 }
             } else {
