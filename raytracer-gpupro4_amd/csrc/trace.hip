@@ -64,6 +64,12 @@ struct RayStack {
         --sp;
         return (!kSpill || sp < KLDS) ? lds[sp * BLOCK] : spill[sp - KLDS];
     }
+    // the entry pop() would return (entry 0 when empty: a read in bounds)
+    __device__ __forceinline__ uint2 peek() const
+    {
+        const int i = sp > 0 ? sp - 1 : 0;
+        return (!kSpill || i < KLDS) ? lds[i * BLOCK] : spill[i - KLDS];
+    }
     // pop() with both words of the entry read by one LDS load: the node
     // comes with the T0 that culls it, instead of a second load after the test
     __device__ __forceinline__ uint2 pop_both()
@@ -322,6 +328,9 @@ __device__ __forceinline__ TraceOut traverse_shared(const TravNode *__restrict__
     int32_t cur = root_ref;
     stk.sp = 0;
     while (true) {
+        // the stack top, read while the fetch is in flight: a pop after a
+        // node with no live child takes it without another LDS round trip
+        const uint2 ptop = stk.peek();
         float4 q0, q1, q2, q3f;
         fetch_step(nodes, tris, cur, q0, q1, q2, q3f);
         if (cur >= 0) {
@@ -349,7 +358,14 @@ __device__ __forceinline__ TraceOut traverse_shared(const TravNode *__restrict__
         } else {
             if (leaf_test_first(r, tris, (uint32_t)cur, q0, q1, q2, ox, oy, oz, dx, dy, dz, self, ANYHIT)) return r;
         }
-        // pop the next live entry
+        // pop the next live entry (a leaf's test may have pushed nothing, so
+        // the top read before it is still the top)
+        if (stk.sp == 0) break;
+        --stk.sp;
+        if (!(__uint_as_float(ptop.y) > r.t)) {
+            cur = (int32_t)ptop.x;
+            continue;
+        }
         bool found = false;
         while (stk.sp > 0) {
             const uint2 e = stk.pop_both();
@@ -445,6 +461,7 @@ __device__ __forceinline__ TraceOut traverse_from(const TravNode *__restrict__ n
     int32_t cur = trav_enter(nodes, root_ref, root_box, start, ox, oy, oz, ix, iy, iz, r, stk);
     if (cur == -1) return r;
     while (true) {
+        const uint2 ptop = stk.peek();  // read while the fetch is in flight
         float4 q0, q1, q2, q3f;
         fetch_step(nodes, tris, cur, q0, q1, q2, q3f);
         if (cur >= 0) {
@@ -469,6 +486,12 @@ __device__ __forceinline__ TraceOut traverse_from(const TravNode *__restrict__ n
             }
         } else {
             if (leaf_test_first(r, tris, (uint32_t)cur, q0, q1, q2, ox, oy, oz, dx, dy, dz, self, ANYHIT)) return r;
+        }
+        if (stk.sp == 0) break;
+        --stk.sp;
+        if (!(__uint_as_float(ptop.y) > r.t)) {
+            cur = (int32_t)ptop.x;
+            continue;
         }
         bool found = false;
         while (stk.sp > 0) {
@@ -543,6 +566,7 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
                                           const TravTri *__restrict__ tris, Stack &stk)
 {
     TraceOut &r = T.r;
+    const uint2 ptop = stk.peek();  // read while the fetch is in flight
     float4 q0, q1, q2, q3f;
     fetch_step(nodes, tris, T.cur, q0, q1, q2, q3f);
     if (T.cur >= 0) {
@@ -570,7 +594,13 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
                       MODE == 1 || (MODE == 2 && T.anyhit)))
             return true;
     }
-    // pop the next live entry
+    // pop the next live entry, the top first
+    if (stk.sp == 0) return true;
+    --stk.sp;
+    if (!(__uint_as_float(ptop.y) > r.t)) {
+        T.cur = (int32_t)ptop.x;
+        return false;
+    }
     while (stk.sp > 0) {
         const uint2 e = stk.pop();
         if (!(__uint_as_float(e.y) > r.t)) {
@@ -803,6 +833,7 @@ __device__ __forceinline__ TraceOut traverse_quad(const TravNode *__restrict__ n
     uint32_t items = 0, item = 0;
     stk.sp = 0;
     while (true) {
+        const uint2 ptop = stk.peek();  // read while the fetch is in flight
         float4 q0, q1, q2, q3f;
         fetch_step(nodes, tris, cur, q0, q1, q2, q3f);
         const bool at_node = cur >= 0;
@@ -842,6 +873,12 @@ __device__ __forceinline__ TraceOut traverse_quad(const TravNode *__restrict__ n
                 if (r.t < before) rank = item;
                 if (stop) return r;
             }
+        }
+        if (stk.sp == 0) break;
+        --stk.sp;
+        if (!(__uint_as_float(ptop.y) > r.t)) {
+            cur = (int32_t)ptop.x;
+            continue;
         }
         bool found = false;
         while (stk.sp > 0) {
