@@ -997,6 +997,207 @@ __device__ __forceinline__ float direct_term(const SceneView &sv, int32_t root_r
     return (ambient + diffuse) * albedo;
 }
 
+// An any-hit ray traced by a group of k lanes (k a power of two, kmask = k -
+// 1, lane s of the group): every lane walks the levels above heap depth D
+// (culled against tmax only, as an any-hit traversal culls until its first
+// hit, so the lanes walk them alike and number the subtrees at depth D and
+// the leaves above it alike); lane s searches the items numbered s modulo k.
+// A lane that accepts a triangle stops, and the others of its group (gmask)
+// stop at their next step.  The lanes of the wave run the loop together
+// (its ballots need them); lanes of no group pass member = false.  Returns
+// whether this lane found the occluder.  The group's answer, "some triangle
+// of the ray's reachable leaves passes the test", is the one-lane traversal's,
+// whichever triangle each finds first.
+template <class Stack>
+__device__ __forceinline__ bool traverse_group_anyhit(const TravNode *__restrict__ nodes,
+                                                      const TravTri *__restrict__ tris, int32_t root_ref,
+                                                      const float *root_box, uint32_t s, uint32_t kmask, uint32_t D,
+                                                      bool member, uint64_t gmask, float ox, float oy, float oz,
+                                                      float dx, float dy, float dz, float tmax, int32_t self,
+                                                      Stack &stk, uint32_t &boxes, uint32_t &ntris)
+{
+    bool found = false, active = member && root_ref != -1;
+    float ix = 0.0f, iy = 0.0f, iz = 0.0f;
+    uint32_t neg = 0;
+    if (active) {
+        ix = 1.0f / dx;
+        iy = 1.0f / dy;
+        iz = 1.0f / dz;
+        neg = (ix < 0.0f ? 1u : 0u) | (iy < 0.0f ? 2u : 0u) | (iz < 0.0f ? 4u : 0u);
+        float t0, t1;
+        box_test(root_box, ox, oy, oz, ix, iy, iz, t0, t1);
+        boxes += s == 0 ? 1u : 0u;
+        if (t0 > tmax || t1 < 0.0f) active = false;
+    }
+    TraceOut r;
+    r.tri = -1;
+    r.u = -1.0f;
+    r.v = -1.0f;
+    r.t = tmax;
+    r.boxes = 0;
+    r.tris = 0;
+    r.leaf_heap = 1;
+    r.pos = -1;
+    int32_t cur = root_ref;
+    uint32_t items = 0;
+    stk.sp = 0;
+    while (true) {
+        if (__ballot(found) & gmask) active = false;
+        if (__ballot(active) == 0) break;
+        if (active) {
+            const uint2 ptop = stk.peek();  // read while the fetch is in flight
+            float4 q0, q1, q2, q3f;
+            fetch_step(nodes, tris, cur, q0, q1, q2, q3f);
+            const bool at_node = cur >= 0;
+            const uint32_t depth = 31u - __clz((uint32_t)__float_as_int(at_node ? q3f.w : q2.w));
+            const bool top = depth < D;
+            bool mine = true;
+            if (depth == D || (top && !at_node)) {
+                mine = (items & kmask) == s;
+                ++items;
+            }
+            bool go = false;
+            if (mine) {
+                if (at_node) {
+                    const int4 q3 = as_int4(q3f);
+                    const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                    const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+                    float a0, a1, b0, b1;
+                    box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
+                    box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+                    boxes += (!top || s == 0) ? 2u : 0u;
+                    const int first_is_b = (neg >> q3.z) & 1u;
+                    const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
+                    const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+                    const int32_t fref = first_is_b ? q3.y : q3.x;
+                    const int32_t sref = first_is_b ? q3.x : q3.y;
+                    const bool f_ok = !(f0 > tmax || f1 < 0.0f);
+                    const bool s_ok = !(s1 < 0.0f) && !(s0 > tmax);
+                    if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
+                    if (f_ok || s_ok) {
+                        cur = f_ok ? fref : sref;
+                        go = true;
+                    }
+                } else if (leaf_test_first(r, tris, (uint32_t)cur, q0, q1, q2, ox, oy, oz, dx, dy, dz, self, true)) {
+                    found = true;
+                    active = false;
+                }
+            }
+            // a pop (an any-hit traversal culls nothing it pushed: every
+            // entry's T0 is at most tmax, the best t until the first hit)
+            if (active && !go) {
+                if (stk.sp == 0) {
+                    active = false;
+                } else {
+                    --stk.sp;
+                    cur = (int32_t)ptop.x;
+                }
+            }
+        }
+    }
+    ntris += r.tris;
+    return found;
+}
+
+// direct_term for the lanes of a wave together (all 64 lanes call it; lit:
+// the lane holds a primary hit to shade).  Per light, the lanes whose hit
+// casts a shadow ray (m of them) each lend their ray to a group of k =
+// 64 / m lanes (a power of two, at most 16; k = 1 above 32 rays), which
+// traces it with traverse_group_anyhit: the wave's idle lanes share the
+// shadow rays' subtrees instead of waiting for the longest ray.  Only
+// occlusion reaches the image, so the result equals direct_term's; the box
+// and triangle counts do not (counted frames do not come here).
+template <int FORM, bool KEEP, class Stack>
+__device__ __forceinline__ float shade_wave(const SceneView &sv, int32_t root_ref, const float *root_box,
+                                            const TraceOut &h, bool lit, float ox, float oy, float oz, float dx,
+                                            float dy, float dz, const float *lights, uint32_t nlights, float ambient,
+                                            float albedo, Stack &stk, uint32_t &boxes, uint32_t &ntris,
+                                            uint32_t &nshadow, uint8_t *occ_out)
+{
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+    float hx = 0.0f, hy = 0.0f, hz = 0.0f, rs = 0.0f;
+    if (lit) {
+        flat_normal(sv.tris[h.pos], nx, ny, nz);
+        mx = nx;
+        my = ny;
+        mz = nz;
+        if (((mx * dx + my * dy) + mz * dz) > 0.0f) {
+            mx = -mx;
+            my = -my;
+            mz = -mz;
+        }
+        hx = ox + h.t * dx;
+        hy = oy + h.t * dy;
+        hz = oz + h.t * dz;
+        rs = (nx * dx + ny * dy) + nz * dz;
+    }
+    float diffuse = 0.0f;
+    for (uint32_t l = 0; l < nlights; ++l) {
+        float lx = 0.0f, ly = 0.0f, lz = 0.0f, dist = 0.0f;
+        bool need = false;
+        if (lit) {
+            lx = hx - lights[3 * l + 0];
+            ly = hy - lights[3 * l + 1];
+            lz = hz - lights[3 * l + 2];
+            dist = sqrtf((lx * lx + ly * ly) + lz * lz);
+            lx = lx / dist;
+            ly = ly / dist;
+            lz = lz / dist;
+            const float ls = (nx * lx + ny * ly) + nz * lz;
+            need = (ls * rs) > 0.0f;
+            if (KEEP && !need) occ_out[l] = 2;
+        }
+        const uint64_t needm = __ballot(need);
+        if (needm == 0) continue;
+        const uint32_t m = (uint32_t)__popcll(needm);
+        bool occluded = false;
+        if (m > 32) {
+            if (need) {
+                const TraceOut sh = traverse_form<FORM, true>(sv.nodes, sv.tris, root_ref, root_box, hx, hy, hz, -lx,
+                                                              -ly, -lz, dist, h.tri, stk);
+                boxes += sh.boxes;
+                ntris += sh.tris;
+                occluded = sh.tri >= 0;
+            }
+        } else {
+            const uint32_t lk = m > 16 ? 1u : m > 8 ? 2u : m > 4 ? 3u : 4u;  // log2 k
+            const uint32_t k = 1u << lk, g = lane >> lk;
+            // the g-th lane casting a ray lends it to group g
+            uint64_t rest = needm;
+            uint32_t src = 0;
+            for (uint32_t j = 0; j < m; ++j) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(rest);
+                rest &= rest - 1ull;
+                if (j == g) src = b;
+            }
+            const float gx = __shfl(hx, src), gy = __shfl(hy, src), gz = __shfl(hz, src);
+            const float gdx = __shfl(-lx, src), gdy = __shfl(-ly, src), gdz = __shfl(-lz, src);
+            const float gdist = __shfl(dist, src);
+            const int32_t gself = __shfl(h.tri, src);
+            const bool member = g < m;
+            const uint64_t kbits = (1ull << k) - 1ull;
+            const uint64_t gmask = member ? kbits << (g * k) : 0ull;
+            const bool found = traverse_group_anyhit(sv.nodes, sv.tris, root_ref, root_box, lane & (k - 1u), k - 1u,
+                                                     kQuadDepth, member, gmask, gx, gy, gz, gdx, gdy, gdz, gdist,
+                                                     gself, stk, boxes, ntris);
+            const uint64_t F = __ballot(found);
+            const uint32_t rank = (uint32_t)__popcll(needm & below);
+            occluded = need && ((F >> (rank * k)) & kbits) != 0ull;
+        }
+        if (need) {
+            nshadow += 1;
+            if (!occluded) {
+                const float dd = -((mx * lx + my * ly) + mz * lz);
+                diffuse = diffuse + (dd > 0.0f ? dd : 0.0f);
+            }
+            if (KEEP) occ_out[l] = occluded ? 1 : 0;
+        }
+    }
+    return (ambient + diffuse) * albedo;
+}
+
 struct RenderParams {
     rt_camera cam;
     uint32_t width, height, nlights;
@@ -1006,6 +1207,7 @@ struct RenderParams {
     uint32_t *cost;             // per tile: max over its lanes of box + triangle tests (atomicMax)
     uint32_t nsplit;            // k_render_direct_quad: the first nsplit tiles of the order get quads
     uint32_t quad_depth;        // the heap depth of the subtrees a quad shares out
+    uint32_t group;             // k_render_direct, form 2, no counts: shade_wave (shadow rays on lane groups)
 };
 
 // A block renders a tile of 16 x BLOCK/16 pixels.  With two or more waves
@@ -1102,7 +1304,32 @@ __global__ __launch_bounds__(BLOCK) void k_render_direct(SceneView sv, RenderPar
     float root_box[6];
     load_root(sv.hdr, root_ref, root_box);
     uint32_t boxes = 0, ntris = 0, nshadow = 0;
-    if (valid) {
+    if (!COUNT && FORM == 2 && rp.group) {
+        // the shadow rays on lane groups: shade_wave needs every lane
+        const uint32_t pix = valid ? py * rp.width + px : 0u;
+        float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+        const float ox = rp.cam.pos[0], oy = rp.cam.pos[1], oz = rp.cam.pos[2];
+        TraceOut h;
+        h.tri = -1;
+        if (valid) {
+            primary_ray(rp.cam, px, py, (float)rp.width, (float)rp.height, 0.0f, 0.0f, dx, dy, dz);
+            h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz, dx, dy, dz,
+                                           10000000000000000.0f, -1, stk);
+            boxes += h.boxes;
+            ntris += h.tris;
+            if (KEEP) store_hit(&hits[pix], h);
+            if (KEEP && h.tri < 0)
+                for (uint32_t l = 0; l < rp.nlights; ++l) occ[(size_t)pix * rp.nlights + l] = 2;
+        }
+        const bool lit = valid && h.tri >= 0;
+        const float shaded = shade_wave<FORM, KEEP>(sv, root_ref, root_box, h, lit, ox, oy, oz, dx, dy, dz, rp.lights,
+                                                    rp.nlights, rp.ambient, rp.albedo, stk, boxes, ntris, nshadow,
+                                                    KEEP && lit ? &occ[(size_t)pix * rp.nlights] : nullptr);
+        if (valid) {
+            const float color = lit ? shaded : rp.env;
+            rgba[pix] = make_float4(color, color, color, 1.0f);
+        }
+    } else if (valid) {
         const uint32_t pix = py * rp.width + px;
         float dx, dy, dz;
         primary_ray(rp.cam, px, py, (float)rp.width, (float)rp.height, 0.0f, 0.0f, dx, dy, dz);
@@ -2190,6 +2417,9 @@ hipError_t launch_render_direct(const Batch &b, uint32_t scene, const rt_render_
     // hits or count keep one lane per ray
     const uint32_t quad_tiles = (keep || cnt || uses_bvh(b, scene)) ? 0u : (uint32_t)tune.quad_tiles;
     rp.quad_depth = kQuadDepth;
+    // shadow rays on lane groups (the shared-fetch form) need the LBVH's
+    // heap indices; counted frames keep one lane per ray
+    rp.group = (!cnt && !uses_bvh(b, scene)) ? 1u : 0u;
     uint32_t ntiles = 0;
     FrameBuffers::OrderSlot *os = nullptr;
 #define RT_L(B, K, F)                                                                               \

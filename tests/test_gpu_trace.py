@@ -196,6 +196,42 @@ def test_render_direct_forms_equal(ctx, rtcore, soup262k, form):
         assert st0[k] == st1[k], k
 
 
+@pytest.mark.parametrize("view", ["sparse", "dense"])
+def test_render_direct_shadow_groups_equal(ctx, rtcore, soup262k, view):
+    """Uncounted frames of the shared-fetch form trace each wave's shadow rays
+    on groups of lanes (up to 16 per ray, each searching every k-th subtree
+    below heap depth 8, stopping when one finds an occluder).  Frame, kept
+    hits and occlusion bytes equal the loop form's one lane per ray, in a view
+    with few shadow rays per wave (large groups) and in the 1080p soup frame
+    (about half the lanes of a wave cast one: groups of one or two)."""
+    s = soup262k[0]
+    if view == "sparse":
+        W, H = 401, 233
+        cam = rtcore.camera_look_at((1.8, 0.9, -0.6), (0.5, 0.5, 0.5), (0, 1, 0), 50.0, W / H)
+        lights = [(0.5, 2.0, -1.0), (1.5, 0.5, 0.5), (-0.5, 0.5, 0.2)]
+    else:
+        W, H = 1920, 1080
+        cam = rtcore.camera_look_at((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), (0, 1, 0), 60.0, W / H)
+        lights = [(0.5, 2.5, -1.0)]
+    saved = ctx.tuning()
+    out = []
+    try:
+        for form, block in ((0, 128), (2, 128), (2, 64)):
+            ctx.set_tuning(block, 20, 4, form)
+            for flags in (0, rtcore.FLAG_KEEP_HITS):
+                st = ctx.render_direct(s, cam, W, H, lights, flags=flags)
+                out.append((form, block, flags, st["shadow_rays"],
+                            ctx.frame_read(W, H, len(lights), hits=bool(flags))))
+    finally:
+        ctx.set_tuning(*saved)
+    ref = {flags: (n, arrs) for form, block, flags, n, arrs in out if form == 0}
+    assert ref[0][0] > 0
+    for form, block, flags, n, arrs in out:
+        assert n == ref[flags][0], (form, block, flags)
+        for x, y in zip(ref[flags][1], arrs):
+            assert (x is None and y is None) or np.array_equal(x, y), (form, block, flags)
+
+
 @pytest.mark.parametrize("block", [64, 128, 256])
 def test_render_direct_tile_order_keeps_the_frame(ctx, rtcore, soup262k, block):
     """rt_render_direct dispatches the tiles of a repeated frame heaviest
