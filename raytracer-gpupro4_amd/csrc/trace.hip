@@ -99,6 +99,28 @@ __device__ __forceinline__ void box_test(const float *bx, float ox, float oy, fl
     t1 = b;
 }
 
+// A child box of a node against a ray, for the node step: its entry t0 and
+// whether the child is live, i.e. box_test's (t0, t1) passes !(t0 > lim ||
+// t1 < 0).  box_test turns a missed box (entry after exit) into (-1, -1),
+// which fails that test; here the miss is one more term of it, so the
+// caller selects the near child's flag instead of its two distances.  Same
+// answer for every input, NaNs included (a comparison with NaN is false in
+// both).
+__device__ __forceinline__ void child_test(const float *bx, float ox, float oy, float oz, float ix, float iy,
+                                           float iz, float lim, float &t0, bool &live)
+{
+    const float dmaxx = (bx[3] - ox) * ix, dmaxy = (bx[4] - oy) * iy, dmaxz = (bx[5] - oz) * iz;
+    const float dminx = (bx[0] - ox) * ix, dminy = (bx[1] - oy) * iy, dminz = (bx[2] - oz) * iz;
+    float a = fminf(dminx, dmaxx);
+    float b = fmaxf(dminx, dmaxx);
+    a = fmaxf(a, fminf(dminy, dmaxy));
+    b = fminf(b, fmaxf(dminy, dmaxy));
+    a = fmaxf(a, fminf(dminz, dmaxz));
+    b = fminf(b, fmaxf(dminz, dmaxz));
+    t0 = a;
+    live = !(a > b) && !(a > lim) && !(b < 0.0f);
+}
+
 // One triangle test: getIntersection (Intersection.hlsl:47-64) and the
 // acceptance rule of LBVH_IntersectP (LBVH.hlsl:400-420).  Returns true when
 // the triangle becomes the current hit.
@@ -265,17 +287,17 @@ __device__ __forceinline__ TraceOut traverse(const TravNode *__restrict__ nodes,
             const int4 q3 = *reinterpret_cast<const int4 *>(&n->ref[0]);
             const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
             const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-            float a0, a1, b0, b1;
-            box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
-            box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+            float a0, b0;
+            bool a_ok, b_ok;
+            child_test(ba, ox, oy, oz, ix, iy, iz, r.t, a0, a_ok);
+            child_test(bb, ox, oy, oz, ix, iy, iz, r.t, b0, b_ok);
             r.boxes += 2;
             const int first_is_b = (neg >> q3.z) & 1u;  // the child on side dirIsNeg[axis]
-            const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
-            const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+            const float s0 = first_is_b ? a0 : b0;
             const int32_t fref = first_is_b ? q3.y : q3.x;
             const int32_t sref = first_is_b ? q3.x : q3.y;
-            const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-            const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+            const bool f_ok = first_is_b ? b_ok : a_ok;
+            const bool s_ok = first_is_b ? a_ok : b_ok;
             if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
             if (f_ok || s_ok) {
                 cur = f_ok ? fref : sref;
@@ -337,17 +359,17 @@ __device__ __forceinline__ TraceOut traverse_shared(const TravNode *__restrict__
             const int4 q3 = as_int4(q3f);
             const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
             const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-            float a0, a1, b0, b1;
-            box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
-            box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+            float a0, b0;
+            bool a_ok, b_ok;
+            child_test(ba, ox, oy, oz, ix, iy, iz, r.t, a0, a_ok);
+            child_test(bb, ox, oy, oz, ix, iy, iz, r.t, b0, b_ok);
             r.boxes += 2;
             const int first_is_b = (neg >> q3.z) & 1u;  // the child on side dirIsNeg[axis]
-            const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
-            const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+            const float s0 = first_is_b ? a0 : b0;
             const int32_t fref = first_is_b ? q3.y : q3.x;
             const int32_t sref = first_is_b ? q3.x : q3.y;
-            const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-            const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+            const bool f_ok = first_is_b ? b_ok : a_ok;
+            const bool s_ok = first_is_b ? a_ok : b_ok;
             // both children: the first now, the second pushed; one of them:
             // that one; neither: a pop
             if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
@@ -468,17 +490,17 @@ __device__ __forceinline__ TraceOut traverse_from(const TravNode *__restrict__ n
             const int4 q3 = as_int4(q3f);
             const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
             const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-            float a0, a1, b0, b1;
-            box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
-            box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+            float a0, b0;
+            bool a_ok, b_ok;
+            child_test(ba, ox, oy, oz, ix, iy, iz, r.t, a0, a_ok);
+            child_test(bb, ox, oy, oz, ix, iy, iz, r.t, b0, b_ok);
             r.boxes += 2;
             const int first_is_b = (neg >> q3.z) & 1u;
-            const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
-            const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+            const float s0 = first_is_b ? a0 : b0;
             const int32_t fref = first_is_b ? q3.y : q3.x;
             const int32_t sref = first_is_b ? q3.x : q3.y;
-            const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-            const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+            const bool f_ok = first_is_b ? b_ok : a_ok;
+            const bool s_ok = first_is_b ? a_ok : b_ok;
             if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
             if (f_ok || s_ok) {
                 cur = f_ok ? fref : sref;
@@ -573,17 +595,17 @@ __device__ __forceinline__ bool trav_step(Trav &T, const TravNode *__restrict__ 
         const int4 q3 = as_int4(q3f);
         const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
         const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-        float a0, a1, b0, b1;
-        box_test(ba, T.ox, T.oy, T.oz, T.ix, T.iy, T.iz, a0, a1);
-        box_test(bb, T.ox, T.oy, T.oz, T.ix, T.iy, T.iz, b0, b1);
+        float a0, b0;
+        bool a_ok, b_ok;
+        child_test(ba, T.ox, T.oy, T.oz, T.ix, T.iy, T.iz, r.t, a0, a_ok);
+        child_test(bb, T.ox, T.oy, T.oz, T.ix, T.iy, T.iz, r.t, b0, b_ok);
         r.boxes += 2;
         const int first_is_b = (T.neg >> q3.z) & 1u;
-        const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
-        const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+        const float s0 = first_is_b ? a0 : b0;
         const int32_t fref = first_is_b ? q3.y : q3.x;
         const int32_t sref = first_is_b ? q3.x : q3.y;
-        const bool f_ok = !(f0 > r.t || f1 < 0.0f);
-        const bool s_ok = !(s1 < 0.0f) && !(s0 > r.t);
+        const bool f_ok = first_is_b ? b_ok : a_ok;
+        const bool s_ok = first_is_b ? a_ok : b_ok;
         if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
         if (f_ok || s_ok) {
             T.cur = f_ok ? fref : sref;
@@ -849,18 +871,18 @@ __device__ __forceinline__ TraceOut traverse_quad(const TravNode *__restrict__ n
                 const int4 q3 = as_int4(q3f);
                 const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
                 const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-                float a0, a1, b0, b1;
-                box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
-                box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+                const float lim = top ? tmax : r.t;
+                float a0, b0;
+                bool a_ok, b_ok;
+                child_test(ba, ox, oy, oz, ix, iy, iz, lim, a0, a_ok);
+                child_test(bb, ox, oy, oz, ix, iy, iz, lim, b0, b_ok);
                 r.boxes += (!top || s == 0) ? 2u : 0u;
                 const int first_is_b = (neg >> q3.z) & 1u;
-                const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
-                const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+                const float s0 = first_is_b ? a0 : b0;
                 const int32_t fref = first_is_b ? q3.y : q3.x;
                 const int32_t sref = first_is_b ? q3.x : q3.y;
-                const float lim = top ? tmax : r.t;
-                const bool f_ok = !(f0 > lim || f1 < 0.0f);
-                const bool s_ok = !(s1 < 0.0f) && !(s0 > lim);
+                const bool f_ok = first_is_b ? b_ok : a_ok;
+                const bool s_ok = first_is_b ? a_ok : b_ok;
                 if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, top ? kNeverCulled : __float_as_uint(s0)));
                 if (f_ok || s_ok) {
                     cur = f_ok ? fref : sref;
@@ -1062,17 +1084,17 @@ __device__ __forceinline__ bool traverse_group_anyhit(const TravNode *__restrict
                     const int4 q3 = as_int4(q3f);
                     const float ba[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
                     const float bb[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-                    float a0, a1, b0, b1;
-                    box_test(ba, ox, oy, oz, ix, iy, iz, a0, a1);
-                    box_test(bb, ox, oy, oz, ix, iy, iz, b0, b1);
+                    float a0, b0;
+                    bool a_ok, b_ok;
+                    child_test(ba, ox, oy, oz, ix, iy, iz, tmax, a0, a_ok);
+                    child_test(bb, ox, oy, oz, ix, iy, iz, tmax, b0, b_ok);
                     boxes += (!top || s == 0) ? 2u : 0u;
                     const int first_is_b = (neg >> q3.z) & 1u;
-                    const float f0 = first_is_b ? b0 : a0, f1 = first_is_b ? b1 : a1;
-                    const float s0 = first_is_b ? a0 : b0, s1 = first_is_b ? a1 : b1;
+                    const float s0 = first_is_b ? a0 : b0;
                     const int32_t fref = first_is_b ? q3.y : q3.x;
                     const int32_t sref = first_is_b ? q3.x : q3.y;
-                    const bool f_ok = !(f0 > tmax || f1 < 0.0f);
-                    const bool s_ok = !(s1 < 0.0f) && !(s0 > tmax);
+                    const bool f_ok = first_is_b ? b_ok : a_ok;
+                    const bool s_ok = first_is_b ? a_ok : b_ok;
                     if (f_ok && s_ok) stk.push(make_uint2((uint32_t)sref, __float_as_uint(s0)));
                     if (f_ok || s_ok) {
                         cur = f_ok ? fref : sref;
