@@ -507,7 +507,8 @@ def workload_build(args, rtcore):
                    "tris": [r["tris"] for r in rows]},
                   higher_is_better=False, builds=rows,
                   roofline={"bound": "hbm", "achieved": last["sort_gb_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": last["sort_frac"], "traffic": None, "kernel": "k_onesweep",
+                            "frac": last["sort_frac"], "traffic": None,
+                            "kernel": "sort passes (k_sort_count + k_sort_scan + k_sort_scatter at this size)",
                             "bytes_per_launch": 16 * last["tris"], "tris": last["tris"]},
                   cpu_baseline=None)
 

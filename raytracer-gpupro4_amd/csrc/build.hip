@@ -96,6 +96,31 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
     return wbase + x - v;
 }
 
+// block_exclusive_scan over a block of NW waves.
+template <int NW>
+__device__ __forceinline__ uint32_t block_exclusive_scan_w(uint32_t v, uint32_t *tmp, uint32_t *total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        uint32_t t = tmp[w];
+        if (w < wave) wbase += t;
+        all += t;
+    }
+    __syncthreads();
+    if (total) *total = all;
+    return wbase + x - v;
+}
+
 __device__ __forceinline__ uint32_t part1by2(uint32_t n)
 {
     n &= 1023u;
@@ -410,31 +435,32 @@ __device__ __forceinline__ uint32_t sort_digit(uint32_t key, int32_t val, uint32
     return (key >> shift) & (kRadix - 1);
 }
 
-template <bool SCENE_PASS, int ITEMS, bool LOOKBACK>
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+template <bool SCENE_PASS, int ITEMS, bool LOOKBACK, int THREADS = kSortThreads>
+__global__ __launch_bounds__(THREADS) void k_sort_scatter(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin, uint32_t *__restrict__ kout,
     int32_t *__restrict__ vout, uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist,
     const SceneDesc *__restrict__ desc, uint32_t S, uint32_t *__restrict__ status,
     uint32_t *__restrict__ tile_counter, const uint32_t *__restrict__ offs, uint32_t ntiles)
 {
-    constexpr int kWaves = kSortThreads / 64;
+    static_assert(THREADS >= kRadix, "a thread per digit");
+    constexpr int kWaves = THREADS / 64;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kWaves][kRadix];
     __shared__ uint32_t s_excl[kRadix];
     __shared__ uint32_t s_gofs[kRadix];
     __shared__ uint32_t s_scan[kWaves];
-    __shared__ uint32_t s_keys[kSortThreads * ITEMS];
-    __shared__ int32_t s_vals[kSortThreads * ITEMS];
+    __shared__ uint32_t s_keys[THREADS * ITEMS];
+    __shared__ int32_t s_vals[THREADS * ITEMS];
     __shared__ uint32_t s_soff[SCENE_PASS ? kRadix : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (LOOKBACK && tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-    for (int j = tid; j < kWaves * kRadix; j += kSortThreads) (&s_wave[0][0])[j] = 0u;
+    for (int j = tid; j < kWaves * kRadix; j += THREADS) (&s_wave[0][0])[j] = 0u;
     if (SCENE_PASS)
-        for (int j = tid; j < kRadix; j += kSortThreads) s_soff[j] = j < (int)S ? desc[j].prim_off : 0xFFFFFFFFu;
+        for (int j = tid; j < kRadix; j += THREADS) s_soff[j] = j < (int)S ? desc[j].prim_off : 0xFFFFFFFFu;
     __syncthreads();
     const uint32_t tile = LOOKBACK ? s_tile : blockIdx.x;
-    const uint32_t base = tile * (uint32_t)(kSortThreads * ITEMS);
+    const uint32_t base = tile * (uint32_t)(THREADS * ITEMS);
 
     uint32_t key[ITEMS];
     int32_t val[ITEMS];
@@ -477,24 +503,29 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     }
     __syncthreads();
 
-    // Thread t owns digit t: per-wave exclusive offsets, tile count, tile scan.
+    // Thread t < 256 owns digit t: per-wave exclusive offsets, tile count,
+    // tile scan (the scans run on every thread; the others add nothing).
     const uint32_t dg = (uint32_t)tid;
+    const bool owner = tid < kRadix;
     uint32_t cnt = 0;
+    if (owner) {
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        const uint32_t c = s_wave[w][dg];
-        s_wave[w][dg] = cnt;
-        cnt += c;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_wave[w][dg];
+            s_wave[w][dg] = cnt;
+            cnt += c;
+        }
     }
     uint32_t tile_total;
-    const uint32_t excl = block_exclusive_scan(cnt, s_scan, &tile_total);
-    s_excl[dg] = excl;
+    const uint32_t excl = block_exclusive_scan_w<kWaves>(cnt, s_scan, &tile_total);
+    if (owner) s_excl[dg] = excl;
 
     if (LOOKBACK) {
         // Decoupled look-back over earlier tiles for this digit.
         uint32_t *my = status + (size_t)tile * kRadix + dg;
         uint32_t prefix = 0;
-        if (tile == 0) {
+        if (!owner) {
+        } else if (tile == 0) {
             __hip_atomic_store(my, kFlagInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(my, kFlagAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -532,12 +563,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
         }
         uint32_t gbase;
         if (SCENE_PASS) {
-            gbase = dg < S ? s_soff[dg] : 0u;
+            gbase = owner && dg < S ? s_soff[dg] : 0u;
         } else {
-            gbase = block_exclusive_scan(hist[dg], s_scan, nullptr);
+            gbase = block_exclusive_scan_w<kWaves>(owner ? hist[dg] : 0u, s_scan, nullptr);
         }
-        s_gofs[dg] = gbase + prefix;
-    } else {
+        if (owner) s_gofs[dg] = gbase + prefix;
+    } else if (owner) {
         s_gofs[dg] = offs[(size_t)dg * ntiles + tile];  // k_sort_scan: global base + earlier tiles
     }
     __syncthreads();
@@ -552,8 +583,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
         }
     }
     __syncthreads();
-    const uint32_t tile_n = (n - base) < (uint32_t)(kSortThreads * ITEMS) ? (n - base) : (uint32_t)(kSortThreads * ITEMS);
-    for (uint32_t i = tid; i < tile_n; i += kSortThreads) {
+    const uint32_t tile_n = (n - base) < (uint32_t)(THREADS * ITEMS) ? (n - base) : (uint32_t)(THREADS * ITEMS);
+    for (uint32_t i = tid; i < tile_n; i += THREADS) {
         const uint32_t k = s_keys[i];
         const int32_t v = s_vals[i];
         const uint32_t d = sort_digit<SCENE_PASS>(k, v, shift, S, s_soff);
@@ -1564,15 +1595,15 @@ hipError_t launch_build(Batch &b, hipStream_t st, hipEvent_t *ev, int cu_count)
             }
             uint32_t *status = b.status + (size_t)p * tiles * kRadix;
             uint32_t *tc = b.counters + kCntTiles + p;
-#define RT_SORT(SC, IT, LB, GRID)                                                                          \
-    hipLaunchKernelGGL((k_sort_scatter<SC, IT, LB>), dim3(GRID), dim3(kSortThreads), 0, st, b.keys[src],    \
+#define RT_SORT(SC, IT, LB, TH, GRID)                                                                      \
+    hipLaunchKernelGGL((k_sort_scatter<SC, IT, LB, TH>), dim3(GRID), dim3(TH), 0, st, b.keys[src],           \
                        b.vals[src], b.keys[dst], b.vals[dst], n, shift, hist, b.d_desc, S, status, tc, offs, lt)
             if (large) {
-                if (scene_pass) RT_SORT(true, kSortItemsLarge, false, lt);
-                else RT_SORT(false, kSortItemsLarge, false, lt);
+                if (scene_pass) RT_SORT(true, kSortItemsLarge, false, kSortThreads, lt);
+                else RT_SORT(false, kSortItemsLarge, false, kSortThreads, lt);
             } else {
-                if (scene_pass) RT_SORT(true, kSortItemsSmall, true, tiles);
-                else RT_SORT(false, kSortItemsSmall, true, tiles);
+                if (scene_pass) RT_SORT(true, kSortItemsSmall, true, kSortThreadsSmall, tiles);
+                else RT_SORT(false, kSortItemsSmall, true, kSortThreadsSmall, tiles);
             }
 #undef RT_SORT
         }

@@ -18,14 +18,19 @@ namespace rt {
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kCodePasses = 4;
-// Onesweep tiles of 256 threads x ITEMS keys: 8 keys per thread below
-// kSortLargeN keys (enough tiles to fill the chip), 32 above (long per-digit
-// runs, so the scatter writes coalesce).  Scratch is sized for the smaller tile.
+// Sort tiles: below kSortLargeN keys, look-back passes over tiles of 512
+// threads x 8 keys (a tile's wait is a chain of look-back round trips over
+// the earlier tiles, so fewer, larger tiles; each wave ranks 8 keys per lane
+// either way: 256 threads measured 51.4 against 49.7 us for four passes at
+// 262,144 keys, 1,024 threads 59.5); from kSortLargeN, count / scan /
+// scatter passes over tiles of 256 threads x 32 keys (long per-digit runs,
+// so the scatter writes coalesce).  Scratch is sized for the smaller tile.
 constexpr int kSortThreads = 256;
 constexpr int kSortItemsSmall = 8;
 constexpr int kSortItemsLarge = 32;
 constexpr uint32_t kSortLargeN = 1u << 20;
-constexpr int kSortTile = kSortThreads * kSortItemsSmall;
+constexpr int kSortThreadsSmall = 512;
+constexpr int kSortTile = kSortThreadsSmall * kSortItemsSmall;
 constexpr int kMaxBatchScenes = kRadix;  // the scene pass is one 8-bit digit
 
 // Bounds / Morton kernels: 256 threads x 4 triangles per block.
