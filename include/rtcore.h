@@ -424,7 +424,10 @@ int rt_tuning_set(rt_ctx *ctx, const rt_tuning *t);
 /* Renders all jobs: with RT_FLAG_PERSISTENT one launch whose blocks take tiles
  * of every job from one queue; without it one launch per job.  Jobs may name
  * scenes of different batches.  stats (optional, synchronises) reports the
- * kernel time and ray counts. */
+ * kernel time and ray counts; with RT_FLAG_COUNT also the box and triangle
+ * tests, and every ray then takes one lane (without it, the live paths and
+ * shadow rays of an LBVH scene's waves share their idle lanes: the same
+ * accumulations, untallied tests). */
 int rt_render_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
                    uint32_t flags, rt_frame_stats *stats);
 /* rt_render_jobs without any device allocation.  No counterpart in the

@@ -1149,14 +1149,15 @@ static int run_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32
     hipEvent_t *ev = timing ? &ctx->ev[6] : nullptr;
     if (!timing && ctx->profiling) ev = prof_events(ctx, 1, 2);
     if (ev) hipEventRecord(ev[0], st);
+    const bool count = (flags & RT_FLAG_COUNT) != 0;  // one lane per ray, tests counted
     if (persistent) {
         RT_HIP(ctx, rt::launch_render_jobs(ctx->jobs_dev, njobs, total, 0, total, true, ctx->tile_counter,
-                                           nullptr, ctx->stats, t, depth, ctx->cu_count, st));
+                                           nullptr, ctx->stats, t, depth, ctx->cu_count, st, count));
     } else {
         for (uint32_t k = 0; k < njobs; ++k)
             RT_HIP(ctx, rt::launch_render_jobs(ctx->jobs_dev, njobs, total, offs[k],
                                                offs[k + 1] - offs[k], false, ctx->tile_counter,
-                                               nullptr, ctx->stats, t, depth, ctx->cu_count, st));
+                                               nullptr, ctx->stats, t, depth, ctx->cu_count, st, count));
     }
     if (ev) hipEventRecord(ev[1], st);
     RT_HIP(ctx, hipEventRecord(ctx->jobs_ev, st));
@@ -1177,8 +1178,8 @@ static int run_jobs(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs, uint32
             stats->primary_rays = c[4];
             stats->shadow_rays = c[2];
             stats->bounce_rays = c[3];
-            stats->box_tests = c[0];
-            stats->tri_tests = c[1];
+            stats->box_tests = count ? c[0] : 0u;
+            stats->tri_tests = count ? c[1] : 0u;
         }
     }
     return RT_OK;
@@ -1222,13 +1223,15 @@ int rt_render_jobs_noalloc(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
     RT_HIP(ctx, rt::launch_zero_words(reinterpret_cast<uint32_t *>(d_stats), 16, st));
     if (stats) RT_HIP(ctx, hipEventRecord(ctx->ev[6], st));
     const uint32_t *offs = ctx->noalloc_offs.data();
+    const bool count = (flags & RT_FLAG_COUNT) != 0;  // one lane per ray, tests counted
     if (flags & RT_FLAG_PERSISTENT) {
         RT_HIP(ctx, rt::launch_render_jobs(d_jobs, njobs, total, 0, total, true, d_counter, nullptr, d_stats,
-                                           ctx->tune, depth, ctx->cu_count, st));
+                                           ctx->tune, depth, ctx->cu_count, st, count));
     } else {
         for (uint32_t k = 0; k < njobs; ++k)
             RT_HIP(ctx, rt::launch_render_jobs(d_jobs, njobs, total, offs[k], offs[k + 1] - offs[k], false,
-                                               d_counter, nullptr, d_stats, ctx->tune, depth, ctx->cu_count, st));
+                                               d_counter, nullptr, d_stats, ctx->tune, depth, ctx->cu_count, st,
+                                               count));
     }
     if (stats) {
         RT_HIP(ctx, hipEventRecord(ctx->ev[7], st));
@@ -1240,8 +1243,8 @@ int rt_render_jobs_noalloc(rt_ctx *ctx, const rt_job_desc *jobs, uint32_t njobs,
         stats->primary_rays = c[4];
         stats->shadow_rays = c[2];
         stats->bounce_rays = c[3];
-        stats->box_tests = c[0];
-        stats->tri_tests = c[1];
+        stats->box_tests = count ? c[0] : 0u;
+        stats->tri_tests = count ? c[1] : 0u;
     }
     return RT_OK;
 }

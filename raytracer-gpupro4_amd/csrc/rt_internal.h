@@ -329,12 +329,14 @@ size_t render_job_bytes();
 // order (device, optional): the tile at each queue position of the job.
 void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_desc &d,
                      const rt_tuning &t, uint32_t tile_off, uint32_t *tiles, const uint32_t *order);
+// count: one lane per ray, box and triangle tests counted into stats; without
+// it, LBVH jobs run with lane groups and only ray counts are kept.
 hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total_tiles,
                               uint32_t first_tile, uint32_t tile_count, bool persistent,
                               uint32_t *tile_counter, const uint32_t *frame_counter,
                               unsigned long long *stats, const rt_tuning &tune, uint32_t max_depth,
                               int cu_count,
-                              hipStream_t st);
+                              hipStream_t st, bool count = false);
 hipError_t launch_bump_counter(uint32_t *c, hipStream_t st);
 hipError_t launch_zero_words(uint32_t *p, uint32_t n, hipStream_t st);
 // Times the launch shapes of one block size on the jobs (persistent

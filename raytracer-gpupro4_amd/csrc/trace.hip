@@ -2295,7 +2295,8 @@ struct RenderJob {
     const uint32_t *order;  // tile at each position of the job's queue; null: row-major
     rt_camera cam;
     uint32_t width, height, tiles_x, tile_off;
-    uint32_t local_base, mode, spp, max_bounces, nlights, pad1;
+    uint32_t local_base, mode, spp, max_bounces, nlights;
+    uint32_t lbvh;  // the nodes carry the LBVH's heap indices (lane groups may split a ray)
     float albedo, sky, ambient, pad;
     uint64_t seed;
     uint32_t frame, pad2;
@@ -2303,11 +2304,11 @@ struct RenderJob {
 };
 
 template <int BLOCK, int KLDS, bool PERSISTENT, int FORM>
-__global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restrict__ jobs,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_render_jobs(const RenderJob *__restrict__ jobs,
                                                        uint32_t njobs, uint32_t total_tiles,
                                                        uint32_t tile_base, uint32_t *tile_counter,
                                                        const uint32_t *frame_counter,
-                                                       unsigned long long *stats)
+                                                       unsigned long long *stats, uint32_t count)
 {
     __shared__ uint2 lds[KLDS * BLOCK];
     __shared__ uint32_t s_tile;
@@ -2338,7 +2339,135 @@ __global__ __launch_bounds__(BLOCK) void k_render_jobs(const RenderJob *__restri
         Tile<BLOCK>::offset(threadIdx.x, ox_, oy_);
         const uint32_t px = (lt % J.tiles_x) * TW + ox_;
         const uint32_t py = (lt / J.tiles_x) * TH + oy_;
-        if (px < J.width && py < J.height) {
+        const bool valid = px < J.width && py < J.height;
+        if (!count && J.lbvh) {
+            // Uncounted LBVH jobs: the lanes of a wave together.  Direct
+            // samples shade with shade_wave (shadow rays on lane groups); GI
+            // samples send each bounce's rays out to lane groups once at most
+            // 32 paths of the wave live (the group takes the ray from its
+            // path's lane, traces it as a quad does, and the path's lane
+            // takes the closest hit back).  Same images; box and triangle
+            // counts are not kept.
+            const SceneView sv{J.nodes, J.tris, J.hdr};
+            int32_t root_ref;
+            float root_box[6];
+            load_root(J.hdr, root_ref, root_box);
+            const uint32_t pix = valid ? py * J.width + px : 0u;
+            const float fw = (float)J.width, fh = (float)J.height;
+            const uint32_t lane = __lane_id();
+            const uint64_t below = (1ull << lane) - 1ull;
+            uint32_t gboxes = 0, gtris = 0;  // not reported
+            float sum = 0.0f;
+            for (uint32_t s = 0; s < J.spp; ++s) {
+                Pcg32 rng{0, 0};
+                float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+                float ox = J.cam.pos[0], oy = J.cam.pos[1], oz = J.cam.pos[2];
+                if (valid) {
+                    rng = pcg_seed(sample_seed(J.seed, J.frame + frame_add, s), pix);
+                    const float jx = 2.0f * pcg_unit(rng) - 1.0f;
+                    const float jy = 2.0f * pcg_unit(rng) - 1.0f;
+                    primary_ray(J.cam, px, py, fw, fh, jx, jy, dx, dy, dz);
+                }
+                if (J.mode == 0) {
+                    TraceOut h;
+                    h.tri = -1;
+                    if (valid) {
+                        h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz, dx, dy,
+                                                       dz, 10000000000000000.0f, -1, stk);
+                        nprimary += 1;
+                    }
+                    const bool lit = valid && h.tri >= 0;
+                    const float shaded = shade_wave<FORM, false>(sv, root_ref, root_box, h, lit, ox, oy, oz, dx, dy,
+                                                                 dz, J.lights, J.nlights, J.ambient, J.albedo, stk,
+                                                                 gboxes, gtris, nshadow, nullptr);
+                    if (valid) sum += lit ? shaded : J.sky;
+                    continue;
+                }
+                float thr = 1.0f, L = 0.0f;
+                int32_t self = -1;
+                bool alive = valid;
+                for (uint32_t b = 0; b <= J.max_bounces; ++b) {
+                    const uint64_t live = __ballot(alive);
+                    if (live == 0) break;
+                    const uint32_t m = (uint32_t)__popcll(live);
+                    TraceOut h;
+                    h.tri = -1;
+                    h.t = 0.0f;
+                    h.pos = 0;
+                    if (m > 32) {
+                        if (alive)
+                            h = traverse_form<FORM, false>(sv.nodes, sv.tris, root_ref, root_box, ox, oy, oz, dx, dy,
+                                                           dz, 10000000000000000.0f, self, stk);
+                    } else {
+                        const uint32_t lk = m > 16 ? 1u : m > 8 ? 2u : m > 4 ? 3u : 4u;  // log2 k
+                        const uint32_t k = 1u << lk, g = lane >> lk;
+                        uint64_t rest = live;
+                        uint32_t src = 0;
+                        for (uint32_t j = 0; j < m; ++j) {
+                            const uint32_t bit = (uint32_t)__builtin_ctzll(rest);
+                            rest &= rest - 1ull;
+                            if (j == g) src = bit;
+                        }
+                        const float gox = __shfl(ox, src), goy = __shfl(oy, src), goz = __shfl(oz, src);
+                        const float gdx = __shfl(dx, src), gdy = __shfl(dy, src), gdz = __shfl(dz, src);
+                        const int32_t gself = __shfl(self, src);
+                        TraceOut gh;
+                        gh.tri = -1;
+                        gh.t = 0.0f;
+                        gh.pos = 0;
+                        if (g < m) {  // the whole group together
+                            uint32_t rank;
+                            gh = traverse_quad<false>(sv.nodes, sv.tris, root_ref, root_box, lane & (k - 1u),
+                                                      kQuadDepth, gox, goy, goz, gdx, gdy, gdz,
+                                                      10000000000000000.0f, gself, stk, rank, k - 1u);
+                            group_closest(gh, rank, k);
+                        }
+                        // the path of rank r takes its hit from group r's first lane
+                        const uint32_t from = alive ? (uint32_t)__popcll(live & below) * k : 0u;
+                        const int32_t tri = __shfl(gh.tri, from);
+                        const float t = __shfl(gh.t, from);
+                        const int32_t pos = __shfl(gh.pos, from);
+                        h.tri = tri;
+                        h.t = t;
+                        h.pos = pos;
+                    }
+                    if (alive) {
+                        if (b == 0) nprimary += 1;
+                        else nbounce += 1;
+                        if (h.tri < 0) {
+                            L = L + thr * J.sky;
+                            alive = false;
+                        } else if (b == J.max_bounces) {
+                            alive = false;
+                        } else {
+                            float nx, ny, nz;
+                            flat_normal(sv.tris[h.pos], nx, ny, nz);
+                            if (((nx * dx + ny * dy) + nz * dz) > 0.0f) {
+                                nx = -nx;
+                                ny = -ny;
+                                nz = -nz;
+                            }
+                            ox = ox + h.t * dx;
+                            oy = oy + h.t * dy;
+                            oz = oz + h.t * dz;
+                            const float r1 = pcg_unit(rng), r2 = pcg_unit(rng);
+                            cosine_dir(nx, ny, nz, r1, r2, dx, dy, dz);
+                            thr = thr * J.albedo;
+                            self = h.tri;
+                        }
+                    }
+                }
+                if (valid) sum += L;
+            }
+            if (valid) {
+                float4 a = J.accum[pix];
+                a.x += sum;
+                a.y += sum;
+                a.z += sum;
+                a.w += (float)J.spp;
+                J.accum[pix] = a;
+            }
+        } else if (valid) {
             const SceneView sv{J.nodes, J.tris, J.hdr};
             int32_t root_ref;
             float root_box[6];
@@ -2718,6 +2847,7 @@ void fill_render_job(void *dst, const Batch &b, uint32_t scene, const rt_job_des
     j->tiles_x = (d.width + TW - 1) / TW;
     j->tile_off = tile_off;
     j->mode = d.mode == RT_MODE_GI ? 1u : 0u;
+    j->lbvh = uses_bvh(b, scene) ? 0u : 1u;
     const uint32_t all = j->tiles_x * ((d.height + TH - 1) / TH);
     j->local_base = d.tile_count ? (d.tile_begin < all ? d.tile_begin : all) : 0u;
     j->spp = d.spp;
@@ -2754,7 +2884,7 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
                               uint32_t first_tile, uint32_t tile_count, bool persistent,
                               uint32_t *tile_counter, const uint32_t *frame_counter,
                               unsigned long long *stats, const rt_tuning &tune, uint32_t max_depth,
-                              int cu_count, hipStream_t st)
+                              int cu_count, hipStream_t st, bool count)
 {
     if (total_tiles == 0 || njobs == 0 || (!persistent && tile_count == 0)) return hipSuccess;
     const RenderJob *jobs = static_cast<const RenderJob *>(d_jobs);
@@ -2765,11 +2895,11 @@ hipError_t launch_render_jobs(const void *d_jobs, uint32_t njobs, uint32_t total
             uint32_t blocks = (uint32_t)cu_count * (uint32_t)(tune.blocks_per_cu > 0 ? tune.blocks_per_cu : 4); \
             if (blocks > total_tiles) blocks = total_tiles;                                         \
             hipLaunchKernelGGL((k_render_jobs<B, K, true, F>), dim3(blocks), dim3(B), 0, st, jobs, njobs, \
-                               total_tiles, 0u, tile_counter, frame_counter, stats);                \
+                               total_tiles, 0u, tile_counter, frame_counter, stats, count ? 1u : 0u); \
         } else {                                                                                    \
             hipLaunchKernelGGL((k_render_jobs<B, K, false, F>), dim3(tile_count), dim3(B), 0, st, jobs, \
                                njobs, first_tile + tile_count, first_tile, tile_counter,            \
-                               frame_counter, stats);                                               \
+                               frame_counter, stats, count ? 1u : 0u);                              \
         }                                                                                           \
     }
     RT_TUNED(RT_L)

@@ -510,19 +510,20 @@ class Context:
         j.tile_begin, j.tile_count, j.tile_order = tile_begin, tile_count, tile_order
         return j
 
-    def render_jobs(self, jobs, persistent=True, want_stats=True):
+    def render_jobs(self, jobs, persistent=True, want_stats=True, flags=0):
+        """flags: FLAG_COUNT keeps one lane per ray and fills box_tests / tri_tests."""
         arr = (JobDesc * len(jobs))(*jobs)
         st = FrameStats()
-        flags = FLAG_PERSISTENT if persistent else 0
+        flags = flags | (FLAG_PERSISTENT if persistent else 0)
         self.check(self.L.rt_render_jobs(self.h, arr, len(jobs), flags,
                                          C.byref(st) if want_stats else None), "rt_render_jobs")
         return st.as_dict() if want_stats else None
 
-    def render_jobs_noalloc(self, jobs, persistent=True, want_stats=True):
+    def render_jobs_noalloc(self, jobs, persistent=True, want_stats=True, flags=0):
         """rt_render_jobs_noalloc: the same render with no device allocation."""
         arr = (JobDesc * len(jobs))(*jobs)
         st = FrameStats()
-        flags = FLAG_PERSISTENT if persistent else 0
+        flags = flags | (FLAG_PERSISTENT if persistent else 0)
         self.check(self.L.rt_render_jobs_noalloc(self.h, arr, len(jobs), flags,
                                                  C.byref(st) if want_stats else None), "rt_render_jobs_noalloc")
         return st.as_dict() if want_stats else None

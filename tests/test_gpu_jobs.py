@@ -114,6 +114,62 @@ def test_jobs_mix_and_launch_mode_invariance(ctx, rtcore):
         ctx.scene_destroy(h)
 
 
+def test_jobs_lane_groups_equal_one_lane(ctx, rtcore):
+    """Uncounted job launches of LBVH scenes share a wave's live GI paths out
+    to lane groups once at most 32 remain (and direct samples' shadow rays);
+    counted launches (RT_FLAG_COUNT) keep one lane per ray.  Both give the same
+    accumulations bit for bit, and the same ray counts, for GI at 4 and 12
+    bounces and for direct jobs, in LBVH scenes and a scene traced through its
+    SAH BVH (which has no heap indices and keeps one lane)."""
+    a = rtcore.gen_object_mix(20000, seed=3)
+    b = rtcore.gen_triangle_soup(40000, 0.01, seed=4)
+    sa = ctx.add_scene(*a, 18)
+    sb = ctx.add_scene(*b, 18)
+    ctx.build(sa)
+    ctx.build(sb)
+    sbvh = ctx.add_scene(*a, 18)
+    ctx.build(sbvh)
+    ctx.bvh_build(sbvh, 2)
+    ctx.scene_select(sbvh, rtcore.STRUCTURE_BVH)
+    specs = [(sa, a, (96, 64), rtcore.MODE_GI, 2, 4), (sb, b, (80, 48), rtcore.MODE_GI, 1, 12),
+             (sa, a, (64, 64), rtcore.MODE_DIRECT, 2, 0), (sb, b, (72, 40), rtcore.MODE_DIRECT, 1, 0),
+             (sbvh, a, (48, 40), rtcore.MODE_GI, 2, 4)]
+    bufs, jobs = [], []
+    for scene, (xyz, _), (w, h), mode, spp, nb in specs:
+        ctr = xyz.mean(0)
+        cam = rtcore.camera_look_at(tuple(ctr + [0.4, 1.5, -4.0]), tuple(ctr), (0, 1, 0), 55.0, w / h)
+        acc = ctx.alloc(16 * w * h)
+        bufs.append((acc, w, h))
+        jobs.append(ctx.job(scene, cam, w, h, acc, mode, spp, nb, 0.7, 1.0, 0.1,
+                            lights=[tuple(ctr + [0, 3, 0]), tuple(ctr + [2, 1, -3])], seed=7, frame=1))
+
+    def run(flags, persistent):
+        for acc, w, h in bufs:
+            ctx.zero(acc, 16 * w * h)
+        st = ctx.render_jobs(jobs, persistent=persistent, flags=flags)
+        return st, [ctx.read_accum(acc, w, h) for acc, w, h in bufs]
+
+    saved = ctx.tuning()
+    try:
+        for shape in [(128, 20, 2), (128, 22, 0), (64, 20, 1)]:
+            ctx.set_tuning(shape[0], shape[1], 4, shape[2])
+            st1, one = run(rtcore.FLAG_COUNT, False)
+            assert st1["box_tests"] > 0
+            for persistent in (False, True):
+                st2, grp = run(0, persistent)
+                assert st2["box_tests"] == 0  # counts come with RT_FLAG_COUNT only
+                for k in ("primary_rays", "shadow_rays", "bounce_rays"):
+                    assert st1[k] == st2[k], (shape, persistent, k)
+                for k in range(len(bufs)):
+                    assert np.array_equal(one[k], grp[k]), (shape, persistent, k)
+    finally:
+        ctx.set_tuning(*saved)
+    for acc, _, _ in bufs:
+        ctx.free(acc)
+    for h in (sa, sb, sbvh):
+        ctx.scene_destroy(h)
+
+
 def test_trace_parity_under_every_launch_shape(ctx, oracle, rtcore):
     xyz, idx = rtcore.gen_triangle_soup(100000, 0.006, seed=3)
     s = ctx.add_scene(xyz, idx, 20)
