@@ -22,7 +22,7 @@ constexpr int kCodePasses = 4;
 // threads x 8 keys (a tile's wait is a chain of look-back round trips over
 // the earlier tiles, so fewer, larger tiles; each wave ranks 8 keys per lane
 // either way: 256 threads measured 51.4 against 49.7 us for four passes at
-// 262,144 keys, 1,024 threads 59.5); from kSortLargeN, count / scan /
+// 262,144 keys, 768 threads 53.8, 1,024 threads 59.5); from kSortLargeN, count / scan /
 // scatter passes over tiles of 256 threads x 32 keys (long per-digit runs,
 // so the scatter writes coalesce).  Scratch is sized for the smaller tile.
 constexpr int kSortThreads = 256;
