@@ -1693,6 +1693,7 @@ struct WhittedParams {
     rt_material defmat;
     const uint32_t *order;  // tile of each block, heaviest first; null: block index order
     uint32_t *cost;         // per tile: max over its lanes of box + triangle tests (atomicMax)
+    uint32_t groups;        // one-pass kernel: live paths on lane groups (uncounted LBVH frames)
 };
 
 // A live path between passes: origin, direction, reflective factor, the id
@@ -1962,9 +1963,12 @@ __device__ __forceinline__ WRay whitted_primary(const WhittedParams &wp, uint32_
     return r;
 }
 
-// One-pass form: every pass of a pixel in one lane.
+// One-pass form: every pass of a pixel in one lane; for uncounted frames
+// over the LBVH, a wave's live paths are shared out to lane groups before
+// each pass (whitted_group_passes), as the split form's remaining passes do.
+// Same frame either way.
 template <int BLOCK, int KLDS, int FORM>
-__global__ __launch_bounds__(BLOCK) void k_render_whitted(SceneView sv, ShadeView sh, WhittedParams wp,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_render_whitted(SceneView sv, ShadeView sh, WhittedParams wp,
                                                           float4 *__restrict__ rgba,
                                                           rt_hit24 *__restrict__ hits,
                                                           uint8_t *__restrict__ occ, int keep, int count,
@@ -1986,7 +1990,24 @@ __global__ __launch_bounds__(BLOCK) void k_render_whitted(SceneView sv, ShadeVie
     load_root(sv.hdr, root_ref, root_box);
     WCount cnt;
     uint32_t nbounce = 0;
-    if (px < wp.width && py < wp.height) {
+    const bool valid = px < wp.width && py < wp.height;
+    if (wp.groups) {  // the wave's live paths on lane groups
+        WRay r;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        bool white = false, alive = valid;
+        if (valid) r = whitted_primary(wp, px, py);
+        if (keep && alive) {  // pass 0 keeps its hit: one lane per pixel
+            alive = whitted_pass<FORM>(sv, sh, wp, root_ref, root_box, r, acc, white, &hits[r.pix], &occ[r.pix], stk,
+                                       cnt);
+            if (!alive)
+                rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f)
+                                    : make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        whitted_group_passes<FORM>(sv, sh, wp, root_ref, root_box, keep ? 1u : 0u, r, acc, white, alive, rgba, stk,
+                                   cnt, nbounce);
+        if (alive)
+            rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : make_float4(acc[0], acc[1], acc[2], acc[3]);
+    } else if (valid) {
         WRay r = whitted_primary(wp, px, py);
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // g_uAccumulation
         bool white = false;
@@ -2138,6 +2159,78 @@ __global__ __launch_bounds__(BLOCK) void k_whitted_rest(SceneView sv, ShadeView 
     count_add(stats, 3, nbounce);
 }
 
+// The passes first_pass .. wp.bounces of the paths a wave holds, one per lane
+// (alive: this lane holds a live path r with accumulator acc): one lane per
+// path while more than 32 of the wave's paths live, then the m live paths
+// shared out to aligned groups of 64 / m lanes (a power of two, at most 16)
+// that take the path's state by shuffles and run the pass as a quad does.
+// Every lane of the wave calls it.  A path's colour is written when it ends;
+// on return alive says the path went through the last pass (the caller
+// writes it).
+template <int FORM, class Stack>
+__device__ __forceinline__ void whitted_group_passes(const SceneView &sv, const ShadeView &sh, const WhittedParams &wp,
+                                                     int32_t root_ref, const float *root_box, uint32_t first_pass,
+                                                     WRay &r, float *acc, bool &white, bool &alive,
+                                                     float4 *__restrict__ rgba, Stack &stk, WCount &cnt,
+                                                     uint32_t &nbounce)
+{
+    const uint32_t lane = __lane_id();
+    for (uint32_t pass = first_pass; pass <= wp.bounces; ++pass) {
+        const uint64_t live = __ballot(alive);
+        if (live == 0) break;
+        const uint32_t m = (uint32_t)__popcll(live);
+        if (m > 32) {  // one lane per path
+            if (alive) {
+                nbounce += pass > 0 ? 1u : 0u;
+                alive = whitted_pass<FORM>(sv, sh, wp, root_ref, root_box, r, acc, white, nullptr, nullptr, stk,
+                                           cnt);
+                if (!alive)
+                    rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f)
+                                        : make_float4(acc[0], acc[1], acc[2], acc[3]);
+            }
+            continue;
+        }
+        const uint32_t lk = m > 16 ? 1u : m > 8 ? 2u : m > 4 ? 3u : 4u;  // log2 k
+        const uint32_t k = 1u << lk, g = lane >> lk, s = lane & (k - 1u);
+        // group g takes the g-th live path
+        uint64_t rest = live;
+        uint32_t src = 0;
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(rest);
+            rest &= rest - 1ull;
+            if (j == g) src = b;
+        }
+        r.ox = __shfl(r.ox, src);
+        r.oy = __shfl(r.oy, src);
+        r.oz = __shfl(r.oz, src);
+        r.dx = __shfl(r.dx, src);
+        r.dy = __shfl(r.dy, src);
+        r.dz = __shfl(r.dz, src);
+        r.fx = __shfl(r.fx, src);
+        r.fy = __shfl(r.fy, src);
+        r.fz = __shfl(r.fz, src);
+        r.self = __shfl(r.self, src);
+        r.pix = __shfl(r.pix, src);
+        r.rng.state = ((uint64_t)(uint32_t)__shfl((uint32_t)(r.rng.state >> 32), src) << 32) |
+                      (uint32_t)__shfl((uint32_t)r.rng.state, src);
+        r.rng.inc = ((uint64_t)(uint32_t)__shfl((uint32_t)(r.rng.inc >> 32), src) << 32) |
+                    (uint32_t)__shfl((uint32_t)r.rng.inc, src);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = __shfl(acc[c], src);
+        white = __shfl(white ? 1 : 0, src) != 0;
+        bool cont = false;
+        if (g < m) {  // the whole group together
+            if (s == 0 && pass > 0) nbounce += 1;
+            cont = whitted_pass<FORM, true>(sv, sh, wp, root_ref, root_box, r, acc, white, nullptr, nullptr,
+                                            stk, cnt, s, k);
+            if (!cont && s == 0)
+                rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f)
+                                    : make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        alive = g < m && s == 0 && cont;  // the group's first lane owns the path from here
+    }
+}
+
 // k_whitted_rest for uncounted frames with the live paths of a wave on lane
 // groups: the wave takes 64 paths, one per lane; before each pass the m
 // paths still alive are shared out to aligned groups of k = 64 / m lanes (a
@@ -2179,60 +2272,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             acc[2] = a.z;
             acc[3] = a.w;
         }
-        for (uint32_t pass = 1; pass <= wp.bounces; ++pass) {
-            const uint64_t live = __ballot(alive);
-            if (live == 0) break;
-            const uint32_t m = (uint32_t)__popcll(live);
-            if (m > 32) {  // one lane per path
-                if (alive) {
-                    nbounce += 1;
-                    alive = whitted_pass<FORM>(sv, sh, wp, root_ref, root_box, r, acc, white, nullptr, nullptr, stk,
-                                               cnt);
-                    if (!alive)
-                        rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f)
-                                            : make_float4(acc[0], acc[1], acc[2], acc[3]);
-                }
-                continue;
-            }
-            const uint32_t lk = m > 16 ? 1u : m > 8 ? 2u : m > 4 ? 3u : 4u;  // log2 k
-            const uint32_t k = 1u << lk, g = lane >> lk, s = lane & (k - 1u);
-            // group g takes the g-th live path
-            uint64_t rest = live;
-            uint32_t src = 0;
-            for (uint32_t j = 0; j < m; ++j) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(rest);
-                rest &= rest - 1ull;
-                if (j == g) src = b;
-            }
-            r.ox = __shfl(r.ox, src);
-            r.oy = __shfl(r.oy, src);
-            r.oz = __shfl(r.oz, src);
-            r.dx = __shfl(r.dx, src);
-            r.dy = __shfl(r.dy, src);
-            r.dz = __shfl(r.dz, src);
-            r.fx = __shfl(r.fx, src);
-            r.fy = __shfl(r.fy, src);
-            r.fz = __shfl(r.fz, src);
-            r.self = __shfl(r.self, src);
-            r.pix = __shfl(r.pix, src);
-            r.rng.state = ((uint64_t)(uint32_t)__shfl((uint32_t)(r.rng.state >> 32), src) << 32) |
-                          (uint32_t)__shfl((uint32_t)r.rng.state, src);
-            r.rng.inc = ((uint64_t)(uint32_t)__shfl((uint32_t)(r.rng.inc >> 32), src) << 32) |
-                        (uint32_t)__shfl((uint32_t)r.rng.inc, src);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc[c] = __shfl(acc[c], src);
-            white = __shfl(white ? 1 : 0, src) != 0;
-            bool cont = false;
-            if (g < m) {  // the whole group together
-                if (s == 0) nbounce += 1;
-                cont = whitted_pass<FORM, true>(sv, sh, wp, root_ref, root_box, r, acc, white, nullptr, nullptr,
-                                                stk, cnt, s, k);
-                if (!cont && s == 0)
-                    rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f)
-                                        : make_float4(acc[0], acc[1], acc[2], acc[3]);
-            }
-            alive = g < m && s == 0 && cont;  // the group's first lane owns the path from here
-        }
+        whitted_group_passes<FORM>(sv, sh, wp, root_ref, root_box, 1u, r, acc, white, alive, rgba, stk, cnt,
+                                   nbounce);
         if (alive)  // through the last pass
             rgba[r.pix] = white ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : make_float4(acc[0], acc[1], acc[2], acc[3]);
     }
@@ -2776,6 +2817,7 @@ hipError_t launch_render_whitted(const Batch &b, uint32_t scene, const rt_whitte
     // the split form's remaining passes with live paths on lane groups: the
     // same heap-index requirement, and uncounted frames only
     const bool groups = !cnt && !uses_bvh(b, scene);
+    wp.groups = groups ? 1u : 0u;
     const uint64_t px = (uint64_t)d.width * d.height;
     RayRec *q[2] = {static_cast<RayRec *>(queue), static_cast<RayRec *>(queue) + px};
     if (!one_pass) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, qcount, d.bounces + 1);

@@ -137,24 +137,24 @@ def test_whitted_forms_identical_at_1080p(ctx, rtcore, room):
 
 @pytest.mark.parametrize("bounces", [5, 12])
 def test_whitted_groups_and_quads_identical_uncounted(ctx, rtcore, room, bounces):
-    """Frames rendered without RT_FLAG_COUNT, where the split form's remaining
-    passes share each wave's live paths out to lane groups (up to 16 lanes a
-    path once at most 32 remain) and the split-quads form gives every path
-    four lanes, equal the wavefront form's one lane per ray, and so do their
-    ray counts, at 1080p and on a small image."""
+    """Frames rendered without RT_FLAG_COUNT, where the one-pass form and the
+    split form's remaining passes share each wave's live paths out to lane
+    groups (up to 16 lanes a path once at most 32 remain) and the split-quads
+    form gives every path four lanes, equal the wavefront form's one lane per
+    ray, and so do their ray counts, at 1080p and on a small image."""
     s = room[0]
     for w, h in ((1920, 1080), (333, 197)):
         cam = rtcore.camera_look_at((0.4, -1.7, 0.9), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 55.0, w / h)
         kw = dict(bounces=bounces, env=(0.2, 0.3, 0.4), seed=11)
         out = {}
-        for form in ("wavefront", "split", "split-quads"):
+        for form in ("wavefront", "one-pass", "split", "split-quads"):
             st = ctx.render_whitted(s, cam, w, h, (0.8, -0.6, 1.5), switches=SHADOWS | PHONG | GLOSS | FORM_BITS[form],
                                     **kw)
             out[form] = (st, ctx.frame_read(w, h, 1)[0])
         a, fa = out["wavefront"]
         print(f"{w}x{h}, {bounces} bounces: " + ", ".join(f"{k} {v[0]['trace_ms']:.3f} ms" for k, v in out.items()),
               flush=True)
-        for form in ("split", "split-quads"):
+        for form in ("one-pass", "split", "split-quads"):
             b, fb = out[form]
             assert np.array_equal(fa, fb), form
             for k in ("primary_rays", "shadow_rays", "bounce_rays"):
